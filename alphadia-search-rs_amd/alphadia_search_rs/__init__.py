@@ -1,0 +1,496 @@
+"""alphadia_search_rs on MI355X (gfx950).
+
+Drop-in for the candidate-selection path of the PyO3 module
+``alphadia_search_rs`` (src/lib.rs:74-93): the same class names, method
+names, argument order, dtypes and return values for ``DIAData``,
+``SpecLibFlat``, ``SelectionParameters``, ``PeakGroupSelection`` and
+``CandidateCollection``, plus the SIMD-backend registry functions.  The work
+runs in libadsr.so: host builders in C++ and the selection pipeline as
+gfx950 HIP kernels.  There is no CPU selection backend.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _native as N
+from ._native import check, ptr
+
+__all__ = [
+    "DIAData",
+    "SpecLibFlat",
+    "SelectionParameters",
+    "PeakGroupSelection",
+    "CandidateCollection",
+    "get_optimal_simd_backend",
+    "set_simd_backend",
+    "clear_simd_backend",
+    "get_current_simd_backend",
+]
+
+
+# ---------------------------------------------------------------------------
+# argument conversion (PyReadonlyArray1<T> semantics: exact dtype, 1-D)
+# ---------------------------------------------------------------------------
+def _arr(name: str, a, dtype, ndim: int = 1) -> np.ndarray:
+    if not isinstance(a, np.ndarray):
+        raise TypeError(
+            f"argument '{name}': '{type(a).__name__}' object cannot be "
+            f"converted to 'PyArray<{np.dtype(dtype).name}, {ndim}>'"
+        )
+    if a.dtype != np.dtype(dtype) or a.ndim != ndim:
+        raise TypeError(
+            f"argument '{name}': type mismatch: expected ndarray of "
+            f"{np.dtype(dtype).name} with {ndim} dimension(s), got "
+            f"{a.dtype.name} with {a.ndim}"
+        )
+    return np.ascontiguousarray(a)
+
+
+def _same_len(names, arrays):
+    n = len(arrays[0])
+    for nm, a in zip(names, arrays):
+        if len(a) != n:
+            raise ValueError(
+                f"array '{nm}' has length {len(a)}, expected {n} "
+                f"(same as '{names[0]}')"
+            )
+    return n
+
+
+# ---------------------------------------------------------------------------
+# SIMD backend registry (src/lib.rs:53-72, src/simd/mod.rs:80-134)
+# ---------------------------------------------------------------------------
+def _name(fn) -> str:
+    buf = C.create_string_buffer(64)
+    check(fn(buf, 64))
+    return buf.value.decode()
+
+
+def get_optimal_simd_backend() -> str:
+    return _name(N.lib.adsr_backend_optimal)
+
+
+def get_current_simd_backend() -> str:
+    return _name(N.lib.adsr_backend_current)
+
+
+def set_simd_backend(backend_name: str) -> None:
+    check(N.lib.adsr_backend_set(str(backend_name).encode()))
+
+
+def clear_simd_backend() -> None:
+    N.lib.adsr_backend_clear()
+
+
+# ---------------------------------------------------------------------------
+# DIAData (src/dia_data/mod.rs:15-135)
+# ---------------------------------------------------------------------------
+class DIAData:
+    """Per-quadrupole-window 1-ppm binned peak index, resident in HBM."""
+
+    def __init__(self):
+        self._h = None
+        self._n_obs = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib.adsr_dia_free(h)
+            self._h = None
+
+    @staticmethod
+    def from_arrays(
+        spectrum_delta_scan_idx,
+        isolation_lower_mz,
+        isolation_upper_mz,
+        spectrum_peak_start_idx,
+        spectrum_peak_stop_idx,
+        spectrum_cycle_idx,
+        spectrum_rt,
+        peak_mz,
+        peak_intensity,
+        cycle,
+    ) -> "DIAData":
+        """src/dia_data/mod.rs:43-72."""
+        d = _arr("spectrum_delta_scan_idx", spectrum_delta_scan_idx, np.int64)
+        lo = _arr("isolation_lower_mz", isolation_lower_mz, np.float32)
+        hi = _arr("isolation_upper_mz", isolation_upper_mz, np.float32)
+        ps = _arr("spectrum_peak_start_idx", spectrum_peak_start_idx, np.int64)
+        pe = _arr("spectrum_peak_stop_idx", spectrum_peak_stop_idx, np.int64)
+        cy = _arr("spectrum_cycle_idx", spectrum_cycle_idx, np.int64)
+        rt = _arr("spectrum_rt", spectrum_rt, np.float32)
+        mz = _arr("peak_mz", peak_mz, np.float32)
+        it = _arr("peak_intensity", peak_intensity, np.float32)
+        cyc = _arr("cycle", cycle, np.float32, ndim=4)
+        names = ["spectrum_delta_scan_idx", "isolation_lower_mz",
+                 "isolation_upper_mz", "spectrum_peak_start_idx",
+                 "spectrum_peak_stop_idx", "spectrum_cycle_idx", "spectrum_rt"]
+        ns = _same_len(names, [d, lo, hi, ps, pe, cy, rt])
+        npk = _same_len(["peak_mz", "peak_intensity"], [mz, it])
+        shape = (C.c_size_t * 4)(*cyc.shape)
+        out = C.c_void_p()
+        check(N.lib.adsr_dia_create(ptr(d), ptr(lo), ptr(hi), ptr(ps), ptr(pe),
+                                    ptr(cy), ptr(rt), ns, ptr(mz), ptr(it), npk,
+                                    ptr(cyc), shape, C.byref(out)))
+        obj = DIAData()
+        obj._h = out.value
+        obj._n_obs = N.lib.adsr_dia_num_observations(obj._h)
+        if N.device_available():
+            check(N.lib.adsr_dia_upload(obj._h))
+        return obj
+
+    @property
+    def num_observations(self) -> int:
+        return int(self._n_obs)
+
+    def get_valid_observations(self, precursor_mz: float) -> list:
+        if not self._h:
+            return []
+        n = C.c_size_t()
+        buf = np.zeros(max(self._n_obs, 1), dtype=np.uint64)
+        check(N.lib.adsr_dia_valid_observations(self._h, float(precursor_mz),
+                                                ptr(buf), len(buf), C.byref(n)))
+        return [int(x) for x in buf[: n.value]]
+
+    def memory_footprint_bytes(self) -> int:
+        if not self._h:
+            return 24  # an empty Vec<QuadrupoleObservation>
+        return int(N.lib.adsr_dia_memory_footprint_bytes(self._h))
+
+    def memory_footprint_mb(self) -> float:
+        return self.memory_footprint_bytes() / (1024.0 * 1024.0)
+
+    @property
+    def has_mobility(self) -> bool:
+        return False
+
+    @property
+    def has_ms1(self) -> bool:
+        return False
+
+    @property
+    def mobility_values(self) -> list:
+        return [1e-6, 0.0]
+
+    @property
+    def rt_values(self) -> np.ndarray:
+        if not self._h:
+            return np.zeros(0, dtype=np.float32)
+        n = N.lib.adsr_dia_rt_values(self._h, None, 0)
+        out = np.empty(n, dtype=np.float32)
+        N.lib.adsr_dia_rt_values(self._h, ptr(out), n)
+        return out
+
+    @property
+    def cycle(self) -> np.ndarray:
+        if not self._h:
+            return np.zeros((0, 0, 0, 0), dtype=np.float32)
+        shape = (C.c_size_t * 4)()
+        N.lib.adsr_dia_cycle_shape(self._h, shape)
+        out = np.empty(tuple(shape), dtype=np.float32)
+        N.lib.adsr_dia_cycle_copy(self._h, ptr(out))
+        return out
+
+    # -- internal views used by tests and tools ------------------------------
+    def _rt_index(self):
+        n = N.lib.adsr_dia_rt_index(self._h, None, 0, None)
+        out = np.empty(n, dtype=np.float32)
+        dt = C.c_float()
+        N.lib.adsr_dia_rt_index(self._h, ptr(out), n, C.byref(dt))
+        return out, dt.value
+
+    def _observation(self, i: int) -> dict:
+        win = np.zeros(2, dtype=np.float32)
+        nc = C.c_uint64()
+        nss = C.c_size_t()
+        ne = C.c_size_t()
+        check(N.lib.adsr_dia_observation(self._h, i, ptr(win), C.byref(nc),
+                                         None, C.byref(nss), None, None,
+                                         C.byref(ne)))
+        ss = np.empty(nss.value, dtype=np.uint32)
+        ci = np.empty(ne.value, dtype=np.uint16)
+        it = np.empty(ne.value, dtype=np.float32)
+        check(N.lib.adsr_dia_observation(self._h, i, None, None, ptr(ss),
+                                         C.byref(nss), ptr(ci), ptr(it),
+                                         C.byref(ne)))
+        return {"isolation_window": win, "num_cycles": nc.value,
+                "slice_starts": ss, "cycle_indices": ci, "intensities": it}
+
+
+# ---------------------------------------------------------------------------
+# SpecLibFlat (src/speclib_flat/mod.rs:6-199)
+# ---------------------------------------------------------------------------
+class SpecLibFlat:
+    """Flat spectral library, precursor arrays sorted by precursor_idx."""
+
+    def __init__(self):
+        self._h = None
+        self._np = 0
+        self._nf = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib.adsr_lib_free(h)
+            self._h = None
+
+    @staticmethod
+    def from_arrays(
+        precursor_idx,
+        precursor_mz_library,
+        precursor_mz,
+        precursor_rt_library,
+        precursor_rt,
+        precursor_naa,
+        flat_frag_start_idx,
+        flat_frag_stop_idx,
+        fragment_mz_library,
+        fragment_mz,
+        fragment_intensity,
+        fragment_cardinality,
+        fragment_charge,
+        fragment_loss_type,
+        fragment_number,
+        fragment_position,
+        fragment_type,
+    ) -> "SpecLibFlat":
+        """src/speclib_flat/mod.rs:98-188."""
+        u64, f32, u8 = np.uint64, np.float32, np.uint8
+        p = [
+            _arr("precursor_idx", precursor_idx, u64),
+            _arr("precursor_mz_library", precursor_mz_library, f32),
+            _arr("precursor_mz", precursor_mz, f32),
+            _arr("precursor_rt_library", precursor_rt_library, f32),
+            _arr("precursor_rt", precursor_rt, f32),
+            _arr("precursor_naa", precursor_naa, u8),
+            _arr("flat_frag_start_idx", flat_frag_start_idx, u64),
+            _arr("flat_frag_stop_idx", flat_frag_stop_idx, u64),
+        ]
+        f = [
+            _arr("fragment_mz_library", fragment_mz_library, f32),
+            _arr("fragment_mz", fragment_mz, f32),
+            _arr("fragment_intensity", fragment_intensity, f32),
+            _arr("fragment_cardinality", fragment_cardinality, u8),
+            _arr("fragment_charge", fragment_charge, u8),
+            _arr("fragment_loss_type", fragment_loss_type, u8),
+            _arr("fragment_number", fragment_number, u8),
+            _arr("fragment_position", fragment_position, u8),
+            _arr("fragment_type", fragment_type, u8),
+        ]
+        n_prec = _same_len(["precursor_idx"] * len(p), p)
+        n_frag = _same_len(["fragment_mz_library"] * len(f), f)
+        out = C.c_void_p()
+        check(N.lib.adsr_lib_create(n_prec, *[ptr(a) for a in p], n_frag,
+                                    *[ptr(a) for a in f], C.byref(out)))
+        obj = SpecLibFlat()
+        obj._h = out.value
+        obj._np = n_prec
+        obj._nf = n_frag
+        if N.device_available():
+            check(N.lib.adsr_lib_upload(obj._h))
+        return obj
+
+    @property
+    def num_precursors(self) -> int:
+        return int(self._np)
+
+    @property
+    def num_fragments(self) -> int:
+        return int(self._nf)
+
+
+# ---------------------------------------------------------------------------
+# SelectionParameters (src/peak_group_selection/parameters.rs:4-75)
+# ---------------------------------------------------------------------------
+def _extract_f32(key, v):
+    if isinstance(v, (bool, np.bool_)) or not isinstance(
+            v, (int, float, np.integer, np.floating)):
+        raise TypeError(f"'{type(v).__name__}' object cannot be converted to "
+                        f"'PyFloat' (key '{key}')")
+    return float(np.float32(v))
+
+
+def _extract_usize(key, v):
+    if isinstance(v, (float, np.floating)) or not isinstance(
+            v, (int, np.integer)):
+        raise TypeError(f"'{type(v).__name__}' object cannot be interpreted "
+                        f"as an integer (key '{key}')")
+    v = int(v)
+    if v < 0 or v >= 2 ** 64:
+        raise OverflowError(f"can't convert {v} to usize (key '{key}')")
+    return v
+
+
+class SelectionParameters:
+    """Parameters of PeakGroupSelection with the reference defaults.
+
+    Attributes are read-only; ``update(dict)`` changes them.  Two extension
+    keys are accepted by ``update``: ``rt_tolerance_per_precursor`` (float32
+    array in library order, i.e. sorted by precursor_idx) and
+    ``sigma_per_window`` (float32 array, one Gaussian sigma in cycles per
+    quadrupole observation).  Without them results equal the reference.
+    """
+
+    _F32 = ("fwhm_rt", "mass_tolerance", "rt_tolerance")
+    _USIZE = ("kernel_size", "peak_length", "candidate_count",
+              "top_k_fragments")
+
+    def __init__(self):
+        object.__setattr__(self, "_v", {
+            "fwhm_rt": 3.0,
+            "kernel_size": 20,
+            "peak_length": 5,
+            "mass_tolerance": 7.0,
+            "rt_tolerance": 200.0,
+            "candidate_count": 3,
+            "top_k_fragments": 12,
+            "rt_tolerance_per_precursor": None,
+            "sigma_per_window": None,
+        })
+
+    def __getattr__(self, name):
+        v = object.__getattribute__(self, "_v")
+        if name in v:
+            return v[name]
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        raise AttributeError(
+            f"attribute '{name}' of 'SelectionParameters' objects is not "
+            "writable")
+
+    def update(self, config: dict) -> None:
+        if not isinstance(config, dict):
+            raise TypeError("update() expects a dict")
+        v = object.__getattribute__(self, "_v")
+        new = dict(v)
+        for k in self._F32:
+            if k in config:
+                new[k] = _extract_f32(k, config[k])
+        for k in self._USIZE:
+            if k in config:
+                new[k] = _extract_usize(k, config[k])
+        for k in ("rt_tolerance_per_precursor", "sigma_per_window"):
+            if k in config:
+                a = config[k]
+                new[k] = None if a is None else np.ascontiguousarray(
+                    np.asarray(a, dtype=np.float32).reshape(-1))
+        v.update(new)
+
+    def _c(self) -> N.SelectionParamsC:
+        v = object.__getattribute__(self, "_v")
+        c = N.SelectionParamsC()
+        c.fwhm_rt = v["fwhm_rt"]
+        c.kernel_size = v["kernel_size"]
+        c.peak_length = v["peak_length"]
+        c.mass_tolerance = v["mass_tolerance"]
+        c.rt_tolerance = v["rt_tolerance"]
+        c.candidate_count = v["candidate_count"]
+        c.top_k_fragments = v["top_k_fragments"]
+        rp = v["rt_tolerance_per_precursor"]
+        sw = v["sigma_per_window"]
+        c.rt_tolerance_per_precursor = ptr(rp) if rp is not None else None
+        c.sigma_per_window = ptr(sw) if sw is not None else None
+        c.n_sigma_per_window = 0 if sw is None else len(sw)
+        return c
+
+    def __repr__(self):
+        v = object.__getattribute__(self, "_v")
+        keys = self._F32 + self._USIZE
+        return "SelectionParameters(" + ", ".join(
+            f"{k}={v[k]}" for k in keys) + ")"
+
+
+# ---------------------------------------------------------------------------
+# CandidateCollection (src/candidate/entry.rs:41-193)
+# ---------------------------------------------------------------------------
+class CandidateCollection:
+    def __init__(self):
+        self._h = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib.adsr_candidates_free(h)
+            self._h = None
+
+    @staticmethod
+    def _wrap(handle) -> "CandidateCollection":
+        c = CandidateCollection()
+        c._h = handle
+        return c
+
+    def len(self) -> int:
+        return int(N.lib.adsr_candidates_len(self._h)) if self._h else 0
+
+    def __len__(self) -> int:
+        return self.len()
+
+    def is_empty(self) -> bool:
+        return self.len() == 0
+
+    @staticmethod
+    def from_arrays(precursor_idxs, ranks, scores, scan_center, scan_start,
+                    scan_stop, cycle_center, cycle_start, cycle_stop
+                    ) -> "CandidateCollection":
+        cols = [precursor_idxs, ranks, scores, scan_center, scan_start,
+                scan_stop, cycle_center, cycle_start, cycle_stop]
+        arrs = [np.ascontiguousarray(np.asarray(c, dtype=np.float32 if i == 2
+                                                else np.uint64))
+                for i, c in enumerate(cols)]
+        n = len(arrs[0])
+        if not all(len(a) == n for a in arrs):
+            raise ValueError("All input arrays must have the same length")
+        out = C.c_void_p()
+        check(N.lib.adsr_candidates_from_arrays(n, *[ptr(a) for a in arrs],
+                                                C.byref(out)))
+        return CandidateCollection._wrap(out.value)
+
+    def to_arrays(self):
+        """(precursor_idx, rank, score, scan_center, scan_start, scan_stop,
+        cycle_center, cycle_start, cycle_stop): u64 columns, score f32."""
+        n = self.len()
+        cols = [np.empty(n, dtype=np.float32 if i == 2 else np.uint64)
+                for i in range(9)]
+        if self._h:
+            check(N.lib.adsr_candidates_export(self._h, *[ptr(a) for a in cols]))
+        return tuple(cols)
+
+
+# ---------------------------------------------------------------------------
+# PeakGroupSelection (src/peak_group_selection/mod.rs:66-128)
+# ---------------------------------------------------------------------------
+class PeakGroupSelection:
+    def __init__(self, params: SelectionParameters):
+        if not isinstance(params, SelectionParameters):
+            raise TypeError("PeakGroupSelection expects SelectionParameters")
+        # the reference keeps its own copy of the parameters (mod.rs:75-85)
+        p = SelectionParameters()
+        object.__getattribute__(p, "_v").update(
+            dict(object.__getattribute__(params, "_v")))
+        self._params = p
+
+    def search(self, dia_data: DIAData, lib: SpecLibFlat) -> CandidateCollection:
+        """Runs on the gfx950 device; raises if no device is available."""
+        if not isinstance(dia_data, DIAData) or not isinstance(lib, SpecLibFlat):
+            raise TypeError("search(dia_data: DIAData, lib: SpecLibFlat)")
+        if not dia_data._h or not lib._h:
+            return CandidateCollection()
+        c = self._params._c()
+        out = C.c_void_p()
+        t0 = time.perf_counter()
+        check(N.lib.adsr_select(dia_data._h, lib._h, C.byref(c), C.byref(out)))
+        dt = time.perf_counter() - t0
+        cand = CandidateCollection._wrap(out.value)
+        if os.environ.get("ADSR_QUIET", "") in ("", "0"):
+            # mod.rs:120-125
+            rate = lib.num_precursors / dt if dt > 0 else float("inf")
+            print(f"Precursors per second: {rate}")
+            print(f"Found {cand.len()} candidates")
+            sys.stdout.flush()
+        return cand

@@ -1,0 +1,1845 @@
+// adsr_device.hip -- gfx950 kernels and launches for
+// PeakGroupSelection.search (src/peak_group_selection/mod.rs:87-195).
+//
+// Pipeline (one stream; captured in a hipGraph for repeated searches):
+//   plan    one thread per precursor: RT cycle window
+//           (src/rt_index/mod.rs:89-138), valid quadrupole windows
+//           (src/dia_data/mod.rs:79-87) and the fragment filter
+//           (src/speclib_flat/mod.rs:240-345).
+//   bins    one thread per (precursor, selected fragment): the 1-ppm bins of
+//           [mz - d, mz + d] (src/mz_index/mod.rs:95-127) via the f32-bit
+//           bucket table, then the CSR entry range of every valid window.
+//   select  one wavefront per precursor, persistent over an atomic queue:
+//           coalesced scan of the CSR entries into an LDS-resident XIC tile
+//           (src/quadrupole_observation/mod.rs:70-107), sparse 21-tap
+//           convolution on the compacted active cycles
+//           (src/convolution/scalar.rs:5-56), exact ln and ordered weighted
+//           accumulation (src/score/scalar.rs:4-25), strict 5-point maxima
+//           and top-k (src/peak_group_selection/mod.rs:23-64,167-192).
+//   compact exclusive scan of per-precursor counts, order-preserving
+//           scatter of the candidate rows (library order, as the rayon
+//           collect at mod.rs:101-119).
+//
+// Every f32 operation follows the reference's evaluation order with one
+// rounding per operation: the library is built with -ffp-contract=off.
+// Skipping work is only done where the skipped terms are exact zeros.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "adsr_internal.hpp"
+#include "adsr_logf.hpp"
+
+namespace adsr {
+
+static const LogfEntry kHostLogfTab[16] = ADSR_LOGF_TABLE;
+__constant__ LogfEntry kDevLogfTab[16] = ADSR_LOGF_TABLE;
+
+constexpr int kWave = 64;
+constexpr int kMaxSel = 64;      // selected fragments per precursor
+constexpr int kObsMax = 4;       // valid windows per precursor
+constexpr int kHalfMax = 15;     // kernel half width (kernel_size <= 30)
+constexpr int kLCapMax = 4096;   // cycles per precursor window
+constexpr int kStashFrag = 48;   // per-thread LDS stash in the plan kernel
+constexpr size_t kMaxDynLds = 160 * 1024 - 512;  // bytes per block
+
+enum StatusBits : uint32_t {
+  kStTooManyFragments = 1u,
+  kStTooManyWindows = 2u,
+  kStWindowTooLong = 4u,
+};
+
+#define HIP_TRY(expr)                                                   \
+  do {                                                                  \
+    hipError_t _e = (expr);                                             \
+    if (_e != hipSuccess)                                               \
+      return set_error(ADSR_ERR_DEVICE, "%s failed: %s", #expr,         \
+                       hipGetErrorString(_e));                          \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// kernel argument blocks
+// ---------------------------------------------------------------------------
+struct KDia {
+  const float *grid;
+  const uint32_t *bucket;
+  uint32_t n_grid, base_key;
+  const float *rt;
+  uint32_t n_rt;
+  float delta_t;
+  uint32_t n_obs;
+  const float *iso_lo, *iso_hi;
+  const uint32_t *ss;  // n_obs * (n_grid + 1)
+  const uint64_t *entry_base;
+  const uint16_t *cycles;
+  const float *intens;
+};
+
+struct KLib {
+  uint32_t n_prec;
+  const float *p_mz, *p_rt;
+  const uint64_t *f_start, *f_stop;
+  const float *f_mz, *f_int;
+  const uint8_t *f_type, *f_num;
+};
+
+struct KParams {
+  float mass_tol;
+  float rt_tol;
+  const float *rt_tol_pp;  // nullable
+  float pad_cycles;        // (kernel_size as f32)
+  uint32_t top_k;
+  uint32_t cand_count;
+  uint32_t half;
+  uint32_t klen;
+  uint32_t per_window;
+  const float *kernels;  // (1 + n_obs*per_window) * klen
+};
+
+struct KPlan {
+  uint4 *rec;         // {c0, c1, nsel | nobs << 16, kernel index}
+  uint16_t *obs;      // [row][kObsMax]
+  uint16_t *sel;      // [row][KS] local fragment index, output order
+  float *sel_mz;      // [row][KS]
+  float *sel_w;       // [row][KS]
+  uint2 *rng;         // [row][kObsMax][KS] CSR entry range, per window
+  uint32_t *b0;       // [row][kObsMax][KS] first bin
+  uint32_t ks;
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+// Rust 1.88 slice::binary_search_by over f32 (src/rt_index/mod.rs:254-265):
+// base moves to mid whenever v[mid] <= key; Ok(base) on equality, else the
+// insertion point.
+__device__ __forceinline__ uint32_t rust_bsearch_f32(const float *v,
+                                                     uint32_t n, float key) {
+  if (n == 0) return 0;
+  uint32_t size = n, base = 0;
+  while (size > 1) {
+    uint32_t half = size >> 1, mid = base + half;
+    if (!(v[mid] > key)) base = mid;
+    size -= half;
+  }
+  const float b = v[base];
+  if (b == key) return base;
+  return base + (b < key ? 1u : 0u);
+}
+
+__device__ __forceinline__ uint32_t grid_lb(const KDia &d, float x) {
+  const float *m = d.grid;
+  const uint32_t n = d.n_grid;
+  if (!(x > m[0])) return 0;
+  if (x > m[n - 1]) return n;
+  uint32_t b = d.bucket[(__float_as_uint(x) >> kBucketShift) - d.base_key];
+  while (b < n && m[b] < x) ++b;
+  return b;
+}
+
+__device__ __forceinline__ uint32_t grid_ub(const KDia &d, float x) {
+  const float *m = d.grid;
+  const uint32_t n = d.n_grid;
+  if (!(x >= m[0])) return 0;
+  if (x >= m[n - 1]) return n;
+  uint32_t b = d.bucket[(__float_as_uint(x) >> kBucketShift) - d.base_key];
+  while (b < n && m[b] <= x) ++b;
+  return b;
+}
+
+// src/mz_index/mod.rs:95-127 -> [start, end)
+__device__ __forceinline__ void mz_range(const KDia &d, float lo, float hi,
+                                         uint32_t &start, uint32_t &end) {
+  start = grid_lb(d, lo);
+  end = start;
+  if (start < d.n_grid) end = max(start, grid_ub(d, hi));
+}
+
+__device__ __forceinline__ uint64_t ballot64(bool p) {
+  return __ballot(p);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// inclusive wave scan of u32
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, kWave);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// plan: one thread per precursor row
+// ---------------------------------------------------------------------------
+// Fragment filter and order of src/speclib_flat/mod.rs:240-345 with the
+// Rust 1.88 select_nth_unstable_by behaviour:
+//   k >= kept        all kept fragments, kept order
+//   k == 0           none
+//   k == kept - 1    drop the first fragment of minimum intensity; the last
+//                    kept fragment takes its place (max_index + swap)
+//   otherwise        stable order by intensity descending, first k
+//                    (insertion sort for <= 16 elements; longer slices use
+//                    introselect, whose order at equal intensities is not
+//                    reproduced)
+// followed by the stable sort by fragment m/z.  Accessor A gives the
+// intensity of fragment j (-inf when the fragment is filtered out) and mz.
+template <class A>
+__device__ uint32_t select_fragments(const A &acc, uint32_t n, uint32_t top_k,
+                                     uint16_t *sel, float *sel_mz,
+                                     float *sel_w, const float *f_int,
+                                     uint32_t ks, uint32_t &status) {
+  uint32_t kept = 0;
+  for (uint32_t j = 0; j < n; ++j) kept += acc.I(j) > 0.0f ? 1u : 0u;
+  const uint32_t k = min(top_k, kept);
+  if (k == 0) return 0;
+  if (k > (uint32_t)kMaxSel || k > ks) {
+    status |= kStTooManyFragments;
+    return 0;
+  }
+  const int mode = (k >= kept) ? 0 : (k == kept - 1 ? 1 : 2);
+  uint32_t drop = 0xffffffffu, last = 0, drop_pos = 0;
+  if (mode == 1) {
+    float mn = 0.0f;
+    uint32_t pos = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+      const float v = acc.I(j);
+      if (!(v > 0.0f)) continue;
+      if (drop == 0xffffffffu || v < mn) {
+        mn = v;
+        drop = j;
+        drop_pos = pos;
+      }
+      last = j;
+      ++pos;
+    }
+  }
+  // pre-sort key of every fragment (its position before the m/z sort);
+  // 0xffffffff marks fragments that are not selected
+  uint32_t pos_kept = 0;
+  for (uint32_t j = 0; j < n; ++j) {
+    const float v = acc.I(j);
+    uint32_t key = 0xffffffffu;
+    if (v > 0.0f) {
+      if (mode == 2) {
+        uint32_t r = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+          const float u = acc.I(i);
+          if (u > v || (u == v && i < j)) ++r;
+        }
+        key = r < k ? r : 0xffffffffu;
+      } else if (mode == 1) {
+        key = (j == drop) ? 0xffffffffu
+                          : ((j == last && last != drop) ? drop_pos : pos_kept);
+      } else {
+        key = pos_kept;
+      }
+      ++pos_kept;
+    }
+    acc.set_key(j, key);
+  }
+  for (uint32_t j = 0; j < n; ++j) {
+    const uint32_t kj = acc.key(j);
+    if (kj == 0xffffffffu) continue;
+    const float mj = acc.mz(j);
+    uint32_t pos = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t ki = acc.key(i);
+      if (i == j || ki == 0xffffffffu) continue;
+      const float mi = acc.mz(i);
+      if (mi < mj || (!(mi > mj) && !(mi < mj) && ki < kj)) ++pos;
+    }
+    sel[pos] = (uint16_t)j;
+    sel_mz[pos] = mj;
+    sel_w[pos] = f_int[j];
+  }
+  return k;
+}
+
+// Fragments beyond the LDS stash: the same filter evaluated from global
+// memory, with the pre-sort keys in a per-row global scratch.
+struct GlobalFragAcc {
+  const float *mz_, *int_;
+  const uint8_t *type_, *num_;
+  uint32_t *key_;
+  __device__ float I(uint32_t j) const {
+    const float v = int_[j];
+    const bool keep = v > 0.0f && !(type_[j] == 121 && num_[j] == 1);
+    return keep ? v : -__builtin_inff();
+  }
+  __device__ float mz(uint32_t j) const { return mz_[j]; }
+  __device__ uint32_t key(uint32_t j) const { return key_[j]; }
+  __device__ void set_key(uint32_t j, uint32_t k) const { key_[j] = k; }
+};
+
+struct StashFragAcc {
+  const float *mz_, *I_;  // LDS, stride kWave (one column per lane)
+  uint32_t *key_;
+  __device__ float I(uint32_t j) const { return I_[j * kWave]; }
+  __device__ float mz(uint32_t j) const { return mz_[j * kWave]; }
+  __device__ uint32_t key(uint32_t j) const { return key_[j * kWave]; }
+  __device__ void set_key(uint32_t j, uint32_t k) const { key_[j * kWave] = k; }
+};
+
+__global__ __launch_bounds__(kWave) void plan_kernel(KDia dia, KLib lib,
+                                                     KParams prm, KPlan plan,
+                                                     uint32_t *key_scratch,
+                                                     uint32_t *status) {
+  __shared__ float st_mz[kStashFrag * kWave];
+  __shared__ float st_I[kStashFrag * kWave];
+  __shared__ uint32_t st_key[kStashFrag * kWave];
+  const uint32_t row = blockIdx.x * kWave + threadIdx.x;
+  if (row >= lib.n_prec) return;
+  uint32_t st = 0;
+  // src/peak_group_selection/mod.rs:139-148 -> rt_index/mod.rs:89-138
+  const float rt_tol = prm.rt_tol_pp ? prm.rt_tol_pp[row] : prm.rt_tol;
+  const float prt = lib.p_rt[row];
+  uint32_t c0 = 0, c1 = 0;
+  const uint32_t n = dia.n_rt;
+  if (n > 0) {
+    const float min_tol = (10.0f + prm.pad_cycles) * dia.delta_t;
+    const float eff = fmaxf(rt_tol, min_tol);
+    const float lo = prt - eff, hi = prt + eff;
+    if (hi < dia.rt[0]) {
+      c0 = c1 = 0;
+    } else if (lo > dia.rt[n - 1]) {
+      c0 = c1 = n;
+    } else {
+      c0 = (lo <= dia.rt[0]) ? 0u : rust_bsearch_f32(dia.rt, n, lo);
+      c1 = (hi >= dia.rt[n - 1]) ? n : rust_bsearch_f32(dia.rt, n, hi);
+    }
+  }
+  // src/dia_data/mod.rs:79-87 valid observations (inclusive, ascending)
+  const float pmz = lib.p_mz[row];
+  uint32_t nobs = 0, first = 0xffffffffu;
+  for (uint32_t o = 0; o < dia.n_obs; ++o) {
+    if (dia.iso_lo[o] <= pmz && dia.iso_hi[o] >= pmz) {
+      if (nobs < (uint32_t)kObsMax) plan.obs[(size_t)row * kObsMax + nobs] = (uint16_t)o;
+      if (first == 0xffffffffu) first = o;
+      ++nobs;
+    }
+  }
+  if (nobs > (uint32_t)kObsMax) st |= kStTooManyWindows;
+  // src/speclib_flat/mod.rs:388-456 get_precursor_filtered(i, true, true, k)
+  const uint64_t fs = lib.f_start[row];
+  const uint32_t nraw = (uint32_t)(lib.f_stop[row] - fs);
+  uint16_t *sel = plan.sel + (size_t)row * plan.ks;
+  float *smz = plan.sel_mz + (size_t)row * plan.ks;
+  float *sw = plan.sel_w + (size_t)row * plan.ks;
+  uint32_t nsel;
+  if (nraw <= (uint32_t)kStashFrag) {
+    for (uint32_t j = 0; j < nraw; ++j) {
+      const float v = lib.f_int[fs + j];
+      const bool keep =
+          v > 0.0f && !(lib.f_type[fs + j] == 121 && lib.f_num[fs + j] == 1);
+      st_I[j * kWave + threadIdx.x] = keep ? v : -__builtin_inff();
+      st_mz[j * kWave + threadIdx.x] = lib.f_mz[fs + j];
+    }
+    StashFragAcc acc{st_mz + threadIdx.x, st_I + threadIdx.x,
+                     st_key + threadIdx.x};
+    nsel = select_fragments(acc, nraw, prm.top_k, sel, smz, sw, lib.f_int + fs,
+                            plan.ks, st);
+  } else {
+    GlobalFragAcc acc{lib.f_mz + fs, lib.f_int + fs, lib.f_type + fs,
+                      lib.f_num + fs, key_scratch + fs};
+    nsel = select_fragments(acc, nraw, prm.top_k, sel, smz, sw, lib.f_int + fs,
+                            plan.ks, st);
+  }
+  const uint32_t kidx =
+      (prm.per_window && first != 0xffffffffu) ? 1u + first : 0u;
+  plan.rec[row] = make_uint4(c0, c1, nsel | (min(nobs, (uint32_t)kObsMax) << 16),
+                             kidx);
+  if (c1 > c0 && c1 - c0 > (uint32_t)kLCapMax) st |= kStWindowTooLong;
+  if (st) atomicOr(status, st);
+}
+
+// ---------------------------------------------------------------------------
+// bins: one thread per (row, selected-fragment slot)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bins_kernel(KDia dia, KLib lib,
+                                                   KParams prm, KPlan plan) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t row = t / plan.ks;
+  const uint32_t slot = (uint32_t)(t - row * plan.ks);
+  if (row >= lib.n_prec) return;
+  const uint4 rec = plan.rec[row];
+  const uint32_t nsel = rec.z & 0xffffu, nobs = rec.z >> 16;
+  if (slot >= nsel || rec.y <= rec.x) return;
+  // src/quadrupole_observation/mod.rs:79-83
+  const float mz = plan.sel_mz[row * plan.ks + slot];
+  const float delta = mz * prm.mass_tol * 1e-6f;
+  const float lo = mz - delta, hi = mz + delta;
+  uint32_t start, end;
+  mz_range(dia, lo, hi, start, end);
+  for (uint32_t j = 0; j < nobs; ++j) {
+    const uint32_t o = plan.obs[row * kObsMax + j];
+    const uint32_t *ss = dia.ss + (size_t)o * (dia.n_grid + 1);
+    const size_t idx = (row * kObsMax + j) * plan.ks + slot;
+    plan.rng[idx] = make_uint2(ss[start], ss[end]);
+    plan.b0[idx] = start;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// select: one wavefront per precursor
+// ---------------------------------------------------------------------------
+struct SelArgs {
+  KDia dia;
+  KLib lib;
+  KParams prm;
+  KPlan plan;
+  uint32_t n_rows;
+  uint32_t *queue;
+  uint32_t chunk;
+  uint32_t lds_per_wave;  // bytes
+  uint32_t lcap;          // max cycles per window (multiple of 64)
+  uint32_t xcap;          // floats in the XIC tile
+  uint32_t mcap;          // u32 words per mask array
+  float *cand_score;      // [row][cand_count]
+  uint32_t *cand_center;  // [row][cand_count]
+  uint32_t *cand_count;   // [row]
+  uint32_t *status;
+  float *dbg_scores;
+  uint32_t dbg_stride, dbg_row0, dbg_rows;
+};
+
+struct WaveLds {
+  float *R;        // lcap
+  float *X;        // xcap
+  uint32_t *M;     // mcap (hit mask per fragment row)
+  uint32_t *D;     // mcap (dilated mask per fragment row)
+  uint16_t *items; // lcap
+  float *w;        // kMaxSel
+  uint32_t *pre;   // kMaxSel + 1
+  uint32_t *eb;    // kMaxSel
+  float *kern;     // 32
+};
+
+__device__ __forceinline__ WaveLds carve(char *base, const SelArgs &a) {
+  WaveLds l;
+  char *p = base;
+  l.R = (float *)p;
+  p += a.lcap * 4;
+  l.X = (float *)p;
+  p += a.xcap * 4;
+  l.M = (uint32_t *)p;
+  p += a.mcap * 4;
+  l.D = (uint32_t *)p;
+  p += a.mcap * 4;
+  l.items = (uint16_t *)p;
+  p += a.lcap * 2;
+  l.w = (float *)p;
+  p += kMaxSel * 4;
+  l.pre = (uint32_t *)p;
+  p += (kMaxSel + 1) * 4;
+  l.eb = (uint32_t *)p;
+  p += kMaxSel * 4;
+  l.kern = (float *)p;
+  return l;
+}
+
+// Adds intensity I to X[f][t] (0.0 + I on the first contribution, as the
+// zero-initialised reference array) and marks the hit and its
+// convolution support in the masks.
+__device__ __forceinline__ void xic_add(const WaveLds &l, uint32_t f,
+                                        uint32_t t, float I, uint32_t L,
+                                        uint32_t wm, uint32_t half) {
+  uint32_t *Mrow = l.M + f * wm;
+  uint32_t *Drow = l.D + f * wm;
+  const uint32_t bit = 1u << (t & 31);
+  const float prev = (Mrow[t >> 5] & bit) ? l.X[f * L + t] : 0.0f;
+  l.X[f * L + t] = prev + I;
+  atomicOr(&Mrow[t >> 5], bit);
+  const uint32_t a = t >= half ? t - half : 0u;
+  const uint32_t b = min(t + half, L - 1);  // inclusive, b - a <= 30
+  const uint32_t wa = a >> 5, wb = b >> 5;
+  if (wa == wb) {
+    const uint32_t n = b - a + 1;
+    const uint32_t m = (n == 32 ? 0xffffffffu : ((1u << n) - 1u)) << (a & 31);
+    atomicOr(&Drow[wa], m);
+  } else {
+    atomicOr(&Drow[wa], 0xffffffffu << (a & 31));
+    const uint32_t nb = (b & 31) + 1;
+    atomicOr(&Drow[wb], nb == 32 ? 0xffffffffu : ((1u << nb) - 1u));
+  }
+}
+
+// src/convolution/scalar.rs:14-36 for one output i (half <= i < L-half)
+// of row f: s = 0 + X[i]*k[h]; then pairs at distance h..1, nearest last.
+// Pairs whose two inputs are both zero contribute exactly +0 and are skipped.
+__device__ __forceinline__ float conv_one(const float *Xrow,
+                                          const uint32_t *Mrow, uint32_t i,
+                                          uint32_t half, const float *k) {
+  const uint32_t start = i - half;
+  const uint32_t wd = start >> 5, sh = start & 31;
+  const uint64_t w64 = (uint64_t)Mrow[wd] | ((uint64_t)Mrow[wd + 1] << 32);
+  const uint32_t nbits = 2 * half + 1;
+  const uint32_t bits = (uint32_t)(w64 >> sh) & ((1u << nbits) - 1u);
+  float s = 0.0f;
+  const float xc = ((bits >> half) & 1u) ? Xrow[i] : 0.0f;
+  s = s + xc * k[half];
+  const uint32_t low = (1u << half) - 1u;
+  uint32_t pairs =
+      (bits & low) | ((__builtin_bitreverse32(bits) >> (31 - 2 * half)) & low);
+  while (pairs) {
+    const uint32_t b = __builtin_ctz(pairs);
+    pairs &= pairs - 1u;
+    const uint32_t d = half - b;
+    const float xl = ((bits >> b) & 1u) ? Xrow[i - d] : 0.0f;
+    const float xr = ((bits >> (2 * half - b)) & 1u) ? Xrow[i + d] : 0.0f;
+    s = s + (xl + xr) * k[b];
+  }
+  return s;
+}
+
+__global__ void reset_queue_kernel(uint32_t *q, uint32_t *status) {
+  if (threadIdx.x == 0) {
+    q[0] = 0;
+    status[0] = 0;
+  }
+}
+
+template <int WAVES>
+__global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ LogfEntry ltab[16];
+  if (threadIdx.x < 16) ltab[threadIdx.x] = kDevLogfTab[threadIdx.x];
+  __syncthreads();
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const WaveLds l = carve(smem + (size_t)wv * a.lds_per_wave, a);
+  const uint32_t half = a.prm.half;
+  const uint32_t cc = a.prm.cand_count;
+  const uint32_t ks = a.plan.ks;
+
+  for (;;) {
+    uint32_t row0 = 0;
+    if (lane == 0) row0 = atomicAdd(a.queue, a.chunk);
+    row0 = __shfl(row0, 0, kWave);
+    if (row0 >= a.n_rows) break;
+    const uint32_t row_end = min(row0 + a.chunk, a.n_rows);
+    for (uint32_t row = row0; row < row_end; ++row) {
+      const uint4 rec = a.plan.rec[row];
+      const uint32_t c0 = rec.x, c1 = rec.y;
+      const uint32_t nsel = rec.z & 0xffffu, nobs = rec.z >> 16;
+      if (c1 <= c0) {  // mod.rs:146-148 -> None
+        if (lane == 0) a.cand_count[row] = 0;
+        continue;
+      }
+      const uint32_t L = c1 - c0;
+      if (L > a.lcap) {  // guarded by the host bound; never expected
+        if (lane == 0) {
+          a.cand_count[row] = 0;
+          atomicOr(a.status, (uint32_t)kStWindowTooLong);
+        }
+        continue;
+      }
+      const uint32_t wm = (L + 31) / 32 + 1;
+      for (uint32_t i = lane; i < L; i += kWave) l.R[i] = 0.0f;
+      if (lane < (int)a.prm.klen)
+        l.kern[lane] = a.prm.kernels[(size_t)rec.w * a.prm.klen + lane];
+      if (lane < (int)nsel) l.w[lane] = a.plan.sel_w[(size_t)row * ks + lane];
+      __builtin_amdgcn_wave_barrier();
+
+      const uint32_t gcap = min(a.xcap / L, a.mcap / wm);
+      for (uint32_t g0 = 0; g0 < nsel && nobs > 0; g0 += gcap) {
+        const uint32_t G = min(gcap, nsel - g0);
+        for (uint32_t i = lane; i < G * wm; i += kWave) {
+          l.M[i] = 0u;
+          l.D[i] = 0u;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // ---- extraction: src/dense_xic_observation/mod.rs:42-55 --------
+        for (uint32_t j = 0; j < nobs; ++j) {
+          const uint32_t o = a.plan.obs[(size_t)row * kObsMax + j];
+          const uint64_t base = a.dia.entry_base[o];
+          const size_t ridx = ((size_t)row * kObsMax + j) * ks + g0;
+          uint32_t len = 0;
+          if (lane < (int)G) {
+            const uint2 r = a.plan.rng[ridx + lane];
+            l.eb[lane] = r.x;
+            len = r.y - r.x;
+          }
+          const uint32_t incl = wave_incl_scan(len);
+          if (lane < (int)G) l.pre[lane] = incl - len;
+          const uint32_t T = __shfl(incl, (int)G - 1, kWave);
+          if (lane == 0) l.pre[G] = T;
+          __builtin_amdgcn_wave_barrier();
+          for (uint32_t q0 = 0; q0 < T; q0 += kWave) {
+            const uint32_t q = q0 + lane;
+            bool hit = false;
+            uint32_t f = 0, t = 0;
+            float I = 0.0f;
+            if (q < T) {
+              // fragment owning concatenated entry q
+              uint32_t lo = 0, hi = G - 1;
+              while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (l.pre[mid] <= q) lo = mid;
+                else hi = mid - 1;
+              }
+              f = lo;
+              const uint32_t e = l.eb[f] + (q - l.pre[f]);
+              const uint32_t cyc = a.dia.cycles[base + e];
+              hit = cyc >= c0 && cyc < c1;
+              if (hit && cyc == c0) {
+                // fill_xic_slice starts at binary_search(c0), which is the
+                // LAST entry equal to c0 within the bin
+                // (src/quadrupole_observation/mod.rs:92-94): an entry equal
+                // to c0 followed by another c0 entry of the same bin is
+                // skipped.
+                const uint32_t *ss = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
+                uint32_t b = a.plan.b0[ridx + f];
+                while (ss[b + 1] <= e) ++b;
+                if (e + 1 < ss[b + 1] && a.dia.cycles[base + e + 1] == c0)
+                  hit = false;
+              }
+              if (hit) {
+                t = cyc - c0;
+                I = a.dia.intens[base + e];
+              }
+            }
+            uint64_t hm = ballot64(hit);
+            if (hm) {
+              const uint32_t key = f * L + t;
+              bool dup = false;
+              uint64_t it = hm;
+              while (it) {
+                const int j2 = __builtin_ctzll(it);
+                it &= it - 1;
+                const uint32_t kj = __shfl(key, j2, kWave);
+                if (hit && lane > j2 && key == kj) dup = true;
+              }
+              if (ballot64(dup) == 0) {
+                if (hit) xic_add(l, f, t, I, L, wm, half);
+              } else {
+                uint64_t it2 = hm;
+                while (it2) {
+                  const int j2 = __builtin_ctzll(it2);
+                  it2 &= it2 - 1;
+                  if (lane == j2) xic_add(l, f, t, I, L, wm, half);
+                  __builtin_amdgcn_wave_barrier();
+                }
+              }
+            }
+            __builtin_amdgcn_wave_barrier();
+          }
+        }
+        // ---- convolution + score, fragment by fragment ------------------
+        for (uint32_t f = 0; f < G; ++f) {
+          const uint32_t *Drow = l.D + f * wm;
+          const uint32_t *Mrow = l.M + f * wm;
+          const float *Xrow = l.X + f * L;
+          const float wf = l.w[g0 + f];
+          uint32_t nitems = 0;
+          for (uint32_t p0 = 0; p0 < L; p0 += kWave) {
+            const uint64_t dw =
+                (uint64_t)Drow[p0 >> 5] | ((uint64_t)Drow[(p0 >> 5) + 1] << 32);
+            if (dw == 0) continue;
+            const uint32_t p = p0 + lane;
+            const bool act = ((dw >> lane) & 1ull) && p >= half && p + half < L;
+            const uint64_t bm = ballot64(act);
+            if (act) {
+              const uint32_t before =
+                  __builtin_popcountll(bm & ((1ull << lane) - 1ull));
+              l.items[nitems + before] = (uint16_t)p;
+            }
+            nitems += (uint32_t)__builtin_popcountll(bm);
+          }
+          __builtin_amdgcn_wave_barrier();
+          for (uint32_t c = 0; c < nitems; c += kWave) {
+            const uint32_t idx = c + lane;
+            if (idx < nitems) {
+              const uint32_t i = l.items[idx];
+              const float y = conv_one(Xrow, Mrow, i, half, l.kern);
+              if (y != 0.0f) {
+                const float v = logf_exact(y + 1.0f, ltab);
+                l.R[i] = l.R[i] + v * wf;
+              }
+            }
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (a.dbg_scores && row >= a.dbg_row0 && row < a.dbg_row0 + a.dbg_rows &&
+          L <= a.dbg_stride) {
+        float *out = a.dbg_scores + (size_t)(row - a.dbg_row0) * a.dbg_stride;
+        for (uint32_t i = lane; i < L; i += kWave) out[i] = l.R[i];
+      }
+      // ---- local maxima and top-k: mod.rs:23-64, 167-192 -----------------
+      uint64_t ismax = 0;
+      for (uint32_t m = 0; m * kWave + lane < L && m < 64; ++m) {
+        const uint32_t i = m * kWave + lane;
+        if (i >= 2 && i + 2 < L) {
+          const float v = l.R[i];
+          if (l.R[i - 2] < l.R[i - 1] && l.R[i - 1] < v && v > l.R[i + 1] &&
+              l.R[i + 1] > l.R[i + 2])
+            ismax |= 1ull << m;
+        }
+      }
+      uint32_t ncand = 0;
+      for (uint32_t r = 0; r < cc; ++r) {
+        float bv = 0.0f;
+        uint32_t bi = 0xffffffffu;
+        uint64_t it = ismax;
+        while (it) {
+          const uint32_t m = __builtin_ctzll(it);
+          it &= it - 1;
+          const uint32_t i = m * kWave + lane;
+          const float v = l.R[i];
+          if (bi == 0xffffffffu || v > bv) {
+            bv = v;
+            bi = i;
+          }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+          const float ov = __shfl_xor(bv, o, kWave);
+          const uint32_t oi = __shfl_xor(bi, o, kWave);
+          const bool take = (oi != 0xffffffffu) &&
+                            (bi == 0xffffffffu || ov > bv || (ov == bv && oi < bi));
+          if (take) {
+            bv = ov;
+            bi = oi;
+          }
+        }
+        if (bi == 0xffffffffu) break;
+        if ((bi & (kWave - 1)) == (uint32_t)lane) ismax &= ~(1ull << (bi / kWave));
+        if (lane == 0) {
+          a.cand_score[(size_t)row * cc + r] = bv;
+          a.cand_center[(size_t)row * cc + r] = c0 + bi;
+        }
+        ++ncand;
+      }
+      if (lane == 0) a.cand_count[row] = ncand;
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// order-preserving compaction
+// ---------------------------------------------------------------------------
+constexpr int kScanBlock = 1024;  // rows per scan block (256 threads x 4)
+
+__global__ __launch_bounds__(256) void scan_local_kernel(
+    const uint32_t *counts, uint32_t n, uint32_t *local, uint32_t *bsum) {
+  __shared__ uint32_t wsum[4];
+  const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * 4;
+  uint32_t v[4], s = 0;
+  for (int k = 0; k < 4; ++k) {
+    v[k] = (base + k < n) ? counts[base + k] : 0u;
+    s += v[k];
+  }
+  const uint32_t incl = wave_incl_scan(s);
+  const int wv = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) wsum[wv] = incl;
+  __syncthreads();
+  uint32_t off = 0;
+  for (int k = 0; k < wv; ++k) off += wsum[k];
+  uint32_t run = off + incl - s;
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) local[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) bsum[blockIdx.x] = off + incl;
+}
+
+__global__ __launch_bounds__(1024) void scan_blocks_kernel(uint32_t *bsum,
+                                                           uint32_t nb,
+                                                           uint32_t *total) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+    const uint32_t i = b0 + threadIdx.x;
+    const uint32_t v = i < nb ? bsum[i] : 0u;
+    const uint32_t incl = wave_incl_scan(v);
+    const int wv = threadIdx.x / kWave;
+    if (lane_id() == kWave - 1) wsum[wv] = incl;
+    __syncthreads();
+    uint32_t off = carry;
+    for (int k = 0; k < wv; ++k) off += wsum[k];
+    if (i < nb) bsum[i] = off + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = off + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) total[0] = carry;
+}
+
+struct CandRow {
+  uint32_t row, rank;
+  float score;
+  uint32_t center;
+};
+
+__global__ __launch_bounds__(256) void scatter_kernel(
+    const uint32_t *counts, const uint32_t *local, const uint32_t *bsum,
+    uint32_t n, uint32_t cc, const float *cs, const uint32_t *cctr,
+    CandRow *out) {
+  const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const uint32_t k = counts[row];
+  const uint32_t off = local[row] + bsum[row / kScanBlock];
+  for (uint32_t r = 0; r < k; ++r) {
+    CandRow c;
+    c.row = row;
+    c.rank = r;
+    c.score = cs[(size_t)row * cc + r];
+    c.center = cctr[(size_t)row * cc + r];
+    out[off + r] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-stage kernels (SimdBackend operations)
+// ---------------------------------------------------------------------------
+// src/convolution/scalar.rs:5-56, one thread per output element
+__global__ void conv_dense_kernel(const float *k, uint32_t klen, const float *x,
+                                  uint32_t rows, uint32_t cols, float *y) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)rows * cols) return;
+  const uint32_t f = (uint32_t)(t / cols), i = (uint32_t)(t % cols);
+  const uint32_t half = klen / 2;
+  float out = 0.0f;
+  if (cols >= klen && i >= half && i + half < cols) {
+    const float *xr = x + (size_t)f * cols;
+    float s = 0.0f;
+    s = s + xr[i] * k[half];
+    for (uint32_t kk = 0; kk < half; ++kk) {
+      const float kv = k[kk];
+      s = s + (xr[i - (half - kk)] + xr[i + (half - kk)]) * kv;
+    }
+    out = s;
+  }
+  y[t] = out;
+}
+
+// src/score/scalar.rs:4-25, one thread per column, rows in order
+__global__ void logdot_kernel(const float *a, uint32_t rows, uint32_t cols,
+                              const float *w, float *out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  float r = 0.0f;
+  for (uint32_t i = 0; i < rows; ++i) {
+    const float v = logf_exact(a[(size_t)i * cols + j] + 1.0f, kDevLogfTab);
+    r = r + v * w[i];
+  }
+  out[j] = r;
+}
+
+__global__ void logf_kernel(const float *x, uint64_t n, float *y) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = logf_exact(x[t], kDevLogfTab);
+}
+
+// src/peak_group_selection/mod.rs:23-64 on one wavefront: flags in parallel,
+// then a stable descending insertion order by lane 0.
+__global__ void maxima_kernel(const float *a, uint32_t n, uint64_t offset,
+                              uint64_t *idx, float *val, uint32_t *count) {
+  if (threadIdx.x != 0) return;
+  uint32_t m = 0;
+  if (n >= 5) {
+    for (uint32_t i = 2; i < n - 2; ++i) {
+      const float v = a[i];
+      if (a[i - 2] < a[i - 1] && a[i - 1] < v && v > a[i + 1] &&
+          a[i + 1] > a[i + 2]) {
+        uint32_t j = m;
+        while (j > 0 && v > val[j - 1]) {
+          val[j] = val[j - 1];
+          idx[j] = idx[j - 1];
+          --j;
+        }
+        val[j] = v;
+        idx[j] = i + offset;
+        ++m;
+      }
+    }
+  }
+  *count = m;
+}
+
+// src/dense_xic_observation/mod.rs:30-64: one thread per fragment row, valid
+// windows in ascending order, bins ascending, entries from the last entry
+// equal to cycle_start (Rust binary_search).
+__global__ void dense_xic_kernel(KDia dia, float pmz, uint32_t c0, uint32_t c1,
+                                 float tol, const float *fmz, uint32_t nf,
+                                 float *out) {
+  const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const uint32_t L = c1 - c0;
+  float *row = out + (size_t)f * L;
+  for (uint32_t i = 0; i < L; ++i) row[i] = 0.0f;
+  const float mz = fmz[f];
+  const float delta = mz * tol * 1e-6f;
+  uint32_t start, end;
+  mz_range(dia, mz - delta, mz + delta, start, end);
+  for (uint32_t o = 0; o < dia.n_obs; ++o) {
+    if (!(dia.iso_lo[o] <= pmz && dia.iso_hi[o] >= pmz)) continue;
+    const uint32_t *ss = dia.ss + (size_t)o * (dia.n_grid + 1);
+    const uint16_t *cy = dia.cycles + dia.entry_base[o];
+    const float *in = dia.intens + dia.entry_base[o];
+    for (uint32_t b = start; b < end; ++b) {
+      const uint32_t s0 = ss[b], n = ss[b + 1] - s0;
+      // Rust binary_search on u16
+      uint32_t pos = 0;
+      if (n > 0) {
+        const uint16_t key = (uint16_t)c0;
+        uint32_t size = n, base = 0;
+        while (size > 1) {
+          const uint32_t h = size >> 1, mid = base + h;
+          if (!(cy[s0 + mid] > key)) base = mid;
+          size -= h;
+        }
+        pos = (cy[s0 + base] == key) ? base : base + (cy[s0 + base] < key ? 1 : 0);
+      }
+      for (uint32_t i = pos; i < n; ++i) {
+        const uint32_t c = cy[s0 + i];
+        if (c >= c1) break;
+        row[c - c0] = row[c - c0] + in[s0 + i];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side device management
+// ---------------------------------------------------------------------------
+static int g_device_state = -1;  // -1 unknown, 0 none, 1 gfx950 present
+static std::mutex g_dev_mu;
+static std::atomic<int> g_backend_override{-1};
+
+static bool device_present() {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  if (g_device_state >= 0) return g_device_state == 1;
+  int n = 0;
+  g_device_state = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return false;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return false;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return false;
+  g_device_state = 1;
+  return true;
+}
+
+static adsr_status require_device() {
+  if (!device_present())
+    return set_error(ADSR_ERR_DEVICE,
+                     "no gfx950 (MI355X) device is visible; the hip backend "
+                     "is the only selection backend");
+  return ADSR_OK;
+}
+
+template <class T>
+static adsr_status dmalloc_copy(T **dst, const T *src, size_t n, size_t &acc) {
+  *dst = nullptr;
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  HIP_TRY(hipMalloc((void **)dst, bytes));
+  if (n) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  acc += bytes;
+  return ADSR_OK;
+}
+
+void device_dia_free(DeviceDia &d) {
+  if (!d.ready && !d.grid) return;
+  (void)hipFree(d.grid);
+  (void)hipFree(d.bucket);
+  (void)hipFree(d.rt);
+  (void)hipFree(d.iso_lo);
+  (void)hipFree(d.iso_hi);
+  (void)hipFree(d.slice_starts);
+  (void)hipFree(d.entry_base);
+  (void)hipFree(d.cycles);
+  (void)hipFree(d.intens);
+  d = DeviceDia();
+}
+
+void device_lib_free(DeviceLib &l) {
+  if (!l.ready && !l.p_mz) return;
+  (void)hipFree(l.p_idx);
+  (void)hipFree(l.p_mz);
+  (void)hipFree(l.p_rt);
+  (void)hipFree(l.f_start);
+  (void)hipFree(l.f_stop);
+  (void)hipFree(l.f_mz);
+  (void)hipFree(l.f_int);
+  (void)hipFree(l.f_type);
+  (void)hipFree(l.f_num);
+  l = DeviceLib();
+}
+
+static adsr_status upload_dia(adsr_dia *d) {
+  std::lock_guard<std::mutex> lk(d->mu);
+  if (d->dev.ready) return ADSR_OK;
+  adsr_status st = require_device();
+  if (st) return st;
+  const Grid &g = global_grid();
+  DeviceDia &v = d->dev;
+  size_t acc = 0;
+#define UP(dst, src, n)                                  \
+  do {                                                   \
+    adsr_status s_ = dmalloc_copy(&(dst), (src), (n), acc); \
+    if (s_) {                                            \
+      device_dia_free(v);                                \
+      return s_;                                         \
+    }                                                    \
+  } while (0)
+  UP(v.grid, g.mz.data(), g.mz.size());
+  UP(v.bucket, g.bucket.data(), g.bucket.size());
+  UP(v.rt, d->rt_index.data(), d->rt_index.size());
+  UP(v.iso_lo, d->iso_lo.data(), d->iso_lo.size());
+  UP(v.iso_hi, d->iso_hi.data(), d->iso_hi.size());
+  UP(v.slice_starts, d->slice_starts.data(), d->slice_starts.size());
+  UP(v.entry_base, d->entry_base.data(), d->entry_base.size());
+  UP(v.cycles, d->cycles.data(), d->cycles.size());
+  UP(v.intens, d->intens.data(), d->intens.size());
+#undef UP
+  v.bytes = acc;
+  v.ready = true;
+  return ADSR_OK;
+}
+
+static adsr_status upload_lib(adsr_lib *l) {
+  std::lock_guard<std::mutex> lk(l->mu);
+  if (l->dev.ready) return ADSR_OK;
+  adsr_status st = require_device();
+  if (st) return st;
+  DeviceLib &v = l->dev;
+  size_t acc = 0;
+#define UP(dst, src, n)                                  \
+  do {                                                   \
+    adsr_status s_ = dmalloc_copy(&(dst), (src), (n), acc); \
+    if (s_) {                                            \
+      device_lib_free(v);                                \
+      return s_;                                         \
+    }                                                    \
+  } while (0)
+  UP(v.p_idx, l->precursor_idx.data(), l->n_prec);
+  UP(v.p_mz, l->precursor_mz.data(), l->n_prec);
+  UP(v.p_rt, l->precursor_rt.data(), l->n_prec);
+  UP(v.f_start, l->frag_start.data(), l->n_prec);
+  UP(v.f_stop, l->frag_stop.data(), l->n_prec);
+  UP(v.f_mz, l->fragment_mz.data(), l->n_frag);
+  UP(v.f_int, l->fragment_intensity.data(), l->n_frag);
+  UP(v.f_type, l->fragment_type.data(), l->n_frag);
+  UP(v.f_num, l->fragment_number.data(), l->n_frag);
+#undef UP
+  v.bytes = acc;
+  v.ready = true;
+  return ADSR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// search workspace and launch
+// ---------------------------------------------------------------------------
+struct LaunchShape {
+  int waves = 1;  // waves per block: 1, 2 or 4 (64/128/256 threads)
+  int chunk = 1;  // precursors per queue grab: 1, 2, 4
+};
+
+struct Workspace {
+  size_t n_rows = 0, ks = 0, cc = 0, nker = 0;
+  uint4 *rec = nullptr;
+  uint16_t *obs = nullptr, *sel = nullptr;
+  float *sel_mz = nullptr, *sel_w = nullptr;
+  uint2 *rng = nullptr;
+  uint32_t *b0 = nullptr;
+  float *cand_score = nullptr;
+  uint32_t *cand_center = nullptr, *cand_count = nullptr;
+  uint32_t *local = nullptr, *bsum = nullptr, *small = nullptr;  // small: queue, status, total
+  CandRow *out = nullptr;
+  float *kernels = nullptr;
+  float *rt_tol_pp = nullptr;
+  uint32_t *key_scratch = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_src = nullptr;
+};
+
+static void ws_free(Workspace &w) {
+  if (w.graph) (void)hipGraphExecDestroy(w.graph);
+  if (w.graph_src) (void)hipGraphDestroy(w.graph_src);
+  (void)hipFree(w.rec);
+  (void)hipFree(w.obs);
+  (void)hipFree(w.sel);
+  (void)hipFree(w.sel_mz);
+  (void)hipFree(w.sel_w);
+  (void)hipFree(w.rng);
+  (void)hipFree(w.b0);
+  (void)hipFree(w.cand_score);
+  (void)hipFree(w.cand_center);
+  (void)hipFree(w.cand_count);
+  (void)hipFree(w.local);
+  (void)hipFree(w.bsum);
+  (void)hipFree(w.small);
+  (void)hipFree(w.out);
+  (void)hipFree(w.kernels);
+  (void)hipFree(w.rt_tol_pp);
+  (void)hipFree(w.key_scratch);
+  for (auto &e : w.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (w.stream) (void)hipStreamDestroy(w.stream);
+  w = Workspace();
+}
+
+static adsr_status ws_alloc(Workspace &w, size_t n_rows, size_t ks, size_t cc,
+                            size_t nker, bool pp, size_t n_frag_scratch) {
+  ws_free(w);
+  w.n_rows = n_rows;
+  w.ks = ks;
+  w.cc = cc;
+  w.nker = nker;
+  const size_t R = std::max<size_t>(n_rows, 1);
+  const size_t nb = (R + kScanBlock - 1) / kScanBlock;
+  HIP_TRY(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+  for (auto &e : w.ev) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipMalloc(&w.rec, R * sizeof(uint4)));
+  HIP_TRY(hipMalloc(&w.obs, R * kObsMax * 2));
+  HIP_TRY(hipMalloc(&w.sel, R * ks * 2));
+  HIP_TRY(hipMalloc(&w.sel_mz, R * ks * 4));
+  HIP_TRY(hipMalloc(&w.sel_w, R * ks * 4));
+  HIP_TRY(hipMalloc(&w.rng, R * kObsMax * ks * sizeof(uint2)));
+  HIP_TRY(hipMalloc(&w.b0, R * kObsMax * ks * 4));
+  HIP_TRY(hipMalloc(&w.cand_score, R * std::max<size_t>(cc, 1) * 4));
+  HIP_TRY(hipMalloc(&w.cand_center, R * std::max<size_t>(cc, 1) * 4));
+  HIP_TRY(hipMalloc(&w.cand_count, R * 4));
+  HIP_TRY(hipMalloc(&w.local, R * 4));
+  HIP_TRY(hipMalloc(&w.bsum, nb * 4));
+  HIP_TRY(hipMalloc(&w.small, 64));
+  HIP_TRY(hipMalloc(&w.out, R * std::max<size_t>(cc, 1) * sizeof(CandRow)));
+  HIP_TRY(hipMalloc(&w.kernels, nker * 4));
+  HIP_TRY(hipMalloc(&w.rt_tol_pp, (pp ? R : 1) * 4));
+  HIP_TRY(hipMalloc(&w.key_scratch, std::max<size_t>(n_frag_scratch, 1) * 4));
+  return ADSR_OK;
+}
+
+struct SearchPlan {
+  KDia dia;
+  KLib lib;
+  KParams prm;
+  KPlan plan;
+  uint32_t lcap = 0, xcap = 0, mcap = 0, lds_per_wave = 0;
+  int cus = 256;
+  std::vector<float> kernels;
+};
+
+// Largest number of RT-index points inside any closed window of width
+// 2*eff (+ rounding slack).  Bounds c1 - c0 for every precursor when the
+// RT axis is sorted; the full axis otherwise.
+static uint32_t window_bound(const adsr_dia *d, float eff) {
+  const std::vector<float> &rt = d->rt_index;
+  const size_t n = rt.size();
+  if (n == 0) return 0;
+  if (!d->rt_sorted || !std::isfinite(eff)) return (uint32_t)n;
+  const double width = 2.0 * (double)eff * (1.0 + 1e-5) + 1e-3;
+  size_t best = 0, j = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (j < i) j = i;
+    while (j < n && (double)rt[j] - (double)rt[i] <= width) ++j;
+    best = std::max(best, j - i);
+  }
+  return (uint32_t)std::min<size_t>(n, best + 2);
+}
+
+static adsr_status prepare(const adsr_dia *d, const adsr_lib *l,
+                           const adsr_selection_params *p, SearchPlan &sp,
+                           size_t &ks_out) {
+  if (p->kernel_size == 0)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT,
+                     "kernel_size must be >= 1 (the reference indexes "
+                     "kernel_array[kernel_size/2])");
+  const size_t klen = (size_t)std::ceil((float)p->kernel_size / 2.0f) * 2;
+  if (klen / 2 > (size_t)kHalfMax)
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "kernel_size %llu exceeds the device limit of %d",
+                     (unsigned long long)p->kernel_size, 2 * kHalfMax);
+  if (p->sigma_per_window && p->n_sigma_per_window != d->n_obs)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT,
+                     "sigma_per_window has %llu values for %zu observations",
+                     (unsigned long long)p->n_sigma_per_window, d->n_obs);
+  sp.kernels.assign(klen, 0.0f);
+  adsr_gaussian_kernel(p->fwhm_rt, 1.0f, p->kernel_size, 1.0f,
+                       sp.kernels.data(), klen);
+  if (p->sigma_per_window) {
+    sp.kernels.resize(klen * (1 + d->n_obs));
+    for (size_t o = 0; o < d->n_obs; ++o)
+      adsr_gaussian_kernel_sigma(p->sigma_per_window[o], klen,
+                                 sp.kernels.data() + klen * (1 + o), klen);
+  }
+  // selected-fragment stride
+  uint64_t ks = std::min<uint64_t>(p->top_k_fragments, l->max_raw_fragments);
+  ks = std::min<uint64_t>(ks, kMaxSel);
+  if (ks == 0) ks = 1;
+  ks_out = ks;
+  // window bound
+  float tol = p->rt_tolerance;
+  if (p->rt_tolerance_per_precursor) {
+    tol = -__builtin_inff();
+    for (size_t i = 0; i < l->n_prec; ++i)
+      tol = std::max(tol, p->rt_tolerance_per_precursor[i]);
+    if (l->n_prec == 0) tol = 0.0f;
+  }
+  const float min_tol = (10.0f + (float)p->kernel_size) * d->delta_t;
+  const float eff = std::max(tol, min_tol);
+  uint32_t lb = window_bound(d, eff);
+  if (lb > (uint32_t)kLCapMax) lb = kLCapMax;  // longer windows are flagged
+  sp.lcap = std::max<uint32_t>(64, (lb + 63) / 64 * 64);
+  const uint32_t wmax = sp.lcap / 32 + 1;
+  // XIC tile: all selected fragments when they fit in 24 KB, else groups
+  const uint32_t xbudget = 6144;  // floats
+  uint32_t g = (uint32_t)ks;
+  if ((uint64_t)g * sp.lcap > xbudget) g = std::max<uint32_t>(1, xbudget / sp.lcap);
+  sp.xcap = g * sp.lcap;
+  sp.mcap = g * wmax;
+  uint32_t bytes = sp.lcap * 4 + sp.xcap * 4 + 2 * sp.mcap * 4 + sp.lcap * 2 +
+                   kMaxSel * 4 + (kMaxSel + 1) * 4 + kMaxSel * 4 + 32 * 4;
+  sp.lds_per_wave = (bytes + 15) / 16 * 16;
+  if (sp.lds_per_wave > kMaxDynLds)
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "per-precursor LDS tile of %u bytes exceeds the device",
+                     sp.lds_per_wave);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
+    sp.cus = prop.multiProcessorCount;
+  return ADSR_OK;
+}
+
+static void set_lds_limits() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    (void)hipFuncSetAttribute((const void *)select_kernel<1>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kMaxDynLds);
+    (void)hipFuncSetAttribute((const void *)select_kernel<2>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kMaxDynLds);
+    (void)hipFuncSetAttribute((const void *)select_kernel<4>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kMaxDynLds);
+  });
+}
+
+static int occupancy_blocks(const void *fn, int threads, size_t lds) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, lds) !=
+      hipSuccess)
+    return 1;
+  return std::max(nb, 1);
+}
+
+static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
+                           LaunchShape shape, float *dbg,
+                           uint32_t dbg_stride, uint32_t dbg_row0,
+                           uint32_t dbg_rows, bool timed_select,
+                           bool plan_only = false) {
+  hipStream_t s = w.stream;
+  uint32_t *queue = w.small, *status = w.small + 1, *total = w.small + 2;
+  reset_queue_kernel<<<1, 64, 0, s>>>(queue, status);
+  if (n_rows && plan_only) {
+    plan_kernel<<<(n_rows + kWave - 1) / kWave, kWave, 0, s>>>(
+        sp.dia, sp.lib, sp.prm, sp.plan, w.key_scratch, status);
+  } else if (n_rows) {
+    plan_kernel<<<(n_rows + kWave - 1) / kWave, kWave, 0, s>>>(
+        sp.dia, sp.lib, sp.prm, sp.plan, w.key_scratch, status);
+    const uint64_t nt = (uint64_t)n_rows * sp.plan.ks;
+    bins_kernel<<<(uint32_t)((nt + 255) / 256), 256, 0, s>>>(sp.dia, sp.lib,
+                                                              sp.prm, sp.plan);
+    SelArgs a;
+    a.dia = sp.dia;
+    a.lib = sp.lib;
+    a.prm = sp.prm;
+    a.plan = sp.plan;
+    a.n_rows = n_rows;
+    a.queue = queue;
+    a.chunk = (uint32_t)shape.chunk;
+    a.lds_per_wave = sp.lds_per_wave;
+    a.lcap = sp.lcap;
+    a.xcap = sp.xcap;
+    a.mcap = sp.mcap;
+    a.cand_score = w.cand_score;
+    a.cand_center = w.cand_center;
+    a.cand_count = w.cand_count;
+    a.status = status;
+    a.dbg_scores = dbg;
+    a.dbg_stride = dbg_stride;
+    a.dbg_row0 = dbg_row0;
+    a.dbg_rows = dbg_rows;
+    while (shape.waves > 1 &&
+           (size_t)sp.lds_per_wave * shape.waves > kMaxDynLds)
+      shape.waves >>= 1;
+    const size_t lds = (size_t)sp.lds_per_wave * shape.waves;
+    const void *fn = shape.waves == 1   ? (const void *)select_kernel<1>
+                     : shape.waves == 2 ? (const void *)select_kernel<2>
+                                        : (const void *)select_kernel<4>;
+    const int per_cu = occupancy_blocks(fn, shape.waves * kWave, lds);
+    const uint64_t want = ((uint64_t)n_rows + shape.chunk - 1) / shape.chunk;
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)per_cu * sp.cus,
+                              (want + shape.waves - 1) / shape.waves));
+    if (timed_select) HIP_TRY(hipEventRecord(w.ev[2], s));
+    if (shape.waves == 1)
+      select_kernel<1><<<blocks, 64, lds, s>>>(a);
+    else if (shape.waves == 2)
+      select_kernel<2><<<blocks, 128, lds, s>>>(a);
+    else
+      select_kernel<4><<<blocks, 256, lds, s>>>(a);
+    if (timed_select) HIP_TRY(hipEventRecord(w.ev[3], s));
+    const uint32_t nb = (n_rows + kScanBlock - 1) / kScanBlock;
+    scan_local_kernel<<<nb, 256, 0, s>>>(w.cand_count, n_rows, w.local, w.bsum);
+    scan_blocks_kernel<<<1, 1024, 0, s>>>(w.bsum, nb, total);
+    scatter_kernel<<<(n_rows + 255) / 256, 256, 0, s>>>(
+        w.cand_count, w.local, w.bsum, n_rows, (uint32_t)sp.prm.cand_count,
+        w.cand_score, w.cand_center, w.out);
+  }
+  HIP_TRY(hipGetLastError());
+  return ADSR_OK;
+}
+
+static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
+                                const adsr_selection_params *p, Workspace &w,
+                                SearchPlan &sp) {
+  set_lds_limits();
+  size_t ks = 1;
+  adsr_status st = prepare(d, l, p, sp, ks);
+  if (st) return st;
+  const size_t cc = (size_t)std::min<uint64_t>(p->candidate_count, 1u << 20);
+  st = ws_alloc(w, l->n_prec, ks, cc, sp.kernels.size(),
+                p->rt_tolerance_per_precursor != nullptr,
+                l->max_raw_fragments > (uint64_t)kStashFrag ? l->n_frag : 0);
+  if (st) return st;
+  HIP_TRY(hipMemcpy(w.kernels, sp.kernels.data(), sp.kernels.size() * 4,
+                    hipMemcpyHostToDevice));
+  if (p->rt_tolerance_per_precursor && l->n_prec)
+    HIP_TRY(hipMemcpy(w.rt_tol_pp, p->rt_tolerance_per_precursor,
+                      l->n_prec * 4, hipMemcpyHostToDevice));
+  const Grid &g = global_grid();
+  const DeviceDia &dv = d->dev;
+  sp.dia = KDia{dv.grid,          dv.bucket,      (uint32_t)g.mz.size(),
+                g.base_key,       dv.rt,          (uint32_t)d->rt_index.size(),
+                d->delta_t,       (uint32_t)d->n_obs, dv.iso_lo,
+                dv.iso_hi,        dv.slice_starts, dv.entry_base,
+                dv.cycles,        dv.intens};
+  const DeviceLib &lv = l->dev;
+  sp.lib = KLib{(uint32_t)l->n_prec, lv.p_mz,  lv.p_rt, lv.f_start, lv.f_stop,
+                lv.f_mz,             lv.f_int, lv.f_type, lv.f_num};
+  sp.prm.mass_tol = p->mass_tolerance;
+  sp.prm.rt_tol = p->rt_tolerance;
+  sp.prm.rt_tol_pp = p->rt_tolerance_per_precursor ? w.rt_tol_pp : nullptr;
+  sp.prm.pad_cycles = (float)p->kernel_size;
+  sp.prm.top_k = (uint32_t)std::min<uint64_t>(p->top_k_fragments, 0xffffffffu);
+  sp.prm.cand_count = (uint32_t)cc;
+  const size_t klen = (size_t)std::ceil((float)p->kernel_size / 2.0f) * 2;
+  sp.prm.half = (uint32_t)(klen / 2);
+  sp.prm.klen = (uint32_t)klen;
+  sp.prm.per_window = p->sigma_per_window ? 1u : 0u;
+  sp.prm.kernels = w.kernels;
+  sp.plan = KPlan{w.rec, w.obs, w.sel, w.sel_mz, w.sel_w, w.rng, w.b0,
+                  (uint32_t)ks};
+  return ADSR_OK;
+}
+
+static adsr_status check_status(Workspace &w) {
+  uint32_t host[3] = {0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(host, w.small, 12, hipMemcpyDeviceToHost, w.stream));
+  HIP_TRY(hipStreamSynchronize(w.stream));
+  const uint32_t st = host[1];
+  if (st & kStTooManyFragments)
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "a precursor selects more than %d fragments", kMaxSel);
+  if (st & kStTooManyWindows)
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "a precursor lies in more than %d isolation windows",
+                     kObsMax);
+  if (st & kStWindowTooLong)
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "an RT window exceeds %d cycles", kLCapMax);
+  return ADSR_OK;
+}
+
+static adsr_status fetch(Workspace &w, const adsr_dia *d, const adsr_lib *l,
+                         const adsr_selection_params *p, adsr_candidates **out) {
+  uint32_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, w.small + 2, 4, hipMemcpyDeviceToHost,
+                         w.stream));
+  HIP_TRY(hipStreamSynchronize(w.stream));
+  if (l->n_prec == 0) total = 0;
+  std::vector<CandRow> rows(total);
+  if (total)
+    HIP_TRY(hipMemcpy(rows.data(), w.out, (size_t)total * sizeof(CandRow),
+                      hipMemcpyDeviceToHost));
+  auto *c = new adsr_candidates();
+  c->precursor_idx.resize(total);
+  c->rank.resize(total);
+  c->score.resize(total);
+  c->scan_center.assign(total, 0);
+  c->scan_start.assign(total, 0);
+  c->scan_stop.assign(total, 0);
+  c->cycle_center.resize(total);
+  c->cycle_start.resize(total);
+  c->cycle_stop.resize(total);
+  const uint64_t pl = p->peak_length, nrt = d->rt_index.size();
+  for (size_t i = 0; i < total; ++i) {
+    const CandRow &r = rows[i];
+    const uint64_t center = r.center;
+    c->precursor_idx[i] = l->precursor_idx[r.row];
+    c->rank[i] = r.rank;
+    c->score[i] = r.score;
+    c->cycle_center[i] = center;
+    c->cycle_start[i] = center - pl;  // usize arithmetic, mod.rs:176
+    c->cycle_stop[i] = std::min<uint64_t>(center + pl + 1, nrt);
+  }
+  *out = c;
+  return ADSR_OK;
+}
+
+static std::map<std::tuple<size_t, uint32_t, size_t>, LaunchShape> g_tuned;
+static std::mutex g_tuned_mu;
+
+}  // namespace adsr
+
+using namespace adsr;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" int adsr_device_available(void) { return device_present() ? 1 : 0; }
+
+extern "C" adsr_status adsr_backend_set(const char *name) {
+  clear_error();
+  if (!name || std::strcmp(name, "hip") != 0)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT,
+                     "Unknown backend '%s'. Available backends: [\"hip\"]",
+                     name ? name : "(null)");
+  if (!device_present())
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "Backend 'hip' is not available on this system");
+  g_backend_override.store(0);
+  return ADSR_OK;
+}
+
+extern "C" void adsr_backend_clear(void) { g_backend_override.store(-1); }
+
+static adsr_status write_name(char *out, size_t cap, const char *name) {
+  if (!out || cap == 0) return set_error(ADSR_ERR_INVALID_ARGUMENT, "no buffer");
+  std::snprintf(out, cap, "%s", name);
+  return ADSR_OK;
+}
+
+extern "C" adsr_status adsr_backend_current(char *out, size_t cap) {
+  clear_error();
+  return write_name(out, cap, device_present() ? "hip" : "none");
+}
+
+extern "C" adsr_status adsr_backend_optimal(char *out, size_t cap) {
+  clear_error();
+  return write_name(out, cap, device_present() ? "hip" : "none");
+}
+
+extern "C" adsr_status adsr_dia_upload(adsr_dia *d) {
+  clear_error();
+  if (!d) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  return upload_dia(d);
+}
+
+extern "C" adsr_status adsr_lib_upload(adsr_lib *l) {
+  clear_error();
+  if (!l) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  return upload_lib(l);
+}
+
+static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
+                                 const adsr_selection_params *p, int repeats,
+                                 int use_graph, int tpb, int ppw,
+                                 double *avg_ms, double *sel_ms,
+                                 adsr_candidates **out) {
+  clear_error();
+  if (!dc || !lc || !p || !out)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  adsr_dia *d = const_cast<adsr_dia *>(dc);
+  adsr_lib *l = const_cast<adsr_lib *>(lc);
+  adsr_status st = upload_dia(d);
+  if (st) return st;
+  st = upload_lib(l);
+  if (st) return st;
+  Workspace w;
+  SearchPlan sp;
+  st = build_search(d, l, p, w, sp);
+  if (st) {
+    ws_free(w);
+    return st;
+  }
+  const uint32_t n = (uint32_t)l->n_prec;
+  LaunchShape shape;
+  bool tune = false;
+  if (tpb == 64 || tpb == 128 || tpb == 256) {
+    shape.waves = tpb / 64;
+    shape.chunk = (ppw == 1 || ppw == 2 || ppw == 4) ? ppw : 1;
+  } else {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    auto key = std::make_tuple((size_t)n, sp.lcap, w.ks);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) shape = it->second;
+    else tune = repeats > 1;
+  }
+  auto run_once = [&](const LaunchShape &sh, bool timed_sel) -> adsr_status {
+    return enqueue(w, sp, n, sh, nullptr, 0, 0, 0, timed_sel);
+  };
+  if (tune) {
+    // autotune: {64,128,256} threads x {1,2,4} precursors per queue grab
+    float best = 1e30f;
+    LaunchShape best_shape;
+    for (int wv : {1, 2, 4})
+      for (int ch : {1, 2, 4}) {
+        LaunchShape sh{wv, ch};
+        st = run_once(sh, true);  // warm
+        if (st) break;
+        st = run_once(sh, true);
+        if (st) break;
+        if (hipStreamSynchronize(w.stream) != hipSuccess) {
+          st = set_error(ADSR_ERR_DEVICE, "synchronize failed");
+          break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, w.ev[2], w.ev[3]);
+        if (ms < best) {
+          best = ms;
+          best_shape = sh;
+        }
+      }
+    if (st) {
+      ws_free(w);
+      return st;
+    }
+    shape = best_shape;
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    g_tuned[std::make_tuple((size_t)n, sp.lcap, w.ks)] = shape;
+  }
+  // first run, eagerly, checks device-side limits
+  st = run_once(shape, true);
+  if (!st) st = check_status(w);
+  if (st) {
+    ws_free(w);
+    return st;
+  }
+  if (repeats > 1 || avg_ms) {
+    const int reps = std::max(repeats, 1);
+    if (use_graph) {
+      if (hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal) !=
+          hipSuccess) {
+        ws_free(w);
+        return set_error(ADSR_ERR_DEVICE, "graph capture failed to start");
+      }
+      st = run_once(shape, false);
+      hipError_t e = hipStreamEndCapture(w.stream, &w.graph_src);
+      if (st || e != hipSuccess ||
+          hipGraphInstantiate(&w.graph, w.graph_src, nullptr, nullptr, 0) !=
+              hipSuccess) {
+        ws_free(w);
+        return st ? st : set_error(ADSR_ERR_DEVICE, "graph capture failed");
+      }
+      // warm replay
+      if (hipGraphLaunch(w.graph, w.stream) != hipSuccess) {
+        ws_free(w);
+        return set_error(ADSR_ERR_DEVICE, "graph launch failed");
+      }
+    }
+    (void)hipEventRecord(w.ev[0], w.stream);
+    for (int r = 0; r < reps; ++r) {
+      if (use_graph) {
+        if (hipGraphLaunch(w.graph, w.stream) != hipSuccess) {
+          ws_free(w);
+          return set_error(ADSR_ERR_DEVICE, "graph launch failed");
+        }
+      } else {
+        st = run_once(shape, false);
+        if (st) {
+          ws_free(w);
+          return st;
+        }
+      }
+    }
+    (void)hipEventRecord(w.ev[1], w.stream);
+    (void)hipStreamSynchronize(w.stream);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, w.ev[0], w.ev[1]);
+    if (avg_ms) *avg_ms = ms / reps;
+    if (sel_ms) {
+      // one more eager run with events around the select kernel only
+      st = run_once(shape, true);
+      (void)hipStreamSynchronize(w.stream);
+      float s2 = 0;
+      (void)hipEventElapsedTime(&s2, w.ev[2], w.ev[3]);
+      *sel_ms = s2;
+    }
+  }
+  st = check_status(w);
+  if (!st) st = fetch(w, d, l, p, out);
+  ws_free(w);
+  return st;
+}
+
+extern "C" adsr_status adsr_select(const adsr_dia *d, const adsr_lib *l,
+                                   const adsr_selection_params *p,
+                                   adsr_candidates **out) {
+  return select_common(d, l, p, 1, 0, 0, 0, nullptr, nullptr, out);
+}
+
+extern "C" adsr_status adsr_select_timed(
+    const adsr_dia *d, const adsr_lib *l, const adsr_selection_params *p,
+    int repeats, int use_graph, int tpb, int ppw, double *avg_ms,
+    double *sel_ms, adsr_candidates **out) {
+  return select_common(d, l, p, repeats, use_graph, tpb, ppw, avg_ms, sel_ms,
+                       out);
+}
+
+extern "C" adsr_status adsr_select_scores(const adsr_dia *dc,
+                                          const adsr_lib *lc,
+                                          const adsr_selection_params *p,
+                                          size_t row_begin, size_t row_end,
+                                          size_t max_cycles, float *scores,
+                                          uint64_t *cstart, uint64_t *cstop) {
+  clear_error();
+  if (!dc || !lc || !p || !scores || !cstart || !cstop)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  adsr_dia *d = const_cast<adsr_dia *>(dc);
+  adsr_lib *l = const_cast<adsr_lib *>(lc);
+  if (row_end > l->n_prec || row_begin > row_end)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "row range out of bounds");
+  adsr_status st = upload_dia(d);
+  if (!st) st = upload_lib(l);
+  if (st) return st;
+  Workspace w;
+  SearchPlan sp;
+  st = build_search(d, l, p, w, sp);
+  if (st) {
+    ws_free(w);
+    return st;
+  }
+  const size_t nr = row_end - row_begin;
+  float *dbg = nullptr;
+  const size_t bytes = std::max<size_t>(nr * max_cycles, 1) * 4;
+  if (hipMalloc(&dbg, bytes) != hipSuccess) {
+    ws_free(w);
+    return set_error(ADSR_ERR_OUT_OF_MEMORY, "debug buffer");
+  }
+  std::vector<float> nanv(nr * max_cycles, __builtin_nanf(""));
+  (void)hipMemcpy(dbg, nanv.data(), nanv.size() * 4, hipMemcpyHostToDevice);
+  LaunchShape shape;
+  st = enqueue(w, sp, (uint32_t)l->n_prec, shape, dbg, (uint32_t)max_cycles,
+               (uint32_t)row_begin, (uint32_t)nr, false);
+  if (!st) st = check_status(w);
+  if (!st && nr) {
+    if (hipMemcpy(scores, dbg, nr * max_cycles * 4, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      st = set_error(ADSR_ERR_DEVICE, "copy of scores failed");
+    std::vector<uint4> rec(nr);
+    if (!st && hipMemcpy(rec.data(), w.rec + row_begin, nr * sizeof(uint4),
+                         hipMemcpyDeviceToHost) != hipSuccess)
+      st = set_error(ADSR_ERR_DEVICE, "copy of plan failed");
+    for (size_t i = 0; !st && i < nr; ++i) {
+      cstart[i] = rec[i].x;
+      cstop[i] = rec[i].y;
+    }
+  }
+  (void)hipFree(dbg);
+  ws_free(w);
+  return st;
+}
+
+extern "C" adsr_status adsr_select_plan(const adsr_dia *dc, const adsr_lib *lc,
+                                        const adsr_selection_params *p,
+                                        size_t row_begin, size_t row_end,
+                                        uint64_t *cstart, uint64_t *cstop,
+                                        uint32_t *n_selected,
+                                        uint64_t *selected, size_t stride) {
+  clear_error();
+  if (!dc || !lc || !p || !cstart || !cstop || !n_selected || !selected)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  adsr_dia *d = const_cast<adsr_dia *>(dc);
+  adsr_lib *l = const_cast<adsr_lib *>(lc);
+  if (row_end > l->n_prec || row_begin > row_end)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "row range out of bounds");
+  adsr_status st = upload_dia(d);
+  if (!st) st = upload_lib(l);
+  if (st) return st;
+  Workspace w;
+  SearchPlan sp;
+  st = build_search(d, l, p, w, sp);
+  if (!st)
+    st = enqueue(w, sp, (uint32_t)l->n_prec, LaunchShape(), nullptr, 0, 0, 0,
+                 false, true);
+  if (!st) st = check_status(w);
+  const size_t nr = row_end - row_begin;
+  if (!st && nr) {
+    std::vector<uint4> rec(nr);
+    std::vector<uint16_t> sel(nr * w.ks);
+    if (hipMemcpy(rec.data(), w.rec + row_begin, nr * sizeof(uint4),
+                  hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(sel.data(), w.sel + row_begin * w.ks, nr * w.ks * 2,
+                  hipMemcpyDeviceToHost) != hipSuccess)
+      st = set_error(ADSR_ERR_DEVICE, "copy of plan failed");
+    for (size_t i = 0; !st && i < nr; ++i) {
+      cstart[i] = rec[i].x;
+      cstop[i] = rec[i].y;
+      const uint32_t ns = rec[i].z & 0xffffu;
+      n_selected[i] = ns;
+      const uint64_t fs = l->frag_start[row_begin + i];
+      for (uint32_t j = 0; j < ns && j < stride; ++j)
+        selected[i * stride + j] = fs + sel[i * w.ks + j];
+    }
+  }
+  ws_free(w);
+  return st;
+}
+
+// ---- per-stage operations --------------------------------------------------
+template <class F>
+static adsr_status with_buffers(size_t in_bytes, size_t out_bytes, F &&fn) {
+  void *din = nullptr, *dout = nullptr;
+  HIP_TRY(hipMalloc(&din, std::max<size_t>(in_bytes, 16)));
+  if (hipMalloc(&dout, std::max<size_t>(out_bytes, 16)) != hipSuccess) {
+    (void)hipFree(din);
+    return set_error(ADSR_ERR_OUT_OF_MEMORY, "device buffer");
+  }
+  adsr_status st = fn(din, dout);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return st;
+}
+
+extern "C" adsr_status adsr_convolution(const float *kernel, size_t klen,
+                                        const float *xic, size_t rows,
+                                        size_t cols, float *out) {
+  clear_error();
+  adsr_status st = require_device();
+  if (st) return st;
+  if (klen == 0)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "empty kernel");
+  const size_t n = rows * cols;
+  if (n == 0) return ADSR_OK;
+  return with_buffers((n + klen) * 4, n * 4, [&](void *din, void *dout) {
+    float *dk = (float *)din, *dx = dk + klen;
+    HIP_TRY(hipMemcpy(dk, kernel, klen * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dx, xic, n * 4, hipMemcpyHostToDevice));
+    conv_dense_kernel<<<(uint32_t)((n + 255) / 256), 256>>>(
+        dk, (uint32_t)klen, dx, (uint32_t)rows, (uint32_t)cols, (float *)dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
+    return ADSR_OK;
+  });
+}
+
+extern "C" adsr_status adsr_axis_log_dot_product(const float *array,
+                                                 size_t rows, size_t cols,
+                                                 const float *weights,
+                                                 float *out) {
+  clear_error();
+  adsr_status st = require_device();
+  if (st) return st;
+  if (cols == 0) return ADSR_OK;
+  const size_t n = rows * cols;
+  return with_buffers((n + rows) * 4, cols * 4, [&](void *din, void *dout) {
+    float *da = (float *)din, *dw = da + n;
+    if (n) HIP_TRY(hipMemcpy(da, array, n * 4, hipMemcpyHostToDevice));
+    if (rows) HIP_TRY(hipMemcpy(dw, weights, rows * 4, hipMemcpyHostToDevice));
+    logdot_kernel<<<(uint32_t)((cols + 255) / 256), 256>>>(
+        da, (uint32_t)rows, (uint32_t)cols, dw, (float *)dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dout, cols * 4, hipMemcpyDeviceToHost));
+    return ADSR_OK;
+  });
+}
+
+extern "C" adsr_status adsr_device_logf(const float *x, size_t n, float *out) {
+  clear_error();
+  adsr_status st = require_device();
+  if (st) return st;
+  if (n == 0) return ADSR_OK;
+  return with_buffers(n * 4, n * 4, [&](void *din, void *dout) {
+    HIP_TRY(hipMemcpy(din, x, n * 4, hipMemcpyHostToDevice));
+    logf_kernel<<<(uint32_t)((n + 255) / 256), 256>>>((const float *)din, n,
+                                                      (float *)dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
+    return ADSR_OK;
+  });
+}
+
+extern "C" adsr_status adsr_find_local_maxima(const float *array, size_t n,
+                                              uint64_t offset,
+                                              uint64_t *indices, float *values,
+                                              size_t *n_out) {
+  clear_error();
+  adsr_status st = require_device();
+  if (st) return st;
+  if (!n_out) return set_error(ADSR_ERR_INVALID_ARGUMENT, "n_out is NULL");
+  const size_t cap = n;  // at most n/2 maxima
+  return with_buffers(n * 4, cap * 12 + 16, [&](void *din, void *dout) {
+    if (n) HIP_TRY(hipMemcpy(din, array, n * 4, hipMemcpyHostToDevice));
+    uint64_t *di = (uint64_t *)dout;
+    float *dv = (float *)(di + cap);
+    uint32_t *dc = (uint32_t *)(dv + cap);
+    maxima_kernel<<<1, 64>>>((const float *)din, (uint32_t)n, offset, di, dv,
+                             dc);
+    HIP_TRY(hipGetLastError());
+    uint32_t m = 0;
+    HIP_TRY(hipMemcpy(&m, dc, 4, hipMemcpyDeviceToHost));
+    if (m && indices) HIP_TRY(hipMemcpy(indices, di, m * 8, hipMemcpyDeviceToHost));
+    if (m && values) HIP_TRY(hipMemcpy(values, dv, m * 4, hipMemcpyDeviceToHost));
+    *n_out = m;
+    return ADSR_OK;
+  });
+}
+
+extern "C" adsr_status adsr_dense_xic(const adsr_dia *dc, float pmz,
+                                      uint64_t c0, uint64_t c1, float tol,
+                                      const float *fmz, size_t nf,
+                                      float *out) {
+  clear_error();
+  if (!dc) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (c1 < c0) return set_error(ADSR_ERR_INVALID_ARGUMENT, "cycle_stop < cycle_start");
+  adsr_dia *d = const_cast<adsr_dia *>(dc);
+  adsr_status st = upload_dia(d);
+  if (st) return st;
+  const size_t L = c1 - c0;
+  if (nf == 0 || L == 0) return ADSR_OK;
+  const Grid &g = global_grid();
+  const DeviceDia &dv = d->dev;
+  KDia kd{dv.grid,       dv.bucket,  (uint32_t)g.mz.size(), g.base_key,
+          dv.rt,         (uint32_t)d->rt_index.size(), d->delta_t,
+          (uint32_t)d->n_obs, dv.iso_lo, dv.iso_hi, dv.slice_starts,
+          dv.entry_base, dv.cycles,  dv.intens};
+  return with_buffers(nf * 4, nf * L * 4, [&](void *din, void *dout) {
+    HIP_TRY(hipMemcpy(din, fmz, nf * 4, hipMemcpyHostToDevice));
+    dense_xic_kernel<<<(uint32_t)((nf + 63) / 64), 64>>>(
+        kd, pmz, (uint32_t)c0, (uint32_t)c1, tol, (const float *)din,
+        (uint32_t)nf, (float *)dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dout, nf * L * 4, hipMemcpyDeviceToHost));
+    return ADSR_OK;
+  });
+}
+
+// Host instance of the device ln; used to check the device code path
+// against the platform libm on machines without a GPU.
+extern "C" float adsr_logf_host(float x) { return logf_exact(x, kHostLogfTab); }

@@ -1,0 +1,119 @@
+// adsr_internal.hpp -- shared host/device declarations of the gfx950
+// candidate-selection library (host builders in adsr_host.cpp, kernels and
+// launches in adsr_device.hip).
+#pragma once
+
+#include <atomic>
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/adsr.h"
+
+namespace adsr {
+
+// ---- m/z grid: src/mz_index/mod.rs:4-6 (RESOLUTION_PPM=1, 150..2000 Th) ----
+// Bin lookups use a table over the top bits of the f32 pattern: for a
+// positive float x, key = (bits(x) >> kBucketShift) - base, and
+// bucket[key] = number of grid points below the smallest float with that key.
+// One bucket spans 16 ulps while a 1-ppm bin spans at least 8.39 ulps, so a
+// bucket holds at most two grid points and every lookup ends after at most
+// two exact comparisons against the grid values.
+constexpr int kBucketShift = 4;
+
+struct Grid {
+  std::vector<float> mz;        // the exact f32 recurrence
+  std::vector<uint32_t> bucket;  // n_bucket + 1 entries
+  uint32_t base_key = 0;         // bits(mz[0]) >> kBucketShift
+  uint32_t n_bucket = 0;
+};
+const Grid &global_grid();
+// Number of grid points < x (lower bound), exact.
+uint32_t grid_lower_bound(const Grid &g, float x);
+// src/mz_index/mod.rs:55-90 find_closest_index, exact.
+uint32_t grid_find_closest(const Grid &g, float x);
+
+// ---- errors -----------------------------------------------------------------
+adsr_status set_error(adsr_status st, const char *fmt, ...);
+void clear_error();
+
+// ---- device-resident copies -------------------------------------------------
+struct DeviceDia {
+  bool ready = false;
+  int device = -1;
+  float *grid = nullptr;
+  uint32_t *bucket = nullptr;
+  float *rt = nullptr;
+  float *iso_lo = nullptr, *iso_hi = nullptr;
+  uint32_t *slice_starts = nullptr;  // n_obs * (n_grid + 1)
+  uint64_t *entry_base = nullptr;    // n_obs + 1
+  uint16_t *cycles = nullptr;
+  float *intens = nullptr;
+  size_t bytes = 0;
+};
+
+struct DeviceLib {
+  bool ready = false;
+  int device = -1;
+  uint64_t *p_idx = nullptr;
+  float *p_mz = nullptr, *p_rt = nullptr;
+  uint64_t *f_start = nullptr, *f_stop = nullptr;
+  float *f_mz = nullptr, *f_int = nullptr;
+  uint8_t *f_type = nullptr, *f_num = nullptr;
+  size_t bytes = 0;
+};
+
+void device_dia_free(DeviceDia &d);
+void device_lib_free(DeviceLib &l);
+
+}  // namespace adsr
+
+// ---- opaque handles ---------------------------------------------------------
+struct adsr_dia {
+  // src/dia_data/mod.rs:15-21
+  std::vector<float> rt_values;  // all spectrum_rt (copied)
+  std::vector<float> cycle;
+  size_t cycle_shape[4] = {0, 0, 0, 0};
+  // src/rt_index/mod.rs:11-14
+  std::vector<float> rt_index;
+  float delta_t = 1.0f;
+  // src/quadrupole_observation/mod.rs:12-25, one per delta_scan_idx,
+  // concatenated: observation o owns slice_starts[o*(B+1) .. (o+1)*(B+1))
+  // and entries [entry_base[o], entry_base[o+1]).
+  size_t n_obs = 0;
+  size_t n_grid = 0;
+  std::vector<float> iso_lo, iso_hi;
+  std::vector<uint64_t> num_cycles;
+  std::vector<uint32_t> slice_starts;
+  std::vector<uint64_t> entry_base;
+  std::vector<uint16_t> cycles;
+  std::vector<float> intens;
+  bool rt_sorted = true;
+  adsr::DeviceDia dev;
+  std::mutex mu;
+};
+
+struct adsr_lib {
+  // src/speclib_flat/mod.rs:6-70, precursor arrays sorted by precursor_idx
+  size_t n_prec = 0, n_frag = 0;
+  std::vector<uint64_t> precursor_idx;
+  std::vector<float> precursor_mz_library, precursor_mz, precursor_rt_library,
+      precursor_rt;
+  std::vector<uint8_t> precursor_naa;
+  std::vector<uint64_t> frag_start, frag_stop;
+  std::vector<float> fragment_mz_library, fragment_mz, fragment_intensity;
+  std::vector<uint8_t> fragment_cardinality, fragment_charge,
+      fragment_loss_type, fragment_number, fragment_position, fragment_type;
+  uint64_t max_raw_fragments = 0;
+  adsr::DeviceLib dev;
+  std::mutex mu;
+};
+
+struct adsr_candidates {
+  // src/candidate/entry.rs:5-16
+  std::vector<uint64_t> precursor_idx, rank, scan_center, scan_start,
+      scan_stop, cycle_center, cycle_start, cycle_stop;
+  std::vector<float> score;
+};

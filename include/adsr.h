@@ -1,0 +1,232 @@
+/*
+ * adsr.h -- C ABI of the MI355X (gfx950) implementation of the
+ * alphadia-search-rs candidate-selection path.
+ *
+ * Plain pointers and sizes only.  Every function returns an adsr_status;
+ * on failure adsr_last_error() returns a thread-local message.  Handles own
+ * host and device copies of their inputs; nothing borrowed outlives a call.
+ *
+ * Each entry point names the reference interface it replaces
+ * (MannLabs/alphadia-search-rs 1.0.1-dev0, path:line).
+ */
+#ifndef ADSR_H
+#define ADSR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum adsr_status {
+  ADSR_OK = 0,
+  ADSR_ERR_INVALID_ARGUMENT = 1, /* maps to ValueError */
+  ADSR_ERR_DEVICE = 2,           /* HIP failure, no gfx950 device */
+  ADSR_ERR_OUT_OF_MEMORY = 3,
+  ADSR_ERR_UNSUPPORTED = 4,      /* input beyond a documented device limit */
+  ADSR_ERR_INTERNAL = 5
+} adsr_status;
+
+typedef struct adsr_dia adsr_dia;               /* DIAData */
+typedef struct adsr_lib adsr_lib;               /* SpecLibFlat */
+typedef struct adsr_candidates adsr_candidates; /* CandidateCollection */
+
+/* SelectionParameters (src/peak_group_selection/parameters.rs:4-43) plus two
+ * optional extensions; with both NULL the result equals the reference.    */
+typedef struct adsr_selection_params {
+  float fwhm_rt;            /* default 3.0 */
+  uint64_t kernel_size;     /* default 20 */
+  uint64_t peak_length;     /* default 5 */
+  float mass_tolerance;     /* ppm, default 7.0 */
+  float rt_tolerance;       /* seconds, default 200.0 */
+  uint64_t candidate_count; /* default 3 */
+  uint64_t top_k_fragments; /* default 12 */
+  /* extension: per-precursor rt_tolerance in library (precursor_idx-sorted)
+   * order, length = number of precursors; NULL = rt_tolerance for all.    */
+  const float *rt_tolerance_per_precursor;
+  /* extension: Gaussian sigma (cycles) per quadrupole observation, used for
+   * precursors whose first valid observation is that window; NULL = the
+   * kernel from fwhm_rt for every precursor.                               */
+  const float *sigma_per_window;
+  uint64_t n_sigma_per_window;
+} adsr_selection_params;
+
+/* Fills the reference defaults (parameters.rs:24-43). */
+void adsr_selection_params_default(adsr_selection_params *p);
+
+const char *adsr_last_error(void);
+
+/* ---- device / backend registry (src/simd/mod.rs:80-134, src/lib.rs:53-72) */
+/* 1 if a gfx950 device is visible and usable. */
+int adsr_device_available(void);
+/* Backend names: "hip".  set fails with INVALID_ARGUMENT for unknown names
+ * and UNSUPPORTED for a known but unavailable backend, like
+ * simd::set_backend (src/simd/mod.rs:97-119).                              */
+adsr_status adsr_backend_set(const char *name);
+void adsr_backend_clear(void);
+/* Writes the current / optimal backend name ("hip" or "none"). */
+adsr_status adsr_backend_current(char *out, size_t cap);
+adsr_status adsr_backend_optimal(char *out, size_t cap);
+
+/* ---- DIAData (src/dia_data/mod.rs:41-135) ------------------------------ */
+/* DIAData.from_arrays (src/dia_data/mod.rs:43-72): builds the 1-ppm binned
+ * CSR per quadrupole window (src/dia_data_builder/mod.rs:11-139) on host
+ * threads.  cycle is copied (shape cycle_shape[4]) and returned unchanged. */
+adsr_status adsr_dia_create(const int64_t *spectrum_delta_scan_idx,
+                            const float *isolation_lower_mz,
+                            const float *isolation_upper_mz,
+                            const int64_t *spectrum_peak_start_idx,
+                            const int64_t *spectrum_peak_stop_idx,
+                            const int64_t *spectrum_cycle_idx,
+                            const float *spectrum_rt, size_t n_spectra,
+                            const float *peak_mz, const float *peak_intensity,
+                            size_t n_peaks, const float *cycle,
+                            const size_t cycle_shape[4], adsr_dia **out);
+/* Copies the CSR and indices into device memory (HBM); idempotent. */
+adsr_status adsr_dia_upload(adsr_dia *d);
+void adsr_dia_free(adsr_dia *d);
+/* num_observations getter (src/dia_data/mod.rs:74-77) */
+size_t adsr_dia_num_observations(const adsr_dia *d);
+/* get_valid_observations (src/dia_data/mod.rs:79-87) */
+adsr_status adsr_dia_valid_observations(const adsr_dia *d, float mz,
+                                        uint64_t *out, size_t cap,
+                                        size_t *n_out);
+/* memory_footprint_bytes (src/dia_data/mod.rs:90-108) */
+uint64_t adsr_dia_memory_footprint_bytes(const adsr_dia *d);
+/* rt_values getter (src/dia_data/mod.rs:128-131): all spectrum_rt */
+size_t adsr_dia_rt_values(const adsr_dia *d, float *out, size_t cap);
+/* cycle getter (src/dia_data/mod.rs:133-135) */
+void adsr_dia_cycle_shape(const adsr_dia *d, size_t shape[4]);
+void adsr_dia_cycle_copy(const adsr_dia *d, float *out);
+/* RT index (src/rt_index/mod.rs:53-75): values and mean delta_t */
+size_t adsr_dia_rt_index(const adsr_dia *d, float *out, size_t cap,
+                         float *delta_t);
+/* Read access to one observation's CSR (src/quadrupole_observation/mod.rs:
+ * 12-25).  Any output pointer may be NULL; sizes are always written.       */
+adsr_status adsr_dia_observation(const adsr_dia *d, size_t obs,
+                                 float isolation_window[2],
+                                 uint64_t *num_cycles, uint32_t *slice_starts,
+                                 size_t *n_slice_starts,
+                                 uint16_t *cycle_indices, float *intensities,
+                                 size_t *n_entries);
+
+/* ---- m/z grid (src/mz_index/mod.rs:8-127) ----------------------------- */
+/* Global 1-ppm grid over 150-2000 Th; copies up to cap values. */
+size_t adsr_mz_grid(float *out, size_t cap);
+
+/* ---- SpecLibFlat (src/speclib_flat/mod.rs:98-199) --------------------- */
+/* SpecLibFlat.from_arrays: stable sort of the precursor arrays by
+ * precursor_idx; fragment arrays keep their order.                        */
+adsr_status adsr_lib_create(
+    size_t n_precursors, const uint64_t *precursor_idx,
+    const float *precursor_mz_library, const float *precursor_mz,
+    const float *precursor_rt_library, const float *precursor_rt,
+    const uint8_t *precursor_naa, const uint64_t *flat_frag_start_idx,
+    const uint64_t *flat_frag_stop_idx, size_t n_fragments,
+    const float *fragment_mz_library, const float *fragment_mz,
+    const float *fragment_intensity, const uint8_t *fragment_cardinality,
+    const uint8_t *fragment_charge, const uint8_t *fragment_loss_type,
+    const uint8_t *fragment_number, const uint8_t *fragment_position,
+    const uint8_t *fragment_type, adsr_lib **out);
+adsr_status adsr_lib_upload(adsr_lib *l);
+void adsr_lib_free(adsr_lib *l);
+size_t adsr_lib_num_precursors(const adsr_lib *l);
+size_t adsr_lib_num_fragments(const adsr_lib *l);
+
+/* ---- PeakGroupSelection.search (src/peak_group_selection/mod.rs:87-128) */
+/* Runs the fused gfx950 pipeline on the device-resident data and returns
+ * the candidate table in library order.  Uploads lazily if needed.        */
+adsr_status adsr_select(const adsr_dia *d, const adsr_lib *l,
+                        const adsr_selection_params *p,
+                        adsr_candidates **out);
+
+/* Timing-oriented variant: runs the pipeline `repeats` times (captured in a
+ * hipGraph after the first run when use_graph != 0) and reports the average
+ * device time per run in *avg_ms; the table of the last run is returned.
+ * threads_per_block in {64,128,256} and precursors_per_wave in {1,2,4}
+ * select the launch shape; 0 = autotuned choice.                          */
+adsr_status adsr_select_timed(const adsr_dia *d, const adsr_lib *l,
+                              const adsr_selection_params *p, int repeats,
+                              int use_graph, int threads_per_block,
+                              int precursors_per_wave, double *avg_ms,
+                              double *select_kernel_avg_ms,
+                              adsr_candidates **out);
+
+/* Diagnostic: the fused kernel's per-cycle score vector for library rows
+ * [row_begin, row_end), written densely (stride max_cycles, NaN-padded),
+ * with each row's cycle_start and cycle_stop.                             */
+adsr_status adsr_select_scores(const adsr_dia *d, const adsr_lib *l,
+                               const adsr_selection_params *p,
+                               size_t row_begin, size_t row_end,
+                               size_t max_cycles, float *scores,
+                               uint64_t *cycle_start, uint64_t *cycle_stop);
+
+/* Diagnostic: the device plan for library rows [row_begin, row_end): the
+ * RT cycle window (src/rt_index/mod.rs:89-138) and the selected fragments
+ * (absolute indices into the fragment arrays, stride `stride`) in the order
+ * of get_precursor_filtered(i, true, true, top_k_fragments)
+ * (src/speclib_flat/mod.rs:388-456).                                      */
+adsr_status adsr_select_plan(const adsr_dia *d, const adsr_lib *l,
+                             const adsr_selection_params *p, size_t row_begin,
+                             size_t row_end, uint64_t *cycle_start,
+                             uint64_t *cycle_stop, uint32_t *n_selected,
+                             uint64_t *selected, size_t stride);
+
+/* ---- CandidateCollection (src/candidate/entry.rs:41-193) -------------- */
+size_t adsr_candidates_len(const adsr_candidates *c);
+/* to_arrays (entry.rs:124-173): nine columns in reference order. */
+adsr_status adsr_candidates_export(const adsr_candidates *c,
+                                   uint64_t *precursor_idx, uint64_t *rank,
+                                   float *score, uint64_t *scan_center,
+                                   uint64_t *scan_start, uint64_t *scan_stop,
+                                   uint64_t *cycle_center,
+                                   uint64_t *cycle_start,
+                                   uint64_t *cycle_stop);
+/* from_arrays (entry.rs:70-120) */
+adsr_status adsr_candidates_from_arrays(
+    size_t n, const uint64_t *precursor_idx, const uint64_t *rank,
+    const float *score, const uint64_t *scan_center,
+    const uint64_t *scan_start, const uint64_t *scan_stop,
+    const uint64_t *cycle_center, const uint64_t *cycle_start,
+    const uint64_t *cycle_stop, adsr_candidates **out);
+void adsr_candidates_free(adsr_candidates *c);
+
+/* ---- per-stage device kernels (SimdBackend ops, src/simd/mod.rs:11-39) --
+ * Host in/out buffers; the library stages them through HBM.               */
+/* GaussianKernel::new (src/kernel/mod.rs:13-30); returns the length. */
+size_t adsr_gaussian_kernel(float fwhm_rt, float sigma_scale_rt,
+                            uint64_t kernel_width, float rt_resolution,
+                            float *out, size_t cap);
+/* gaussian_kernel_1d (src/kernel/mod.rs:32-56) for an explicit sigma in
+ * cycles, as used by the sigma_per_window extension.                      */
+size_t adsr_gaussian_kernel_sigma(float sigma, uint64_t kernel_width,
+                                  float *out, size_t cap);
+/* convolution (src/convolution/mod.rs:10-12 -> scalar.rs:5-56) */
+adsr_status adsr_convolution(const float *kernel, size_t kernel_len,
+                             const float *xic, size_t rows, size_t cols,
+                             float *out);
+/* axis_log_dot_product (src/score/mod.rs:15-17 -> scalar.rs:4-25) */
+adsr_status adsr_axis_log_dot_product(const float *array, size_t rows,
+                                      size_t cols, const float *weights,
+                                      float *out);
+/* find_local_maxima (src/peak_group_selection/mod.rs:23-64); values sorted
+ * descending, ties in ascending index order.                              */
+adsr_status adsr_find_local_maxima(const float *array, size_t n,
+                                   uint64_t offset, uint64_t *indices,
+                                   float *values, size_t *n_out);
+/* DenseXICObservation::new (src/dense_xic_observation/mod.rs:30-64) on the
+ * device; out is [nf, cycle_stop - cycle_start] row-major.                */
+adsr_status adsr_dense_xic(const adsr_dia *d, float precursor_mz,
+                           uint64_t cycle_start, uint64_t cycle_stop,
+                           float mass_tolerance, const float *fragment_mz,
+                           size_t nf, float *out);
+/* Device ln(x) as used by the score; for exhaustive checks against libm. */
+adsr_status adsr_device_logf(const float *x, size_t n, float *out);
+/* The same ln evaluated by the host CPU (identical code and constants). */
+float adsr_logf_host(float x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADSR_H */

@@ -1,0 +1,271 @@
+"""ctypes binding of the CPU restatement in oracle/adsr_oracle.c.
+
+TEST INFRASTRUCTURE: used by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg as the checker.  The product (libadsr.so and the
+alphadia_search_rs package) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+vp = C.c_void_p
+sz = C.c_size_t
+f32 = C.c_float
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("fwhm_rt", f32), ("kernel_size", sz), ("peak_length", sz),
+        ("mass_tolerance", f32), ("rt_tolerance", f32),
+        ("candidate_count", sz), ("top_k_fragments", sz),
+        ("rt_tolerance_per_precursor", vp), ("sigma_per_window", vp),
+    ]
+
+
+class OrcCands(C.Structure):
+    _fields_ = [("n", sz), ("precursor_idx", vp), ("rank", vp),
+                ("cycle_center", vp), ("cycle_start", vp), ("cycle_stop", vp),
+                ("score", vp)]
+
+
+class OrcObs(C.Structure):
+    _fields_ = [("isolation_window", f32 * 2), ("num_cycles", sz),
+                ("slice_starts", vp), ("cycle_indices", vp),
+                ("intensities", vp), ("n_entries", sz)]
+
+
+class OrcDia(C.Structure):
+    _fields_ = [("n_obs", sz), ("obs", C.POINTER(OrcObs)), ("rt", vp),
+                ("n_rt", sz), ("delta_t", f32), ("grid", vp), ("n_grid", sz)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built (make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+
+        def sig(name, res, *args):
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = list(args)
+
+        sig("orc_ppm_index", sz, f32, f32, f32, vp, sz)
+        sig("orc_global_grid", vp, C.POINTER(sz))
+        sig("orc_find_closest_index", sz, vp, sz, f32)
+        sig("orc_mz_range_indices", None, vp, sz, f32, f32, C.POINTER(sz), C.POINTER(sz))
+        sig("orc_rust_bsearch_f32", C.c_int, vp, sz, f32, C.POINTER(sz))
+        sig("orc_rust_bsearch_u16", C.c_int, vp, sz, C.c_uint16, C.POINTER(sz))
+        sig("orc_mean_delta_t", f32, vp, sz)
+        sig("orc_cycle_idx_limits", None, vp, sz, f32, f32, f32, sz, C.POINTER(sz), C.POINTER(sz))
+        sig("orc_dia_build", C.POINTER(OrcDia), vp, vp, vp, vp, vp, vp, vp, sz, vp, vp, sz, C.c_int)
+        sig("orc_dia_free", None, C.POINTER(OrcDia))
+        sig("orc_valid_observations", sz, C.POINTER(OrcDia), f32, vp)
+        sig("orc_fill_xic_slice_raw", None, vp, sz, vp, vp, vp, vp, sz, sz, f32, f32)
+        sig("orc_dense_xic", None, C.POINTER(OrcDia), f32, sz, sz, f32, vp, sz, vp)
+        sig("orc_gaussian_kernel", sz, f32, f32, sz, f32, vp, sz)
+        sig("orc_gaussian_kernel_sigma", sz, f32, sz, vp, sz)
+        sig("orc_ndarray_sum", f32, vp, sz)
+        sig("orc_convolution", None, vp, sz, vp, sz, sz, vp)
+        sig("orc_axis_log_dot_product", None, vp, sz, sz, vp, vp)
+        sig("orc_find_local_maxima", sz, vp, sz, sz, vp, vp)
+        sig("orc_filter_sort_fragments", sz, sz, vp, vp, vp, vp, C.c_int, C.c_int, sz, vp)
+        sig("orc_lib_build", vp, sz, vp, vp, vp, vp, vp, sz, vp, vp, vp, vp)
+        sig("orc_lib_free", None, vp)
+        sig("orc_select", C.POINTER(OrcCands), C.POINTER(OrcDia), vp, C.POINTER(OrcParams), sz, sz, C.c_int)
+        sig("orc_cands_free", None, C.POINTER(OrcCands))
+        sig("orc_select_one", C.c_int, C.POINTER(OrcDia), vp, C.POINTER(OrcParams), sz,
+            C.POINTER(sz), C.POINTER(sz), C.POINTER(sz), vp, vp, vp, vp, vp, sz)
+        sig("orc_logf", f32, f32)
+        _lib = L
+    return _lib
+
+
+def p(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+def c32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def grid() -> np.ndarray:
+    n = sz()
+    ptr_ = lib().orc_global_grid(C.byref(n))
+    return np.ctypeslib.as_array(C.cast(ptr_, C.POINTER(f32)), (n.value,)).copy()
+
+
+class Dia:
+    """Oracle DIAData built from the 10 from_arrays inputs."""
+
+    def __init__(self, arrays, nthreads: int = 0):
+        d, lo, hi, ps, pe, cy, rt, mz, it, _cyc = arrays
+        self._keep = [np.ascontiguousarray(a) for a in (d, lo, hi, ps, pe, cy, rt, mz, it)]
+        k = self._keep
+        nt = nthreads or (os.cpu_count() or 1)
+        self.h = lib().orc_dia_build(p(k[0]), p(k[1]), p(k[2]), p(k[3]), p(k[4]), p(k[5]),
+                                     p(k[6]), len(k[0]), p(k[7]), p(k[8]), len(k[7]), nt)
+        self._keep = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_dia_free(self.h)
+            self.h = None
+
+    @property
+    def n_obs(self):
+        return self.h.contents.n_obs
+
+    def rt(self):
+        d = self.h.contents
+        return np.ctypeslib.as_array(C.cast(d.rt, C.POINTER(f32)), (d.n_rt,)).copy() if d.n_rt else np.zeros(0, np.float32)
+
+    def observation(self, i):
+        d = self.h.contents
+        o = d.obs[i]
+        ss = np.ctypeslib.as_array(C.cast(o.slice_starts, C.POINTER(C.c_uint32)), (d.n_grid + 1,)).copy()
+        n = o.n_entries
+        ci = np.ctypeslib.as_array(C.cast(o.cycle_indices, C.POINTER(C.c_uint16)), (n,)).copy() if n else np.zeros(0, np.uint16)
+        it = np.ctypeslib.as_array(C.cast(o.intensities, C.POINTER(f32)), (n,)).copy() if n else np.zeros(0, np.float32)
+        return {"isolation_window": np.array(o.isolation_window[:], dtype=np.float32),
+                "num_cycles": o.num_cycles, "slice_starts": ss,
+                "cycle_indices": ci, "intensities": it}
+
+    def valid_observations(self, mz):
+        out = np.zeros(max(self.n_obs, 1), dtype=np.uint64)
+        n = lib().orc_valid_observations(self.h, float(mz), p(out))
+        return [int(x) for x in out[:n]]
+
+    def dense_xic(self, precursor_mz, c0, c1, tol, frag_mz):
+        fm = c32(frag_mz)
+        out = np.zeros((len(fm), c1 - c0), dtype=np.float32)
+        lib().orc_dense_xic(self.h, float(precursor_mz), c0, c1, float(tol), p(fm), len(fm), p(out))
+        return out
+
+
+class Lib:
+    def __init__(self, arrays):
+        (pidx, _pmzl, pmz, _prtl, prt, _naa, fs, fe, _fmzl, fmz, fint,
+         _card, _chg, _loss, fnum, _pos, ftype) = arrays
+        k = [np.ascontiguousarray(a) for a in (pidx, pmz, prt, fs, fe, fmz, fint, fnum, ftype)]
+        self.h = lib().orc_lib_build(len(k[0]), p(k[0]), p(k[1]), p(k[2]), p(k[3]), p(k[4]),
+                                     len(k[5]), p(k[5]), p(k[6]), p(k[7]), p(k[8]))
+        self.n_prec = len(k[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_lib_free(self.h)
+            self.h = None
+
+
+def params(fwhm_rt=3.0, kernel_size=20, peak_length=5, mass_tolerance=7.0,
+           rt_tolerance=200.0, candidate_count=3, top_k_fragments=12,
+           rt_tolerance_per_precursor=None, sigma_per_window=None):
+    prm = OrcParams(fwhm_rt, kernel_size, peak_length, mass_tolerance, rt_tolerance,
+                    candidate_count, top_k_fragments, None, None)
+    keep = []
+    if rt_tolerance_per_precursor is not None:
+        a = c32(rt_tolerance_per_precursor)
+        keep.append(a)
+        prm.rt_tolerance_per_precursor = p(a)
+    if sigma_per_window is not None:
+        a = c32(sigma_per_window)
+        keep.append(a)
+        prm.sigma_per_window = p(a)
+    prm._keep = keep
+    return prm
+
+
+def select(dia: Dia, library: Lib, prm: OrcParams, row_begin=0, row_end=None, nthreads=0):
+    """Candidate table (9 columns like CandidateCollection.to_arrays)."""
+    if row_end is None:
+        row_end = library.n_prec
+    nt = nthreads or (os.cpu_count() or 1)
+    c = lib().orc_select(dia.h, library.h, C.byref(prm), row_begin, row_end, nt)
+    n = c.contents.n
+
+    def col(ptr_, ct, dt):
+        if n == 0:
+            return np.zeros(0, dtype=dt)
+        return np.ctypeslib.as_array(C.cast(ptr_, C.POINTER(ct)), (n,)).astype(dt, copy=True)
+
+    cc = c.contents
+    out = (col(cc.precursor_idx, C.c_uint64, np.uint64), col(cc.rank, C.c_uint64, np.uint64),
+           col(cc.score, f32, np.float32), np.zeros(n, np.uint64), np.zeros(n, np.uint64),
+           np.zeros(n, np.uint64), col(cc.cycle_center, C.c_uint64, np.uint64),
+           col(cc.cycle_start, C.c_uint64, np.uint64), col(cc.cycle_stop, C.c_uint64, np.uint64))
+    lib().orc_cands_free(c)
+    return out
+
+
+def select_one(dia: Dia, library: Lib, prm: OrcParams, row: int, max_cells=1 << 20):
+    """Intermediate arrays of one precursor: dict or None (empty RT range)."""
+    c0, c1, nf = sz(), sz(), sz()
+    k = max(int(prm.top_k_fragments) if prm.top_k_fragments < (1 << 20) else 1024, 1)
+    fm = np.zeros(k + 1024, np.float32)
+    fw = np.zeros(k + 1024, np.float32)
+    xic = np.zeros(max_cells, np.float32)
+    conv = np.zeros(max_cells, np.float32)
+    score = np.zeros(max_cells, np.float32)
+    ok = lib().orc_select_one(dia.h, library.h, C.byref(prm), row, C.byref(c0), C.byref(c1),
+                              C.byref(nf), p(fm), p(fw), p(xic), p(conv), p(score), max_cells)
+    if ok == 0:
+        return None
+    if ok < 0:
+        raise ValueError("intermediate arrays exceed max_cells")
+    L = c1.value - c0.value
+    F = nf.value
+    return {"cycle_start": c0.value, "cycle_stop": c1.value, "fragment_mz": fm[:F].copy(),
+            "fragment_intensity": fw[:F].copy(), "xic": xic[:F * L].reshape(F, L).copy(),
+            "conv": conv[:F * L].reshape(F, L).copy(), "score": score[:L].copy()}
+
+
+def filter_sort_fragments(mz, intensity, number, ftype, non_zero=True, filter_y1=True, top_k=None):
+    mz, it = c32(mz), c32(intensity)
+    num = np.ascontiguousarray(number, dtype=np.uint8)
+    ty = np.ascontiguousarray(ftype, dtype=np.uint8)
+    out = np.zeros(max(len(mz), 1), dtype=np.uint64)
+    k = (1 << 64) - 1 if top_k is None else int(top_k)
+    n = lib().orc_filter_sort_fragments(len(mz), p(mz), p(it), p(num), p(ty),
+                                        int(non_zero), int(filter_y1), k, p(out))
+    return out[:n].astype(np.int64)
+
+
+def gaussian_kernel(fwhm_rt, sigma_scale_rt, kernel_width, rt_resolution):
+    out = np.zeros(kernel_width + 2, np.float32)
+    n = lib().orc_gaussian_kernel(fwhm_rt, sigma_scale_rt, kernel_width, rt_resolution, p(out), len(out))
+    return out[:n].copy()
+
+
+def convolution(kernel, xic):
+    k, x = c32(kernel), c32(xic)
+    rows, cols = x.shape
+    out = np.zeros_like(x)
+    lib().orc_convolution(p(k), len(k), p(x), rows, cols, p(out))
+    return out
+
+
+def axis_log_dot_product(a, w):
+    a, w = c32(a), c32(w)
+    rows, cols = a.shape
+    out = np.zeros(cols, np.float32)
+    lib().orc_axis_log_dot_product(p(a), rows, cols, p(w), p(out))
+    return out
+
+
+def find_local_maxima(a, offset):
+    a = c32(a)
+    idx = np.zeros(max(len(a), 1), np.uint64)
+    val = np.zeros(max(len(a), 1), np.float32)
+    n = lib().orc_find_local_maxima(p(a), len(a), offset, p(idx), p(val))
+    return idx[:n].astype(np.int64), val[:n].copy()

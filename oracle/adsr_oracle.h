@@ -1,0 +1,170 @@
+/*
+ * adsr_oracle.h -- CPU restatement of alphadia-search-rs candidate selection.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path and the timed CPU baseline.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product library never links it.
+ *
+ * Every function restates one function of the reference (Rust crate
+ * alphadia_search_rs 1.0.1-dev0) and cites the file:line it follows.  All
+ * float arithmetic is f32, evaluated left to right with one rounding per
+ * operation and no FMA contraction (Rust semantics; build with
+ * -ffp-contract=off).  ln/exp are glibc logf/expf, the libm the Rust binary
+ * calls on x86-64 Linux.
+ */
+#ifndef ADSR_ORACLE_H
+#define ADSR_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- m/z grid: src/mz_index/mod.rs ---------------------------------- */
+size_t orc_ppm_index(float resolution_ppm, float mz_start, float mz_end,
+                     float *out, size_t cap);
+size_t orc_find_closest_index(const float *grid, size_t n, float mz);
+void orc_mz_range_indices(const float *grid, size_t n, float lower_mz,
+                          float upper_mz, size_t *start, size_t *end);
+
+/* ---- Rust core slice::binary_search_by (last-equal semantics) -------- */
+/* returns 1 for Ok(idx), 0 for Err(idx) */
+int orc_rust_bsearch_f32(const float *v, size_t n, float key, size_t *idx);
+int orc_rust_bsearch_u16(const uint16_t *v, size_t n, uint16_t key,
+                         size_t *idx);
+
+/* ---- RT index: src/rt_index/mod.rs ---------------------------------- */
+float orc_mean_delta_t(const float *rt, size_t n);
+void orc_cycle_idx_limits(const float *rt, size_t n, float delta_t,
+                          float precursor_rt, float rt_tolerance,
+                          size_t num_cycles_padding, size_t *lower,
+                          size_t *upper);
+
+/* ---- DIA data: src/dia_data*, src/quadrupole_observation ------------ */
+typedef struct orc_obs {
+  float isolation_window[2];
+  size_t num_cycles;
+  uint32_t *slice_starts; /* n_grid + 1 */
+  uint16_t *cycle_indices;
+  float *intensities;
+  size_t n_entries;
+} orc_obs;
+
+typedef struct orc_dia {
+  size_t n_obs;
+  orc_obs *obs;
+  float *rt; /* RT index values (delta_scan_idx == 0 spectra) */
+  size_t n_rt;
+  float delta_t;
+  const float *grid; /* global 1-ppm grid (shared) */
+  size_t n_grid;
+} orc_dia;
+
+const float *orc_global_grid(size_t *n);
+orc_dia *orc_dia_build(const int64_t *spectrum_delta_scan_idx,
+                       const float *isolation_lower_mz,
+                       const float *isolation_upper_mz,
+                       const int64_t *spectrum_peak_start_idx,
+                       const int64_t *spectrum_peak_stop_idx,
+                       const int64_t *spectrum_cycle_idx,
+                       const float *spectrum_rt, size_t n_spectra,
+                       const float *peak_mz, const float *peak_intensity,
+                       size_t n_peaks, int nthreads);
+void orc_dia_free(orc_dia *d);
+size_t orc_valid_observations(const orc_dia *d, float precursor_mz,
+                              size_t *out);
+void orc_fill_xic_slice_raw(const float *grid, size_t n_grid,
+                            const uint32_t *slice_starts,
+                            const uint16_t *cycle_indices,
+                            const float *intensities, float *dense_xic,
+                            size_t cycle_start_idx, size_t cycle_stop_idx,
+                            float mass_tolerance, float mz);
+void orc_fill_xic_slice(const orc_dia *d, size_t obs_idx, float *dense_xic,
+                        size_t cycle_start_idx, size_t cycle_stop_idx,
+                        float mass_tolerance, float mz);
+void orc_dense_xic(const orc_dia *d, float precursor_mz,
+                   size_t cycle_start_idx, size_t cycle_stop_idx,
+                   float mass_tolerance, const float *fragment_mz, size_t nf,
+                   float *out);
+
+/* ---- kernel / convolution / score / maxima -------------------------- */
+size_t orc_gaussian_kernel(float fwhm_rt, float sigma_scale_rt,
+                           size_t kernel_width, float rt_resolution,
+                           float *out, size_t cap);
+size_t orc_gaussian_kernel_sigma(float sigma, size_t kernel_width, float *out,
+                                 size_t cap);
+float orc_ndarray_sum(const float *x, size_t n);
+void orc_convolution(const float *kernel, size_t kernel_len, const float *xic,
+                     size_t rows, size_t cols, float *out);
+void orc_axis_log_dot_product(const float *a, size_t rows, size_t cols,
+                              const float *w, float *out);
+size_t orc_find_local_maxima(const float *a, size_t n, size_t offset,
+                             size_t *indices, float *values);
+
+/* ---- fragment filtering: src/speclib_flat/mod.rs:240-345 ------------ */
+size_t orc_filter_sort_fragments(size_t n, const float *fragment_mz,
+                                 const float *fragment_intensity,
+                                 const uint8_t *fragment_number,
+                                 const uint8_t *fragment_type, int non_zero,
+                                 int filter_y1_ions, size_t top_k,
+                                 size_t *out_order);
+
+/* ---- spectral library: src/speclib_flat/mod.rs:98-188 --------------- */
+typedef struct orc_lib {
+  size_t n_prec, n_frag;
+  uint64_t *precursor_idx;
+  float *precursor_mz, *precursor_rt;
+  uint64_t *frag_start, *frag_stop;
+  float *fragment_mz, *fragment_intensity;
+  uint8_t *fragment_number, *fragment_type;
+} orc_lib;
+
+orc_lib *orc_lib_build(size_t n_prec, const uint64_t *precursor_idx,
+                       const float *precursor_mz, const float *precursor_rt,
+                       const uint64_t *flat_frag_start_idx,
+                       const uint64_t *flat_frag_stop_idx, size_t n_frag,
+                       const float *fragment_mz,
+                       const float *fragment_intensity,
+                       const uint8_t *fragment_number,
+                       const uint8_t *fragment_type);
+void orc_lib_free(orc_lib *l);
+
+/* ---- selection: src/peak_group_selection/mod.rs:87-195 -------------- */
+typedef struct orc_params {
+  float fwhm_rt;
+  size_t kernel_size;
+  size_t peak_length;
+  float mass_tolerance;
+  float rt_tolerance;
+  size_t candidate_count;
+  size_t top_k_fragments;
+  const float *rt_tolerance_per_precursor; /* NULL or n_prec (library order) */
+  const float *sigma_per_window;           /* NULL or n_obs */
+} orc_params;
+
+typedef struct orc_cands {
+  size_t n;
+  uint64_t *precursor_idx, *rank, *cycle_center, *cycle_start, *cycle_stop;
+  float *score;
+} orc_cands;
+
+orc_cands *orc_select(const orc_dia *d, const orc_lib *l, const orc_params *p,
+                      size_t row_begin, size_t row_end, int nthreads);
+void orc_cands_free(orc_cands *c);
+
+/* One precursor, with intermediate arrays exposed for stage-wise checks.
+ * Returns 0 when the RT range is empty (reference returns None).
+ * xic/conv must hold top_k*L floats, score L floats.                     */
+int orc_select_one(const orc_dia *d, const orc_lib *l, const orc_params *p,
+                   size_t row, size_t *cycle_start, size_t *cycle_stop,
+                   size_t *n_frag, float *frag_mz, float *frag_w, float *xic,
+                   float *conv, float *score, size_t cap_cells);
+
+float orc_logf(float x); /* glibc logf, as called by the reference */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

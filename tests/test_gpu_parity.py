@@ -1,0 +1,203 @@
+"""HIP path vs the CPU oracle, through the C ABI (libadsr.so).
+
+Integer, index and candidate columns must match bit for bit; f32 XIC sums,
+convolutions and scores are compared bit for bit as well (the device
+evaluates the reference's operations in the reference's order, and ln is
+bit-identical to glibc logf).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import synthetic
+from conftest import assert_tables_equal, oracle_params_for, params_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ads():
+    import alphadia_search_rs as m
+    import alphadia_search_rs._native as N
+    if not N.device_available():
+        pytest.fail("gfx950 device not available for a gpu-marked test")
+    return m
+
+
+def _search(ads, gen, **extra):
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    p = params_for(gen["spec"], **extra)
+    if gen.get("rt_tolerance_per_precursor") is not None and \
+            "rt_tolerance_per_precursor" not in extra:
+        p.update({"rt_tolerance_per_precursor": gen["rt_tolerance_per_precursor"]})
+    return d, lib, p, ads.PeakGroupSelection(p).search(d, lib).to_arrays()
+
+
+def _oracle(gen, **extra):
+    od, ol = oracle.Dia(gen["dia"]), oracle.Lib(gen["lib"])
+    kw = dict(extra)
+    if gen.get("rt_tolerance_per_precursor") is not None and \
+            "rt_tolerance_per_precursor" not in kw:
+        kw["rt_tolerance_per_precursor"] = gen["rt_tolerance_per_precursor"]
+    return oracle.select(od, ol, oracle_params_for(gen["spec"], **kw))
+
+
+def test_device_logf_matches_libm_sample(ads):
+    import alphadia_search_rs._native as N
+    rng = np.random.default_rng(7)
+    bits = rng.integers(0x00800000, 0x7f800000, 1 << 22, dtype=np.uint32)
+    extra = np.array([0x3f800000, 0x3f800001, 0x3f7fffff, 0x7f7fffff, 0x00800000,
+                      0x00000001, 0x007fffff, 0x7f800000], dtype=np.uint32)
+    x = np.concatenate([bits, extra]).view(np.float32)
+    y = np.empty_like(x)
+    N.check(N.lib.adsr_device_logf(N.ptr(x), len(x), N.ptr(y)))
+    ref = np.array([oracle.lib().orc_logf(float(v)) for v in x[-8:]], np.float32)
+    assert np.array_equal(y[-8:].view(np.uint32), ref.view(np.uint32))
+    host = np.array([N.lib.adsr_logf_host(float(v)) for v in x[:20000]], np.float32)
+    assert np.array_equal(y[:20000].view(np.uint32), host.view(np.uint32))
+
+
+def test_stage_convolution_and_score(ads, run0, oracle_run0):
+    import ctypes
+    import alphadia_search_rs._native as N
+    od, ol = oracle_run0
+    prm = oracle_params_for(run0["spec"])
+    rows = [3, 17, 101, 555, 1999]
+    k = oracle.gaussian_kernel(run0["spec"].fwhm_rt, 1.0, 20, 1.0)
+    kd = np.zeros(32, np.float32)
+    n = N.lib.adsr_gaussian_kernel(run0["spec"].fwhm_rt, 1.0, 20, 1.0, N.ptr(kd), 32)
+    assert n == 20 and np.array_equal(kd[:n].view(np.uint32), k.view(np.uint32))
+    for r in rows:
+        one = oracle.select_one(od, ol, prm, r)
+        if one is None:
+            continue
+        x = np.ascontiguousarray(one["xic"])
+        F, L = x.shape
+        conv = np.zeros_like(x)
+        N.check(N.lib.adsr_convolution(N.ptr(k), len(k), N.ptr(x), F, L, N.ptr(conv)))
+        assert np.array_equal(conv.view(np.uint32), one["conv"].view(np.uint32))
+        sc = np.zeros(L, np.float32)
+        w = np.ascontiguousarray(one["fragment_intensity"])
+        c = np.ascontiguousarray(one["conv"])
+        N.check(N.lib.adsr_axis_log_dot_product(N.ptr(c), F, L, N.ptr(w), N.ptr(sc)))
+        assert np.array_equal(sc.view(np.uint32), one["score"].view(np.uint32))
+        idx = np.zeros(L, np.uint64)
+        val = np.zeros(L, np.float32)
+        nm = N.c_size()
+        N.check(N.lib.adsr_find_local_maxima(N.ptr(sc), L, one["cycle_start"], N.ptr(idx),
+                                             N.ptr(val), ctypes.byref(nm)))
+        oi, ov = oracle.find_local_maxima(one["score"], one["cycle_start"])
+        assert nm.value == len(oi)
+        assert np.array_equal(idx[: nm.value].astype(np.int64), oi)
+        assert np.array_equal(val[: nm.value].view(np.uint32), ov.view(np.uint32))
+
+
+def test_stage_dense_xic(ads, run0, oracle_run0):
+    import alphadia_search_rs._native as N
+    od, ol = oracle_run0
+    prm = oracle_params_for(run0["spec"])
+    d = ads.DIAData.from_arrays(*run0["dia"])
+    lib = run0["lib"]
+    for r in [0, 5, 77, 800, 1500]:
+        one = oracle.select_one(od, ol, prm, r)
+        if one is None:
+            continue
+        fm = np.ascontiguousarray(one["fragment_mz"])
+        F = len(fm)
+        L = one["cycle_stop"] - one["cycle_start"]
+        out = np.zeros((F, L), np.float32)
+        N.check(N.lib.adsr_dense_xic(d._h, float(lib[2][r]), one["cycle_start"],
+                                     one["cycle_stop"], run0["spec"].mass_tolerance,
+                                     N.ptr(fm), F, N.ptr(out)))
+        assert np.array_equal(out.view(np.uint32), one["xic"].view(np.uint32))
+
+
+def test_fused_scores_bitwise(ads, run0, oracle_run0):
+    import ctypes
+    import alphadia_search_rs._native as N
+    od, ol = oracle_run0
+    prm = oracle_params_for(run0["spec"])
+    d = ads.DIAData.from_arrays(*run0["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*run0["lib"])
+    p = params_for(run0["spec"])
+    r0, r1, stride = 0, 256, 256
+    scores = np.zeros((r1 - r0, stride), np.float32)
+    cs = np.zeros(r1 - r0, np.uint64)
+    ce = np.zeros(r1 - r0, np.uint64)
+    c = p._c()
+    N.check(N.lib.adsr_select_scores(d._h, lib._h, ctypes.byref(c), r0, r1, stride,
+                                     N.ptr(scores), N.ptr(cs), N.ptr(ce)))
+    for i, r in enumerate(range(r0, r1)):
+        one = oracle.select_one(od, ol, prm, r)
+        if one is None:
+            assert ce[i] <= cs[i]
+            continue
+        assert cs[i] == one["cycle_start"] and ce[i] == one["cycle_stop"]
+        L = one["cycle_stop"] - one["cycle_start"]
+        got = scores[i, :L]
+        bad = np.flatnonzero(got.view(np.uint32) != one["score"].view(np.uint32))
+        assert bad.size == 0, f"row {r}: score differs at cycles {bad[:8]}"
+
+
+def test_fused_plan_fragments(ads, run0, oracle_run0):
+    import ctypes
+    import alphadia_search_rs._native as N
+    d = ads.DIAData.from_arrays(*run0["dia"])
+    libo = ads.SpecLibFlat.from_arrays(*run0["lib"])
+    p = params_for(run0["spec"])
+    c = p._c()
+    n = libo.num_precursors
+    stride = 64
+    cs = np.zeros(n, np.uint64)
+    ce = np.zeros(n, np.uint64)
+    ns = np.zeros(n, np.uint32)
+    sel = np.zeros((n, stride), np.uint64)
+    N.check(N.lib.adsr_select_plan(d._h, libo._h, ctypes.byref(c), 0, n, N.ptr(cs),
+                                   N.ptr(ce), N.ptr(ns), N.ptr(sel), stride))
+    lib = run0["lib"]
+    for r in range(0, n, 7):
+        a, b = int(lib[6][r]), int(lib[7][r])
+        want = a + oracle.filter_sort_fragments(lib[9][a:b], lib[10][a:b], lib[14][a:b],
+                                                lib[16][a:b], True, True, 12)
+        assert ns[r] == len(want)
+        assert np.array_equal(sel[r, : ns[r]].astype(np.int64), want)
+
+
+def test_workload0_candidates_match_oracle(ads, run0):
+    got = _search(ads, run0)[3]
+    want = _oracle(run0)
+    assert len(got[0]) > 0
+    assert_tables_equal(got, want, "workload 0")
+
+
+@pytest.mark.parametrize("variant", ["decoys_uneven", "per_window_sigma", "ties"])
+def test_small_variants_match_oracle(ads, variant):
+    if variant == "decoys_uneven":
+        spec = synthetic.config(2, n_cycles=600, n_windows=12, peaks_per_spectrum=800,
+                                n_targets=1500)
+        gen = synthetic.generate(spec)
+        got = _search(ads, gen)[3]
+        want = _oracle(gen)
+    elif variant == "per_window_sigma":
+        spec = synthetic.config(1, n_cycles=500, n_windows=6, peaks_per_spectrum=600,
+                                n_targets=800, half_window_min=80, half_window_max=80)
+        gen = synthetic.generate(spec)
+        sig = np.linspace(1.5, 3.0, 6).astype(np.float32)
+        got = _search(ads, gen, sigma_per_window=sig)[3]
+        want = _oracle(gen, sigma_per_window=sig)
+    else:
+        # coarse intensities and many duplicate (bin, cycle) entries
+        spec = synthetic.config(0, n_cycles=300, n_windows=4, peaks_per_spectrum=3000,
+                                n_targets=600, half_window_min=40, half_window_max=40)
+        gen = synthetic.generate(spec)
+        dia = list(gen["dia"])
+        dia[8] = np.round(dia[8] / 200.0).astype(np.float32) * 200.0
+        # snap every 3rd peak onto its neighbour: same bin, same cycle
+        mz = dia[7].copy()
+        mz[1::3] = mz[0:-1:3][: len(mz[1::3])]
+        dia[7] = mz
+        gen = dict(gen, dia=tuple(dia))
+        got = _search(ads, gen, top_k_fragments=9)[3]
+        want = _oracle(gen, top_k_fragments=9)
+    assert_tables_equal(got, want, variant)
