@@ -88,8 +88,11 @@ _sig("adsr_lib_free", None, vp)
 _sig("adsr_lib_num_precursors", c_size, vp)
 _sig("adsr_lib_num_fragments", c_size, vp)
 _sig("adsr_select", I, vp, vp, C.POINTER(SelectionParamsC), C.POINTER(vp))
-_sig("adsr_select_timed", I, vp, vp, C.POINTER(SelectionParamsC), I, I, I, I,
-     C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(vp))
+_sig("adsr_select_timed", I, vp, vp, C.POINTER(SelectionParamsC), I, I,
+     C.POINTER(I), C.POINTER(I), C.POINTER(C.c_double), C.POINTER(C.c_double),
+     C.POINTER(C.c_double), C.POINTER(vp))
+_sig("adsr_select_algorithmic_bytes", I, vp, vp, C.POINTER(SelectionParamsC),
+     C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("adsr_select_scores", I, vp, vp, C.POINTER(SelectionParamsC), c_size,
      c_size, c_size, vp, vp, vp)
 _sig("adsr_select_plan", I, vp, vp, C.POINTER(SelectionParamsC), c_size, c_size,

@@ -806,6 +806,51 @@ __global__ __launch_bounds__(256) void scatter_kernel(
   }
 }
 
+// Algorithmic bytes per (row, selected fragment): slice_starts of the bin
+// range, every CSR entry's cycle index, the intensity of entries in the RT
+// window, per valid window.  Reduced with one atomic per block.
+__global__ __launch_bounds__(256) void traffic_kernel(KDia dia, KLib lib,
+                                                      KParams prm, KPlan plan,
+                                                      unsigned long long *acc) {
+  __shared__ unsigned long long part[3];
+  if (threadIdx.x < 3) part[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t row = t / plan.ks;
+  const uint32_t slot = (uint32_t)(t - row * plan.ks);
+  unsigned long long bytes = 0, ents = 0, hits = 0;
+  if (row < lib.n_prec) {
+    const uint4 rec = plan.rec[row];
+    const uint32_t nsel = rec.z & 0xffffu, nobs = rec.z >> 16;
+    if (slot == 0)
+      bytes += 24 + 10ull * (lib.f_stop[row] - lib.f_start[row]);
+    if (slot < nsel && rec.y > rec.x) {
+      const float mz = plan.sel_mz[row * plan.ks + slot];
+      const float delta = mz * prm.mass_tol * 1e-6f;
+      uint32_t start, end;
+      mz_range(dia, mz - delta, mz + delta, start, end);
+      for (uint32_t j = 0; j < nobs; ++j) {
+        const size_t idx = (row * kObsMax + j) * plan.ks + slot;
+        const uint2 r = plan.rng[idx];
+        const uint32_t o = plan.obs[row * kObsMax + j];
+        bytes += 4ull * (end - start + 1);
+        const uint16_t *cy = dia.cycles + dia.entry_base[o];
+        for (uint32_t e = r.x; e < r.y; ++e) {
+          ++ents;
+          const uint32_t c = cy[e];
+          if (c >= rec.x && c < rec.y) ++hits;
+        }
+      }
+    }
+  }
+  bytes += 2 * ents + 4 * hits;
+  atomicAdd(&part[0], bytes);
+  atomicAdd(&part[1], ents);
+  atomicAdd(&part[2], hits);
+  __syncthreads();
+  if (threadIdx.x < 3) atomicAdd(&acc[threadIdx.x], part[threadIdx.x]);
+}
+
 // ---------------------------------------------------------------------------
 // per-stage kernels (SimdBackend operations)
 // ---------------------------------------------------------------------------
@@ -1466,10 +1511,11 @@ extern "C" adsr_status adsr_lib_upload(adsr_lib *l) {
   return upload_lib(l);
 }
 
+// Shared body of adsr_select / adsr_select_timed.
 static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
-                                 const adsr_selection_params *p, int repeats,
-                                 int use_graph, int tpb, int ppw,
-                                 double *avg_ms, double *sel_ms,
+                                 const adsr_selection_params *p, int warmup,
+                                 int steps, int *tpb, int *ppw, double *step_ms,
+                                 double *kernel_ms, double *graph_ms,
                                  adsr_candidates **out) {
   clear_error();
   if (!dc || !lc || !p || !out)
@@ -1489,107 +1535,129 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
     return st;
   }
   const uint32_t n = (uint32_t)l->n_prec;
-  LaunchShape shape;
-  bool tune = false;
-  if (tpb == 64 || tpb == 128 || tpb == 256) {
-    shape.waves = tpb / 64;
-    shape.chunk = (ppw == 1 || ppw == 2 || ppw == 4) ? ppw : 1;
-  } else {
-    std::lock_guard<std::mutex> lk(g_tuned_mu);
-    auto key = std::make_tuple((size_t)n, sp.lcap, w.ks);
-    auto it = g_tuned.find(key);
-    if (it != g_tuned.end()) shape = it->second;
-    else tune = repeats > 1;
-  }
-  auto run_once = [&](const LaunchShape &sh, bool timed_sel) -> adsr_status {
-    return enqueue(w, sp, n, sh, nullptr, 0, 0, 0, timed_sel);
-  };
-  if (tune) {
-    // autotune: {64,128,256} threads x {1,2,4} precursors per queue grab
-    float best = 1e30f;
-    LaunchShape best_shape;
-    for (int wv : {1, 2, 4})
-      for (int ch : {1, 2, 4}) {
-        LaunchShape sh{wv, ch};
-        st = run_once(sh, true);  // warm
-        if (st) break;
-        st = run_once(sh, true);
-        if (st) break;
-        if (hipStreamSynchronize(w.stream) != hipSuccess) {
-          st = set_error(ADSR_ERR_DEVICE, "synchronize failed");
-          break;
-        }
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, w.ev[2], w.ev[3]);
-        if (ms < best) {
-          best = ms;
-          best_shape = sh;
-        }
-      }
-    if (st) {
-      ws_free(w);
-      return st;
-    }
-    shape = best_shape;
-    std::lock_guard<std::mutex> lk(g_tuned_mu);
-    g_tuned[std::make_tuple((size_t)n, sp.lcap, w.ks)] = shape;
-  }
-  // first run, eagerly, checks device-side limits
-  st = run_once(shape, true);
-  if (!st) st = check_status(w);
-  if (st) {
+  auto fail = [&](adsr_status s2) {
     ws_free(w);
-    return st;
+    return s2;
+  };
+  auto sync = [&]() -> adsr_status {
+    if (hipStreamSynchronize(w.stream) != hipSuccess)
+      return set_error(ADSR_ERR_DEVICE, "stream synchronize failed: %s",
+                       hipGetErrorString(hipGetLastError()));
+    return ADSR_OK;
+  };
+  LaunchShape shape;
+  const int want_tpb = tpb ? *tpb : 0, want_ppw = ppw ? *ppw : 0;
+  if (want_tpb == 64 || want_tpb == 128 || want_tpb == 256) {
+    shape.waves = want_tpb / 64;
+    shape.chunk = (want_ppw == 1 || want_ppw == 2 || want_ppw == 4) ? want_ppw : 1;
+  } else if (steps > 0) {
+    auto key = std::make_tuple((size_t)n, sp.lcap, w.ks);
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tuned_mu);
+      auto it = g_tuned.find(key);
+      if (it != g_tuned.end()) {
+        shape = it->second;
+        cached = true;
+      }
+    }
+    if (!cached) {
+      // autotuner: {64,128,256} threads x {1,2,4} precursors per queue grab,
+      // each timed on its second launch
+      float best = 1e30f;
+      for (int wv : {1, 2, 4})
+        for (int ch : {1, 2, 4}) {
+          LaunchShape sh{wv, ch};
+          for (int rep = 0; rep < 2; ++rep) {
+            st = enqueue(w, sp, n, sh, nullptr, 0, 0, 0, true);
+            if (st) return fail(st);
+          }
+          st = sync();
+          if (st) return fail(st);
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, w.ev[2], w.ev[3]);
+          if (ms < best) {
+            best = ms;
+            shape = sh;
+          }
+        }
+      std::lock_guard<std::mutex> lk(g_tuned_mu);
+      g_tuned[key] = shape;
+    }
   }
-  if (repeats > 1 || avg_ms) {
-    const int reps = std::max(repeats, 1);
-    if (use_graph) {
+  if (tpb) *tpb = shape.waves * 64;
+  if (ppw) *ppw = shape.chunk;
+  // first search: checks the device-side limits before anything is timed
+  st = enqueue(w, sp, n, shape, nullptr, 0, 0, 0, false);
+  if (!st) st = check_status(w);
+  if (st) return fail(st);
+  for (int i = 0; i < warmup; ++i) {
+    st = enqueue(w, sp, n, shape, nullptr, 0, 0, 0, false);
+    if (st) return fail(st);
+  }
+  if (steps > 0) {
+    std::vector<hipEvent_t> ev(2 * (size_t)steps, nullptr);
+    for (auto &e : ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        for (auto &e2 : ev)
+          if (e2) (void)hipEventDestroy(e2);
+        return fail(set_error(ADSR_ERR_DEVICE, "event creation failed"));
+      }
+    st = sync();
+    if (!st && hipEventRecord(w.ev[0], w.stream) != hipSuccess)
+      st = set_error(ADSR_ERR_DEVICE, "event record failed");
+    for (int i = 0; !st && i < steps; ++i) {
+      w.ev[2] = ev[2 * i];
+      w.ev[3] = ev[2 * i + 1];
+      st = enqueue(w, sp, n, shape, nullptr, 0, 0, 0, true);
+    }
+    if (!st && hipEventRecord(w.ev[1], w.stream) != hipSuccess)
+      st = set_error(ADSR_ERR_DEVICE, "event record failed");
+    if (!st) st = sync();
+    double ksum = 0.0;
+    float total = 0.0f;
+    if (!st) {
+      (void)hipEventElapsedTime(&total, w.ev[0], w.ev[1]);
+      for (int i = 0; i < steps; ++i) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+        ksum += ms;
+      }
+    }
+    // the per-step events belong to `ev`; restore the workspace's own pair
+    w.ev[2] = nullptr;
+    w.ev[3] = nullptr;
+    (void)hipEventCreate(&w.ev[2]);
+    (void)hipEventCreate(&w.ev[3]);
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (st) return fail(st);
+    if (step_ms) *step_ms = total / steps;
+    if (kernel_ms) *kernel_ms = ksum / steps;
+    if (graph_ms) {
       if (hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal) !=
-          hipSuccess) {
-        ws_free(w);
-        return set_error(ADSR_ERR_DEVICE, "graph capture failed to start");
-      }
-      st = run_once(shape, false);
+          hipSuccess)
+        return fail(set_error(ADSR_ERR_DEVICE, "graph capture failed to start"));
+      st = enqueue(w, sp, n, shape, nullptr, 0, 0, 0, false);
       hipError_t e = hipStreamEndCapture(w.stream, &w.graph_src);
-      if (st || e != hipSuccess ||
+      if (st) return fail(st);
+      if (e != hipSuccess ||
           hipGraphInstantiate(&w.graph, w.graph_src, nullptr, nullptr, 0) !=
-              hipSuccess) {
-        ws_free(w);
-        return st ? st : set_error(ADSR_ERR_DEVICE, "graph capture failed");
-      }
-      // warm replay
-      if (hipGraphLaunch(w.graph, w.stream) != hipSuccess) {
-        ws_free(w);
-        return set_error(ADSR_ERR_DEVICE, "graph launch failed");
-      }
-    }
-    (void)hipEventRecord(w.ev[0], w.stream);
-    for (int r = 0; r < reps; ++r) {
-      if (use_graph) {
-        if (hipGraphLaunch(w.graph, w.stream) != hipSuccess) {
-          ws_free(w);
-          return set_error(ADSR_ERR_DEVICE, "graph launch failed");
-        }
-      } else {
-        st = run_once(shape, false);
-        if (st) {
-          ws_free(w);
-          return st;
-        }
-      }
-    }
-    (void)hipEventRecord(w.ev[1], w.stream);
-    (void)hipStreamSynchronize(w.stream);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, w.ev[0], w.ev[1]);
-    if (avg_ms) *avg_ms = ms / reps;
-    if (sel_ms) {
-      // one more eager run with events around the select kernel only
-      st = run_once(shape, true);
-      (void)hipStreamSynchronize(w.stream);
-      float s2 = 0;
-      (void)hipEventElapsedTime(&s2, w.ev[2], w.ev[3]);
-      *sel_ms = s2;
+              hipSuccess)
+        return fail(set_error(ADSR_ERR_DEVICE, "graph capture failed"));
+      if (hipGraphLaunch(w.graph, w.stream) != hipSuccess)
+        return fail(set_error(ADSR_ERR_DEVICE, "graph launch failed"));
+      st = sync();
+      if (st) return fail(st);
+      (void)hipEventRecord(w.ev[0], w.stream);
+      for (int i = 0; i < steps; ++i)
+        if (hipGraphLaunch(w.graph, w.stream) != hipSuccess)
+          return fail(set_error(ADSR_ERR_DEVICE, "graph launch failed"));
+      (void)hipEventRecord(w.ev[1], w.stream);
+      st = sync();
+      if (st) return fail(st);
+      float gms = 0.0f;
+      (void)hipEventElapsedTime(&gms, w.ev[0], w.ev[1]);
+      *graph_ms = gms / steps;
     }
   }
   st = check_status(w);
@@ -1601,15 +1669,59 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
 extern "C" adsr_status adsr_select(const adsr_dia *d, const adsr_lib *l,
                                    const adsr_selection_params *p,
                                    adsr_candidates **out) {
-  return select_common(d, l, p, 1, 0, 0, 0, nullptr, nullptr, out);
+  return select_common(d, l, p, 0, 0, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, out);
 }
 
 extern "C" adsr_status adsr_select_timed(
     const adsr_dia *d, const adsr_lib *l, const adsr_selection_params *p,
-    int repeats, int use_graph, int tpb, int ppw, double *avg_ms,
-    double *sel_ms, adsr_candidates **out) {
-  return select_common(d, l, p, repeats, use_graph, tpb, ppw, avg_ms, sel_ms,
-                       out);
+    int warmup, int steps, int *tpb, int *ppw, double *step_ms,
+    double *kernel_ms, double *graph_ms, adsr_candidates **out) {
+  return select_common(d, l, p, std::max(warmup, 0), std::max(steps, 0), tpb,
+                       ppw, step_ms, kernel_ms, graph_ms, out);
+}
+
+extern "C" adsr_status adsr_select_algorithmic_bytes(
+    const adsr_dia *dc, const adsr_lib *lc, const adsr_selection_params *p,
+    double *bytes, double *entries, double *hits) {
+  clear_error();
+  if (!dc || !lc || !p || !bytes)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  adsr_dia *d = const_cast<adsr_dia *>(dc);
+  adsr_lib *l = const_cast<adsr_lib *>(lc);
+  adsr_status st = upload_dia(d);
+  if (!st) st = upload_lib(l);
+  if (st) return st;
+  Workspace w;
+  SearchPlan sp;
+  st = build_search(d, l, p, w, sp);
+  const uint32_t n = (uint32_t)l->n_prec;
+  if (!st) st = enqueue(w, sp, n, LaunchShape(), nullptr, 0, 0, 0, false);
+  if (!st) st = check_status(w);
+  unsigned long long *acc = nullptr;
+  if (!st && hipMalloc(&acc, 32) != hipSuccess)
+    st = set_error(ADSR_ERR_OUT_OF_MEMORY, "accumulator");
+  if (!st) {
+    unsigned long long zero[4] = {0, 0, 0, 0};
+    (void)hipMemcpy(acc, zero, 32, hipMemcpyHostToDevice);
+    const uint64_t nt = (uint64_t)n * sp.plan.ks;
+    if (nt)
+      traffic_kernel<<<(uint32_t)((nt + 255) / 256), 256, 0, w.stream>>>(
+          sp.dia, sp.lib, sp.prm, sp.plan, acc);
+    unsigned long long host[3] = {0, 0, 0};
+    uint32_t total = 0;
+    if (hipStreamSynchronize(w.stream) != hipSuccess ||
+        hipMemcpy(host, acc, 24, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&total, w.small + 2, 4, hipMemcpyDeviceToHost) != hipSuccess)
+      st = set_error(ADSR_ERR_DEVICE, "traffic count failed");
+    if (n == 0) total = 0;
+    *bytes = (double)host[0] + 68.0 * total;
+    if (entries) *entries = (double)host[1];
+    if (hits) *hits = (double)host[2];
+  }
+  (void)hipFree(acc);
+  ws_free(w);
+  return st;
 }
 
 extern "C" adsr_status adsr_select_scores(const adsr_dia *dc,

@@ -141,17 +141,33 @@ adsr_status adsr_select(const adsr_dia *d, const adsr_lib *l,
                         const adsr_selection_params *p,
                         adsr_candidates **out);
 
-/* Timing-oriented variant: runs the pipeline `repeats` times (captured in a
- * hipGraph after the first run when use_graph != 0) and reports the average
- * device time per run in *avg_ms; the table of the last run is returned.
- * threads_per_block in {64,128,256} and precursors_per_wave in {1,2,4}
- * select the launch shape; 0 = autotuned choice.                          */
+/* Timing-oriented variant of adsr_select.  Runs `warmup` untimed searches,
+ * then `steps` timed ones on the library's stream: *step_ms is the average
+ * device time of one search (plan, bins, select, compaction), measured with
+ * HIP events around the whole loop, and *select_kernel_ms the average
+ * duration of the select kernel, from event pairs around each launch.  When
+ * graph_step_ms is non-NULL the search is also captured once in a hipGraph
+ * and `steps` replays are timed.  threads_per_block in {64,128,256} and
+ * precursors_per_wave in {1,2,4} pick the launch shape; 0 = autotuned.
+ * The chosen shape is written back when the pointers are non-NULL.  The
+ * candidate table of the last search is returned.                        */
 adsr_status adsr_select_timed(const adsr_dia *d, const adsr_lib *l,
-                              const adsr_selection_params *p, int repeats,
-                              int use_graph, int threads_per_block,
-                              int precursors_per_wave, double *avg_ms,
-                              double *select_kernel_avg_ms,
+                              const adsr_selection_params *p, int warmup,
+                              int steps, int *threads_per_block,
+                              int *precursors_per_wave, double *step_ms,
+                              double *select_kernel_ms, double *graph_step_ms,
                               adsr_candidates **out);
+
+/* Algorithmic HBM bytes of one search over the whole library: per
+ * precursor 24 (m/z, RT, fragment range) + 10 per raw fragment + for every
+ * valid window and selected fragment 4*(bins+1) (slice_starts) + 2 per CSR
+ * entry in those bins (cycle index) + 4 per entry inside the RT window
+ * (intensity), + 68 per candidate written.  Counted on the device.       */
+adsr_status adsr_select_algorithmic_bytes(const adsr_dia *d,
+                                          const adsr_lib *l,
+                                          const adsr_selection_params *p,
+                                          double *bytes, double *entries,
+                                          double *hits);
 
 /* Diagnostic: the fused kernel's per-cycle score vector for library rows
  * [row_begin, row_end), written densely (stride max_cycles, NaN-padded),
