@@ -30,6 +30,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -51,6 +52,7 @@ constexpr int kHalfMax = 15;     // kernel half width (kernel_size <= 30)
 constexpr int kLCapMax = 4096;   // cycles per precursor window
 constexpr int kStashFrag = 48;   // per-thread LDS stash in the plan kernel
 constexpr size_t kMaxDynLds = 160 * 1024 - 512;  // bytes per block
+constexpr uint32_t kLdsBudget = 12 * 1024;       // preferred bytes per wave
 
 enum StatusBits : uint32_t {
   kStTooManyFragments = 1u,
@@ -406,7 +408,7 @@ struct SelArgs {
   uint32_t lds_per_wave;  // bytes
   uint32_t lcap;          // max cycles per window (multiple of 64)
   uint32_t xcap;          // floats in the XIC tile
-  uint32_t mcap;          // u32 words per mask array
+  uint32_t gcap;          // fragment rows in the XIC tile
   float *cand_score;      // [row][cand_count]
   uint32_t *cand_center;  // [row][cand_count]
   uint32_t *cand_count;   // [row]
@@ -415,93 +417,72 @@ struct SelArgs {
   uint32_t dbg_stride, dbg_row0, dbg_rows;
 };
 
+// Per-wave LDS.  XIC rows have kXPad zero floats on both sides so that a
+// block of 4 outputs reads its 21-tap neighbourhood with aligned 16-byte
+// loads; M holds one hit bit per cycle (64-bit words, one zero word before
+// and after each row); BD one bit per active 4-cycle output block.
+constexpr int kXPad = 16;
+constexpr int kItemCap = 1024;
+constexpr int kTilesInFlight = 8;
+
+struct LdsShape {
+  uint32_t lp;     // XIC row stride (floats)
+  uint32_t wm;     // M row stride (u64 words)
+  uint32_t wb;     // BD row stride (u64 words)
+};
+
+__host__ __device__ inline LdsShape lds_shape(uint32_t lcap) {
+  LdsShape s;
+  s.lp = lcap + 2 * kXPad;
+  s.wm = lcap / 64 + 2;
+  s.wb = (lcap / 64 + 3) / 4 * 4 / 4 + 1;
+  return s;
+}
+
+__host__ __device__ inline uint32_t lds_bytes(uint32_t lcap, uint32_t gcap) {
+  const LdsShape s = lds_shape(lcap);
+  uint32_t b = lcap * 4                 // R
+               + gcap * s.lp * 4        // X
+               + gcap * s.wm * 8        // M
+               + gcap * s.wb * 8        // BD
+               + kItemCap * 2           // items (u16)
+               + lcap                   // own
+               + kMaxSel * 4            // w
+               + 32 * 4;                // kernel taps
+  return (b + 15) / 16 * 16;
+}
+
 struct WaveLds {
-  float *R;        // lcap
-  float *X;        // xcap
-  uint32_t *M;     // mcap (hit mask per fragment row)
-  uint32_t *D;     // mcap (dilated mask per fragment row)
-  uint16_t *items; // lcap
-  float *w;        // kMaxSel
-  uint32_t *pre;   // kMaxSel + 1
-  uint32_t *eb;    // kMaxSel
-  float *kern;     // 32
+  float *R;
+  float *X;
+  uint64_t *M;
+  uint64_t *BD;
+  uint16_t *items;
+  uint8_t *own;
+  float *w;
+  float *kern;
 };
 
 __device__ __forceinline__ WaveLds carve(char *base, const SelArgs &a) {
+  const LdsShape sh = lds_shape(a.lcap);
   WaveLds l;
   char *p = base;
   l.R = (float *)p;
   p += a.lcap * 4;
   l.X = (float *)p;
-  p += a.xcap * 4;
-  l.M = (uint32_t *)p;
-  p += a.mcap * 4;
-  l.D = (uint32_t *)p;
-  p += a.mcap * 4;
+  p += a.gcap * sh.lp * 4;
+  l.M = (uint64_t *)p;
+  p += a.gcap * sh.wm * 8;
+  l.BD = (uint64_t *)p;
+  p += a.gcap * sh.wb * 8;
   l.items = (uint16_t *)p;
-  p += a.lcap * 2;
+  p += kItemCap * 2;
+  l.kern = (float *)p;
+  p += 32 * 4;
   l.w = (float *)p;
   p += kMaxSel * 4;
-  l.pre = (uint32_t *)p;
-  p += (kMaxSel + 1) * 4;
-  l.eb = (uint32_t *)p;
-  p += kMaxSel * 4;
-  l.kern = (float *)p;
+  l.own = (uint8_t *)p;
   return l;
-}
-
-// Adds intensity I to X[f][t] (0.0 + I on the first contribution, as the
-// zero-initialised reference array) and marks the hit and its
-// convolution support in the masks.
-__device__ __forceinline__ void xic_add(const WaveLds &l, uint32_t f,
-                                        uint32_t t, float I, uint32_t L,
-                                        uint32_t wm, uint32_t half) {
-  uint32_t *Mrow = l.M + f * wm;
-  uint32_t *Drow = l.D + f * wm;
-  const uint32_t bit = 1u << (t & 31);
-  const float prev = (Mrow[t >> 5] & bit) ? l.X[f * L + t] : 0.0f;
-  l.X[f * L + t] = prev + I;
-  atomicOr(&Mrow[t >> 5], bit);
-  const uint32_t a = t >= half ? t - half : 0u;
-  const uint32_t b = min(t + half, L - 1);  // inclusive, b - a <= 30
-  const uint32_t wa = a >> 5, wb = b >> 5;
-  if (wa == wb) {
-    const uint32_t n = b - a + 1;
-    const uint32_t m = (n == 32 ? 0xffffffffu : ((1u << n) - 1u)) << (a & 31);
-    atomicOr(&Drow[wa], m);
-  } else {
-    atomicOr(&Drow[wa], 0xffffffffu << (a & 31));
-    const uint32_t nb = (b & 31) + 1;
-    atomicOr(&Drow[wb], nb == 32 ? 0xffffffffu : ((1u << nb) - 1u));
-  }
-}
-
-// src/convolution/scalar.rs:14-36 for one output i (half <= i < L-half)
-// of row f: s = 0 + X[i]*k[h]; then pairs at distance h..1, nearest last.
-// Pairs whose two inputs are both zero contribute exactly +0 and are skipped.
-__device__ __forceinline__ float conv_one(const float *Xrow,
-                                          const uint32_t *Mrow, uint32_t i,
-                                          uint32_t half, const float *k) {
-  const uint32_t start = i - half;
-  const uint32_t wd = start >> 5, sh = start & 31;
-  const uint64_t w64 = (uint64_t)Mrow[wd] | ((uint64_t)Mrow[wd + 1] << 32);
-  const uint32_t nbits = 2 * half + 1;
-  const uint32_t bits = (uint32_t)(w64 >> sh) & ((1u << nbits) - 1u);
-  float s = 0.0f;
-  const float xc = ((bits >> half) & 1u) ? Xrow[i] : 0.0f;
-  s = s + xc * k[half];
-  const uint32_t low = (1u << half) - 1u;
-  uint32_t pairs =
-      (bits & low) | ((__builtin_bitreverse32(bits) >> (31 - 2 * half)) & low);
-  while (pairs) {
-    const uint32_t b = __builtin_ctz(pairs);
-    pairs &= pairs - 1u;
-    const uint32_t d = half - b;
-    const float xl = ((bits >> b) & 1u) ? Xrow[i - d] : 0.0f;
-    const float xr = ((bits >> (2 * half - b)) & 1u) ? Xrow[i + d] : 0.0f;
-    s = s + (xl + xr) * k[b];
-  }
-  return s;
 }
 
 __global__ void reset_queue_kernel(uint32_t *q, uint32_t *status) {
@@ -511,7 +492,94 @@ __global__ void reset_queue_kernel(uint32_t *q, uint32_t *status) {
   }
 }
 
-template <int WAVES>
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// True when entry e (relative to the window) of a hit with cycle == c0 is
+// followed, within its bin, by another entry with cycle == c0: the
+// reference's binary_search(c0) starts at the last such entry
+// (src/quadrupole_observation/mod.rs:92-94).  Rare; reads global memory.
+__device__ __forceinline__ bool skipped_duplicate(const SelArgs &a, uint32_t o,
+                                                  uint64_t base, uint32_t b0,
+                                                  uint32_t e, uint32_t c0) {
+  const uint32_t *ss = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
+  uint32_t b = b0;
+  while (ss[b + 1] <= e) ++b;
+  return e + 1 < ss[b + 1] && a.dia.cycles[base + e + 1] == c0;
+}
+
+// One dilation step of a 192-bit window (prev, cur, next) by +-s.
+__device__ __forceinline__ void dilate_step(uint64_t &p, uint64_t &c,
+                                            uint64_t &n, uint32_t s) {
+  const uint64_t p2 = p | (p << s) | (p >> s) | (c << (64 - s));
+  const uint64_t c2 = c | (c << s) | (c >> s) | (p >> (64 - s)) | (n << (64 - s));
+  const uint64_t n2 = n | (n << s) | (n >> s) | (c >> (64 - s));
+  p = p2;
+  c = c2;
+  n = n2;
+}
+
+// Bits 0,4,...,60 of x packed into bits 0..15.
+__device__ __forceinline__ uint32_t pack_nibbles(uint64_t x) {
+  x &= 0x1111111111111111ull;
+  x = (x | (x >> 3)) & 0x0303030303030303ull;
+  x = (x | (x >> 6)) & 0x000f000f000f000full;
+  x = (x | (x >> 12)) & 0x000000ff000000ffull;
+  x = (x | (x >> 24)) & 0xffffull;
+  return (uint32_t)x;
+}
+
+// Four consecutive outputs i0..i0+3 of src/convolution/scalar.rs:14-36
+// (s = 0 + x[i]*k[h], then s += (x[i-d] + x[i+d]) * k[h-d] for d = h..1),
+// from the zero-padded row: xs points at X[i0 - 12] (16-byte aligned).
+__device__ __forceinline__ void conv4_h10(const float *xs, const float (&k)[11],
+                                          float (&y)[4]) {
+  float x[28];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const float4 v = *reinterpret_cast<const float4 *>(xs + 4 * q);
+    x[4 * q + 0] = v.x;
+    x[4 * q + 1] = v.y;
+    x[4 * q + 2] = v.z;
+    x[4 * q + 3] = v.w;
+  }
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    const int c = 12 + o;
+    float s = 0.0f;
+    s = s + x[c] * k[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+      const int d = 10 - j;
+      s = s + (x[c - d] + x[c + d]) * k[j];
+    }
+    y[o] = s;
+  }
+}
+
+// Same for any half width <= 15, taps read from LDS.
+__device__ __forceinline__ void conv4_any(const float *row, uint32_t i0,
+                                          uint32_t half, const float *k,
+                                          float (&y)[4]) {
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    const float *xc = row + i0 + o;
+    float s = 0.0f;
+    s = s + xc[0] * k[half];
+    for (uint32_t j = 0; j < half; ++j) {
+      const uint32_t d = half - j;
+      s = s + (xc[-(int)d] + xc[d]) * k[j];
+    }
+    y[o] = s;
+  }
+}
+
+template <int WAVES, int HALF>
 __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
   extern __shared__ __align__(16) char smem[];
   __shared__ LogfEntry ltab[16];
@@ -520,158 +588,245 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   const WaveLds l = carve(smem + (size_t)wv * a.lds_per_wave, a);
-  const uint32_t half = a.prm.half;
+  const LdsShape sh = lds_shape(a.lcap);
+  const uint32_t half = HALF ? (uint32_t)HALF : a.prm.half;
   const uint32_t cc = a.prm.cand_count;
   const uint32_t ks = a.plan.ks;
 
   for (;;) {
     uint32_t row0 = 0;
     if (lane == 0) row0 = atomicAdd(a.queue, a.chunk);
-    row0 = __shfl(row0, 0, kWave);
+    row0 = uni(__shfl(row0, 0, kWave));
     if (row0 >= a.n_rows) break;
     const uint32_t row_end = min(row0 + a.chunk, a.n_rows);
     for (uint32_t row = row0; row < row_end; ++row) {
       const uint4 rec = a.plan.rec[row];
-      const uint32_t c0 = rec.x, c1 = rec.y;
-      const uint32_t nsel = rec.z & 0xffffu, nobs = rec.z >> 16;
+      const uint32_t c0 = uni(rec.x), c1 = uni(rec.y);
+      const uint32_t nsel = uni(rec.z & 0xffffu), nobs = uni(rec.z >> 16);
       if (c1 <= c0) {  // mod.rs:146-148 -> None
         if (lane == 0) a.cand_count[row] = 0;
         continue;
       }
       const uint32_t L = c1 - c0;
-      if (L > a.lcap) {  // guarded by the host bound; never expected
+      if (L > a.lcap) {  // excluded by the host bound; flagged if it happens
         if (lane == 0) {
           a.cand_count[row] = 0;
           atomicOr(a.status, (uint32_t)kStWindowTooLong);
         }
         continue;
       }
-      const uint32_t wm = (L + 31) / 32 + 1;
-      for (uint32_t i = lane; i < L; i += kWave) l.R[i] = 0.0f;
-      if (lane < (int)a.prm.klen)
-        l.kern[lane] = a.prm.kernels[(size_t)rec.w * a.prm.klen + lane];
+      const uint32_t nw = (L + 63) / 64;  // 64-cycle words per row
+      for (uint32_t i = 4 * lane; i < L; i += 4 * kWave)
+        *reinterpret_cast<float4 *>(l.R + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+      const float *kg = a.prm.kernels + (size_t)uni(rec.w) * a.prm.klen;
+      float k10[11];
+      if (HALF == 10) {
+#pragma unroll
+        for (int j = 0; j < 11; ++j) k10[j] = kg[j];
+      } else {
+        if (lane <= (int)half) l.kern[lane] = kg[lane];
+      }
       if (lane < (int)nsel) l.w[lane] = a.plan.sel_w[(size_t)row * ks + lane];
       __builtin_amdgcn_wave_barrier();
 
-      const uint32_t gcap = min(a.xcap / L, a.mcap / wm);
-      for (uint32_t g0 = 0; g0 < nsel && nobs > 0; g0 += gcap) {
-        const uint32_t G = min(gcap, nsel - g0);
-        for (uint32_t i = lane; i < G * wm; i += kWave) {
-          l.M[i] = 0u;
-          l.D[i] = 0u;
+      for (uint32_t g0 = 0; g0 < nsel && nobs > 0; g0 += a.gcap) {
+        const uint32_t G = min(a.gcap, nsel - g0);
+        // zero the XIC rows (with their pads) and the hit masks
+        {
+          float4 *xz = reinterpret_cast<float4 *>(l.X);
+          const uint32_t n4 = G * sh.lp / 4;
+          for (uint32_t i = lane; i < n4; i += kWave)
+            xz[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+          for (uint32_t i = lane; i < G * sh.wm; i += kWave) l.M[i] = 0ull;
         }
         __builtin_amdgcn_wave_barrier();
-        // ---- extraction: src/dense_xic_observation/mod.rs:42-55 --------
+        // ---- extraction: src/dense_xic_observation/mod.rs:42-55 ----------
         for (uint32_t j = 0; j < nobs; ++j) {
-          const uint32_t o = a.plan.obs[(size_t)row * kObsMax + j];
+          const uint32_t o = uni(a.plan.obs[(size_t)row * kObsMax + j]);
           const uint64_t base = a.dia.entry_base[o];
           const size_t ridx = ((size_t)row * kObsMax + j) * ks + g0;
-          uint32_t len = 0;
+          uint32_t rx = 0, rlen = 0, rb0 = 0;
           if (lane < (int)G) {
             const uint2 r = a.plan.rng[ridx + lane];
-            l.eb[lane] = r.x;
-            len = r.y - r.x;
+            rx = r.x;
+            rlen = r.y - r.x;
+            rb0 = a.plan.b0[ridx + lane];
           }
-          const uint32_t incl = wave_incl_scan(len);
-          if (lane < (int)G) l.pre[lane] = incl - len;
-          const uint32_t T = __shfl(incl, (int)G - 1, kWave);
-          if (lane == 0) l.pre[G] = T;
-          __builtin_amdgcn_wave_barrier();
-          for (uint32_t q0 = 0; q0 < T; q0 += kWave) {
-            const uint32_t q = q0 + lane;
-            bool hit = false;
-            uint32_t f = 0, t = 0;
-            float I = 0.0f;
-            if (q < T) {
-              // fragment owning concatenated entry q
-              uint32_t lo = 0, hi = G - 1;
-              while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (l.pre[mid] <= q) lo = mid;
-                else hi = mid - 1;
-              }
-              f = lo;
-              const uint32_t e = l.eb[f] + (q - l.pre[f]);
-              const uint32_t cyc = a.dia.cycles[base + e];
-              hit = cyc >= c0 && cyc < c1;
-              if (hit && cyc == c0) {
-                // fill_xic_slice starts at binary_search(c0), which is the
-                // LAST entry equal to c0 within the bin
-                // (src/quadrupole_observation/mod.rs:92-94): an entry equal
-                // to c0 followed by another c0 entry of the same bin is
-                // skipped.
-                const uint32_t *ss = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
-                uint32_t b = a.plan.b0[ridx + f];
-                while (ss[b + 1] <= e) ++b;
-                if (e + 1 < ss[b + 1] && a.dia.cycles[base + e + 1] == c0)
-                  hit = false;
-              }
-              if (hit) {
-                t = cyc - c0;
-                I = a.dia.intens[base + e];
+          // 64-entry tiles; tile T belongs to the last fragment whose first
+          // tile tp is <= T (fragments without entries are skipped)
+          const uint32_t nt = (rlen + kWave - 1) / kWave;
+          const uint32_t tincl = wave_incl_scan(nt);
+          const uint32_t tp = tincl - nt;
+          const uint32_t NT = rdlane(tincl, G - 1);
+          for (uint32_t t0 = 0; t0 < NT; t0 += kTilesInFlight) {
+            uint32_t tf[kTilesInFlight], tk[kTilesInFlight];
+#pragma unroll
+            for (int u = 0; u < kTilesInFlight; ++u) {
+              const uint32_t T = t0 + u;
+              const uint64_t fm = ballot64(lane < (int)G && nt > 0 && tp <= T);
+              const uint32_t f = fm ? 63u - (uint32_t)__builtin_clzll(fm) : 0u;
+              tf[u] = T < NT ? f : 0xffffffffu;
+              tk[u] = T - rdlane(tp, f);
+            }
+            uint32_t cyc[kTilesInFlight];
+#pragma unroll
+            for (int u = 0; u < kTilesInFlight; ++u) {
+              cyc[u] = 0xffffffffu;
+              if (tf[u] != 0xffffffffu) {
+                const uint32_t k = tk[u] * kWave + lane;
+                if (k < rdlane(rlen, tf[u]))
+                  cyc[u] = a.dia.cycles[base + rdlane(rx, tf[u]) + k];
               }
             }
-            uint64_t hm = ballot64(hit);
-            if (hm) {
-              const uint32_t key = f * L + t;
-              bool dup = false;
-              uint64_t it = hm;
-              while (it) {
-                const int j2 = __builtin_ctzll(it);
-                it &= it - 1;
-                const uint32_t kj = __shfl(key, j2, kWave);
-                if (hit && lane > j2 && key == kj) dup = true;
+            uint32_t hitm = 0;
+#pragma unroll
+            for (int u = 0; u < kTilesInFlight; ++u) {
+              const uint32_t c = cyc[u];
+              bool hit = c >= c0 && c < c1;
+              if (hit && c == c0) {
+                const uint32_t e = rdlane(rx, tf[u]) + tk[u] * kWave + lane;
+                if (skipped_duplicate(a, o, base, rdlane(rb0, tf[u]), e, c0))
+                  hit = false;
               }
+              hitm |= (hit ? 1u : 0u) << u;
+            }
+            float inten[kTilesInFlight];
+#pragma unroll
+            for (int u = 0; u < kTilesInFlight; ++u) {
+              inten[u] = 0.0f;
+              if ((hitm >> u) & 1u) {
+                const uint32_t e = rdlane(rx, tf[u]) + tk[u] * kWave + lane;
+                inten[u] = a.dia.intens[base + e];
+              }
+            }
+            // accumulate in entry order (tiles ascending, lanes ascending);
+            // a tile holds one fragment, so hits collide only on equal cycles
+#pragma unroll
+            for (int u = 0; u < kTilesInFlight; ++u) {
+              const bool hit = (hitm >> u) & 1u;
+              const uint64_t hm = ballot64(hit);
+              if (hm == 0) continue;
+              const uint32_t f = tf[u];
+              const uint32_t t = hit ? cyc[u] - c0 : 0u;
+              if (hit) l.own[t] = (uint8_t)lane;
+              __builtin_amdgcn_wave_barrier();
+              const bool dup = hit && l.own[t] != (uint8_t)lane;
+              float *xr = l.X + f * sh.lp + kXPad;
+              uint64_t *mr = l.M + f * sh.wm + 1;
               if (ballot64(dup) == 0) {
-                if (hit) xic_add(l, f, t, I, L, wm, half);
+                if (hit) {
+                  xr[t] = xr[t] + inten[u];
+                  atomicOr((unsigned long long *)&mr[t >> 6], 1ull << (t & 63));
+                }
               } else {
                 uint64_t it2 = hm;
                 while (it2) {
                   const int j2 = __builtin_ctzll(it2);
                   it2 &= it2 - 1;
-                  if (lane == j2) xic_add(l, f, t, I, L, wm, half);
+                  if (lane == j2) {
+                    xr[t] = xr[t] + inten[u];
+                    atomicOr((unsigned long long *)&mr[t >> 6], 1ull << (t & 63));
+                  }
                   __builtin_amdgcn_wave_barrier();
                 }
               }
+              __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
           }
         }
-        // ---- convolution + score, fragment by fragment ------------------
-        for (uint32_t f = 0; f < G; ++f) {
-          const uint32_t *Drow = l.D + f * wm;
-          const uint32_t *Mrow = l.M + f * wm;
-          const float *Xrow = l.X + f * L;
-          const float wf = l.w[g0 + f];
-          uint32_t nitems = 0;
-          for (uint32_t p0 = 0; p0 < L; p0 += kWave) {
-            const uint64_t dw =
-                (uint64_t)Drow[p0 >> 5] | ((uint64_t)Drow[(p0 >> 5) + 1] << 32);
-            if (dw == 0) continue;
-            const uint32_t p = p0 + lane;
-            const bool act = ((dw >> lane) & 1ull) && p >= half && p + half < L;
-            const uint64_t bm = ballot64(act);
-            if (act) {
-              const uint32_t before =
-                  __builtin_popcountll(bm & ((1ull << lane) - 1ull));
-              l.items[nitems + before] = (uint16_t)p;
+        __builtin_amdgcn_wave_barrier();
+        // ---- active 4-cycle output blocks: hits dilated by +-half, within
+        //      the computed range [half, L-half) of the convolution --------
+        for (uint32_t idx = lane; idx < G * sh.wb * 4; idx += kWave) {
+          const uint32_t f = idx / (sh.wb * 4), w = idx % (sh.wb * 4);
+          uint32_t blocks = 0;
+          if (w < nw) {
+            const uint64_t *mr = l.M + f * sh.wm + 1;
+            uint64_t p = mr[(int)w - 1], c = mr[w], n = mr[w + 1];
+            uint32_t rem = half, s = 1;
+            while (rem) {
+              const uint32_t st = min(s, rem);
+              dilate_step(p, c, n, st);
+              rem -= st;
+              s <<= 1;
             }
-            nitems += (uint32_t)__builtin_popcountll(bm);
+            // valid outputs: half <= pos < L - half
+            const uint32_t lo = 64 * w, hi = lo + 64;
+            uint64_t vm = ~0ull;
+            if (half > lo) vm &= (half - lo >= 64) ? 0ull : (~0ull << (half - lo));
+            const uint32_t end = L >= half ? L - half : 0u;
+            if (end < hi) vm &= (end <= lo) ? 0ull : (~0ull >> (hi - end));
+            c &= vm;
+            const uint64_t nib = c | (c >> 1) | (c >> 2) | (c >> 3);
+            blocks = pack_nibbles(nib);
           }
-          __builtin_amdgcn_wave_barrier();
-          for (uint32_t c = 0; c < nitems; c += kWave) {
+          reinterpret_cast<uint16_t *>(l.BD + f * sh.wb)[w] = (uint16_t)blocks;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // ---- compaction of (fragment, block) items, fragment-major -------
+        uint32_t nitems = 0;
+        const uint32_t nbw = (nw * 16 + 63) / 64;  // 64-block words per row
+        auto flush = [&](uint32_t n) {
+          // conv + ln + weighted score, 1 block per lane per pass; score
+          // updates are applied fragment by fragment (reference row order)
+          for (uint32_t c = 0; c < n; c += kWave) {
             const uint32_t idx = c + lane;
-            if (idx < nitems) {
-              const uint32_t i = l.items[idx];
-              const float y = conv_one(Xrow, Mrow, i, half, l.kern);
-              if (y != 0.0f) {
-                const float v = logf_exact(y + 1.0f, ltab);
-                l.R[i] = l.R[i] + v * wf;
+            const bool v = idx < n;
+            const uint32_t item = v ? l.items[idx] : l.items[c];
+            const uint32_t f = item >> 10, i0 = (item & 0x3ffu) * 4;
+            const float *row = l.X + f * sh.lp + kXPad;
+            float y[4];
+            if (HALF == 10) conv4_h10(row + i0 - 12, k10, y);
+            else conv4_any(row, i0, half, l.kern, y);
+            const float wf = l.w[g0 + f];
+            // outputs outside [half, L - half) stay zero (scalar.rs:30-38)
+            float t[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              t[q] = 0.0f;
+              const uint32_t pos = i0 + q;
+              const bool inside = pos >= half && pos + half < L;
+              if (v && inside && y[q] != 0.0f)
+                t[q] = logf_exact(y[q] + 1.0f, ltab) * wf;
+            }
+            const uint32_t flo = rdlane(item >> 10, 0);
+            const uint32_t last = min(n - c, (uint32_t)kWave) - 1;
+            const uint32_t fhi = rdlane(item >> 10, last);
+            for (uint32_t ff = flo; ff <= fhi; ++ff) {
+              if (v && f == ff) {
+                float4 *rp = reinterpret_cast<float4 *>(l.R + i0);
+                float4 r = *rp;
+                r.x = r.x + t[0];
+                r.y = r.y + t[1];
+                r.z = r.z + t[2];
+                r.w = r.w + t[3];
+                *rp = r;
               }
+              __builtin_amdgcn_wave_barrier();
             }
           }
           __builtin_amdgcn_wave_barrier();
+        };
+        for (uint32_t f = 0; f < G; ++f) {
+          for (uint32_t q = 0; q < nbw; ++q) {
+            const uint64_t bw = l.BD[f * sh.wb + q];
+            const uint32_t blo = uni((uint32_t)bw), bhi = uni((uint32_t)(bw >> 32));
+            if ((blo | bhi) == 0u) continue;
+            const uint32_t before =
+                __builtin_amdgcn_mbcnt_hi(bhi, __builtin_amdgcn_mbcnt_lo(blo, 0u));
+            const uint64_t bwu = ((uint64_t)bhi << 32) | blo;
+            if ((bwu >> lane) & 1ull)
+              l.items[nitems + before] = (uint16_t)((f << 10) | (q * 64 + lane));
+            nitems += (uint32_t)__builtin_popcountll(bwu);
+            if (nitems > (uint32_t)(kItemCap - kWave)) {
+              __builtin_amdgcn_wave_barrier();
+              flush(nitems);
+              nitems = 0;
+            }
+          }
         }
+        __builtin_amdgcn_wave_barrier();
+        if (nitems) flush(nitems);
       }
       __builtin_amdgcn_wave_barrier();
       if (a.dbg_scores && row >= a.dbg_row0 && row < a.dbg_row0 + a.dbg_rows &&
@@ -1182,7 +1337,7 @@ struct SearchPlan {
   KLib lib;
   KParams prm;
   KPlan plan;
-  uint32_t lcap = 0, xcap = 0, mcap = 0, lds_per_wave = 0;
+  uint32_t lcap = 0, xcap = 0, gcap = 0, lds_per_wave = 0;
   int cus = 256;
   std::vector<float> kernels;
 };
@@ -1248,16 +1403,18 @@ static adsr_status prepare(const adsr_dia *d, const adsr_lib *l,
   uint32_t lb = window_bound(d, eff);
   if (lb > (uint32_t)kLCapMax) lb = kLCapMax;  // longer windows are flagged
   sp.lcap = std::max<uint32_t>(64, (lb + 63) / 64 * 64);
-  const uint32_t wmax = sp.lcap / 32 + 1;
-  // XIC tile: all selected fragments when they fit in 24 KB, else groups
-  const uint32_t xbudget = 6144;  // floats
+  // XIC tile: all selected fragments when the wave's LDS stays within
+  // kLdsBudget, else groups of fragments processed one after another
+  static const uint32_t budget = [] {
+    const char *e = std::getenv("ADSR_LDS_BUDGET");
+    const long v = e ? std::strtol(e, nullptr, 10) : 0;
+    return v > 0 ? (uint32_t)v : kLdsBudget;
+  }();
   uint32_t g = (uint32_t)ks;
-  if ((uint64_t)g * sp.lcap > xbudget) g = std::max<uint32_t>(1, xbudget / sp.lcap);
-  sp.xcap = g * sp.lcap;
-  sp.mcap = g * wmax;
-  uint32_t bytes = sp.lcap * 4 + sp.xcap * 4 + 2 * sp.mcap * 4 + sp.lcap * 2 +
-                   kMaxSel * 4 + (kMaxSel + 1) * 4 + kMaxSel * 4 + 32 * 4;
-  sp.lds_per_wave = (bytes + 15) / 16 * 16;
+  while (g > 1 && lds_bytes(sp.lcap, g) > budget) --g;
+  sp.gcap = g;
+  sp.xcap = g * lds_shape(sp.lcap).lp;
+  sp.lds_per_wave = lds_bytes(sp.lcap, g);
   if (sp.lds_per_wave > kMaxDynLds)
     return set_error(ADSR_ERR_UNSUPPORTED,
                      "per-precursor LDS tile of %u bytes exceeds the device",
@@ -1271,15 +1428,13 @@ static adsr_status prepare(const adsr_dia *d, const adsr_lib *l,
 static void set_lds_limits() {
   static std::once_flag once;
   std::call_once(once, [] {
-    (void)hipFuncSetAttribute((const void *)select_kernel<1>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kMaxDynLds);
-    (void)hipFuncSetAttribute((const void *)select_kernel<2>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kMaxDynLds);
-    (void)hipFuncSetAttribute((const void *)select_kernel<4>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kMaxDynLds);
+    const void *fns[] = {
+        (const void *)select_kernel<1, 10>, (const void *)select_kernel<2, 10>,
+        (const void *)select_kernel<4, 10>, (const void *)select_kernel<1, 0>,
+        (const void *)select_kernel<2, 0>,  (const void *)select_kernel<4, 0>};
+    for (const void *fn : fns)
+      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kMaxDynLds);
   });
 }
 
@@ -1319,7 +1474,7 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
     a.lds_per_wave = sp.lds_per_wave;
     a.lcap = sp.lcap;
     a.xcap = sp.xcap;
-    a.mcap = sp.mcap;
+    a.gcap = sp.gcap;
     a.cand_score = w.cand_score;
     a.cand_center = w.cand_center;
     a.cand_count = w.cand_count;
@@ -1332,21 +1487,29 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
            (size_t)sp.lds_per_wave * shape.waves > kMaxDynLds)
       shape.waves >>= 1;
     const size_t lds = (size_t)sp.lds_per_wave * shape.waves;
-    const void *fn = shape.waves == 1   ? (const void *)select_kernel<1>
-                     : shape.waves == 2 ? (const void *)select_kernel<2>
-                                        : (const void *)select_kernel<4>;
+    const bool h10 = sp.prm.half == 10;
+    const void *fn =
+        h10 ? (shape.waves == 1   ? (const void *)select_kernel<1, 10>
+               : shape.waves == 2 ? (const void *)select_kernel<2, 10>
+                                  : (const void *)select_kernel<4, 10>)
+            : (shape.waves == 1   ? (const void *)select_kernel<1, 0>
+               : shape.waves == 2 ? (const void *)select_kernel<2, 0>
+                                  : (const void *)select_kernel<4, 0>);
     const int per_cu = occupancy_blocks(fn, shape.waves * kWave, lds);
     const uint64_t want = ((uint64_t)n_rows + shape.chunk - 1) / shape.chunk;
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>((uint64_t)per_cu * sp.cus,
                               (want + shape.waves - 1) / shape.waves));
     if (timed_select) HIP_TRY(hipEventRecord(w.ev[2], s));
-    if (shape.waves == 1)
-      select_kernel<1><<<blocks, 64, lds, s>>>(a);
-    else if (shape.waves == 2)
-      select_kernel<2><<<blocks, 128, lds, s>>>(a);
-    else
-      select_kernel<4><<<blocks, 256, lds, s>>>(a);
+    if (h10) {
+      if (shape.waves == 1) select_kernel<1, 10><<<blocks, 64, lds, s>>>(a);
+      else if (shape.waves == 2) select_kernel<2, 10><<<blocks, 128, lds, s>>>(a);
+      else select_kernel<4, 10><<<blocks, 256, lds, s>>>(a);
+    } else {
+      if (shape.waves == 1) select_kernel<1, 0><<<blocks, 64, lds, s>>>(a);
+      else if (shape.waves == 2) select_kernel<2, 0><<<blocks, 128, lds, s>>>(a);
+      else select_kernel<4, 0><<<blocks, 256, lds, s>>>(a);
+    }
     if (timed_select) HIP_TRY(hipEventRecord(w.ev[3], s));
     const uint32_t nb = (n_rows + kScanBlock - 1) / kScanBlock;
     scan_local_kernel<<<nb, 256, 0, s>>>(w.cand_count, n_rows, w.local, w.bsum);

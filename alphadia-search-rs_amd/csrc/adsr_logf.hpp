@@ -47,24 +47,22 @@ constexpr double kLogfA1 = 0x1.5575b0be00b6ap-2;
 constexpr double kLogfA2 = -0x1.ffffef20a4123p-2;
 
 // The table is passed in so that device code can read it from LDS or
-// constant memory and host code from a static array.
+// constant memory and host code from a static array.  Branch-free: the main
+// path runs on a sanitised input (subnormals normalised as in the reference
+// algorithm) and zero, negative, infinite and NaN inputs are selected
+// afterwards.  glibc's early return for x == 1 is not needed: the main path
+// yields +0 for it.  Checked against glibc logf on all 2^32 inputs.
 __host__ __device__ __forceinline__ float logf_exact(float x,
                                                      const LogfEntry *tab) {
-  uint32_t ix = __builtin_bit_cast(uint32_t, x);
-  if (ix == 0x3f800000u) return 0.0f;
-  if (ix - 0x00800000u >= 0x7f800000u - 0x00800000u) {
-    // x < 0x1p-126, inf or nan
-    if (ix * 2u == 0u) return -__builtin_inff();
-    if (ix == 0x7f800000u) return x;
-    if ((ix & 0x80000000u) || ix * 2u >= 0xff000000u)
-      return __builtin_nanf("");
-    ix = __builtin_bit_cast(uint32_t, x * 0x1p23f);  // subnormal: normalise
-    ix -= 23u << 23;
-  }
-  const uint32_t tmp = ix - 0x3f330000u;
+  const uint32_t ix = __builtin_bit_cast(uint32_t, x);
+  const bool special = (ix - 0x00800000u) >= (0x7f800000u - 0x00800000u);
+  const bool sub = ix != 0u && ix < 0x00800000u;  // positive subnormal
+  const uint32_t ixn = __builtin_bit_cast(uint32_t, x * 0x1p23f) - (23u << 23);
+  const uint32_t iu = sub ? ixn : (special ? 0x3f800000u : ix);
+  const uint32_t tmp = iu - 0x3f330000u;
   const int i = (int)((tmp >> 19) & 15u);
   const int k = (int32_t)tmp >> 23;
-  const uint32_t iz = ix - (tmp & 0xff800000u);
+  const uint32_t iz = iu - (tmp & 0xff800000u);
   const double invc = tab[i].invc, logc = tab[i].logc;
   const double z = (double)__builtin_bit_cast(float, iz);
   const double r = __builtin_fma(z, invc, -1.0);
@@ -73,7 +71,13 @@ __host__ __device__ __forceinline__ float logf_exact(float x,
   double y = __builtin_fma(kLogfA1, r, kLogfA2);
   y = __builtin_fma(kLogfA0, r2, y);
   y = __builtin_fma(y, r2, y0 + r);
-  return (float)y;
+  float res = (float)y;
+  if (special && !sub) {
+    res = (ix << 1) == 0u ? -__builtin_inff()
+          : ix == 0x7f800000u ? x
+                             : __builtin_nanf("");
+  }
+  return res;
 }
 
 }  // namespace adsr

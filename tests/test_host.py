@@ -1,0 +1,253 @@
+"""Host side of the product (no GPU needed): the C ABI exports, the Python
+surface of the reference's PyO3 module, the host CSR builder against the
+oracle, kernel weights, the host instance of the device ln, and errors."""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_kats.json")))
+
+
+def _header_functions():
+    src = open(os.path.join(ROOT, "include", "adsr.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(adsr_\w+)\s*\(", src)))
+
+
+def test_c_abi_exports_every_declared_symbol():
+    import alphadia_search_rs._native as N
+    names = _header_functions()
+    assert len(names) >= 35
+    missing = [n for n in names if not hasattr(N.lib, n)]
+    assert not missing, missing
+
+
+def test_library_loads_without_gpu_and_reports_it():
+    import alphadia_search_rs as ads
+    import alphadia_search_rs._native as N
+    name = ads.get_optimal_simd_backend()
+    assert name in ("hip", "none")
+    assert (name == "hip") == N.device_available()
+
+
+def _dia_arrays(a):
+    return (np.array(a["spectrum_delta_scan_idx"], np.int64),
+            np.array(a["isolation_lower_mz"], np.float32),
+            np.array(a["isolation_upper_mz"], np.float32),
+            np.array(a["spectrum_peak_start_idx"], np.int64),
+            np.array(a["spectrum_peak_stop_idx"], np.int64),
+            np.array(a["spectrum_cycle_idx"], np.int64),
+            np.array(a["spectrum_rt"], np.float32),
+            np.array(a["peak_mz"], np.float32),
+            np.array(a["peak_intensity"], np.float32),
+            np.zeros((1, 1, 1, 1), np.float32))
+
+
+def test_dia_surface_and_kats():
+    import alphadia_search_rs as ads
+    empty = ads.DIAData()
+    assert empty.num_observations == 0 and empty.get_valid_observations(500.0) == []
+    case = GOLD["builder_three_windows"]
+    d = ads.DIAData.from_arrays(*_dia_arrays(case["arrays"]))
+    assert d.num_observations == case["num_observations"]
+    for mz, want in case["valid"]:
+        assert d.get_valid_observations(mz) == want
+    for o, w in enumerate(case["windows"]):
+        assert d._observation(o)["isolation_window"].tolist() == w
+    assert d.has_mobility is False and d.has_ms1 is False
+    assert d.mobility_values == [1e-6, 0.0]
+    assert np.array_equal(d.rt_values, np.array(case["arrays"]["spectrum_rt"], np.float32))
+    assert d.cycle.shape == (1, 1, 1, 1)
+    assert d.memory_footprint_bytes() > 0
+    assert abs(d.memory_footprint_mb() - d.memory_footprint_bytes() / 2 ** 20) < 1e-12
+    case = GOLD["builder_cycle_order"]
+    d = ads.DIAData.from_arrays(*_dia_arrays(case["arrays"]))
+    g = oracle.grid()
+    b = oracle.lib().orc_find_closest_index(oracle.p(g), len(g), case["mz"])
+    o = d._observation(0)
+    s, e = o["slice_starts"][b], o["slice_starts"][b + 1]
+    assert o["cycle_indices"][s:e].tolist() == case["cycles"]
+    assert o["intensities"][s:e].tolist() == case["intensities"]
+
+
+def test_dia_memory_footprint_formula():
+    """src/dia_data/mod.rs:90-108 with QuadrupoleObservation::memory_footprint_bytes."""
+    import alphadia_search_rs as ads
+    case = GOLD["builder_three_windows"]
+    d = ads.DIAData.from_arrays(*_dia_arrays(case["arrays"]))
+    B = len(oracle.grid())
+    n_rt = 2
+    want = n_rt * 4 + 24 + 3 * (8 + 8 + 72 + (B + 1) * 4) + 12 * 6
+    assert d.memory_footprint_bytes() == want
+
+
+@pytest.mark.parametrize("spec_kw", [
+    dict(n_cycles=120, n_windows=5, peaks_per_spectrum=900, n_targets=50),
+    dict(n_cycles=80, n_windows=3, peaks_per_spectrum=2500, n_targets=50),
+])
+def test_host_csr_equals_oracle(spec_kw):
+    import alphadia_search_rs as ads
+    import synthetic
+    gen = synthetic.generate(synthetic.config(0, **spec_kw))
+    # shuffle spectrum order so the (bin, cycle) sort must reorder cycles
+    dia = list(gen["dia"])
+    perm = np.random.default_rng(1).permutation(len(dia[0]))
+    for i in range(7):
+        dia[i] = dia[i][perm]
+    d = ads.DIAData.from_arrays(*dia)
+    od = oracle.Dia(dia)
+    assert d.num_observations == od.n_obs
+    for o in range(od.n_obs):
+        a, b = d._observation(o), od.observation(o)
+        for k in ("slice_starts", "cycle_indices", "intensities", "isolation_window"):
+            assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), (o, k)
+        assert a["num_cycles"] == b["num_cycles"]
+    rt, dt = d._rt_index()
+    assert np.array_equal(rt, od.rt()) and dt == np.float32(od.h.contents.delta_t)
+
+
+def test_host_grid_equals_oracle():
+    import alphadia_search_rs._native as N
+    n = N.lib.adsr_mz_grid(None, 0)
+    g = np.empty(n, np.float32)
+    N.lib.adsr_mz_grid(N.ptr(g), n)
+    assert np.array_equal(g, oracle.grid())
+
+
+def test_gaussian_kernel_equals_oracle():
+    import alphadia_search_rs._native as N
+    for args in [(3.0, 1.0, 20, 1.0), (4.7096, 1.0, 20, 1.0), (10.0, 1.0, 30, 60.0),
+                 (0.5, 1.0, 5, 1.0), (1.0, 1.0, 11, 1.0), (2.0, 1.5, 7, 0.5)]:
+        want = oracle.gaussian_kernel(*args)
+        got = np.zeros(64, np.float32)
+        n = N.lib.adsr_gaussian_kernel(*args, N.ptr(got), 64)
+        assert n == len(want)
+        assert np.array_equal(got[:n].view(np.uint32), want.view(np.uint32)), args
+    for sigma in (1.5, 2.0, 3.25):
+        got = np.zeros(20, np.float32)
+        N.lib.adsr_gaussian_kernel_sigma(sigma, 20, N.ptr(got), 20)
+        want = np.zeros(20, np.float32)
+        oracle.lib().orc_gaussian_kernel_sigma(sigma, 20, oracle.p(want), 20)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_host_instance_of_device_ln_matches_libm():
+    """The device ln (adsr_logf.hpp) compiled for the host vs glibc logf."""
+    import alphadia_search_rs._native as N
+    rng = np.random.default_rng(0)
+    bits = np.concatenate([
+        rng.integers(0, 2 ** 32, 200_000, dtype=np.uint64).astype(np.uint32),
+        rng.integers(0x3f800000, 0x4b000000, 200_000, dtype=np.uint32),  # [1, 2^23)
+        np.arange(0x3f800000, 0x3f800000 + 20000, dtype=np.uint32),
+        np.array([0, 1, 0x007fffff, 0x00800000, 0x7f7fffff, 0x7f800000, 0x80000000,
+                  0xbf800000, 0xff800000, 0x7fc00000], dtype=np.uint32)])
+    x = bits.view(np.float32)
+    f_dev = N.lib.adsr_logf_host
+    f_ref = oracle.lib().orc_logf
+    for v in x:
+        a, b = np.float32(f_dev(float(v))), np.float32(f_ref(float(v)))
+        if np.isnan(a) and np.isnan(b):
+            continue
+        assert a.view(np.uint32) == b.view(np.uint32), (hex(int(np.float32(v).view(np.uint32))), a, b)
+
+
+def test_selection_parameters_surface():
+    import alphadia_search_rs as ads
+    p = ads.SelectionParameters()
+    d = GOLD["selection_defaults"]
+    for k in ("fwhm_rt", "kernel_size", "peak_length", "mass_tolerance", "rt_tolerance",
+              "candidate_count", "top_k_fragments"):
+        assert getattr(p, k) == d[k], k
+    p.update(GOLD["selection_update"]["update"])
+    assert p.rt_tolerance == GOLD["selection_update"]["rt_tolerance"]
+    assert p.fwhm_rt == 3.0
+    with pytest.raises(AttributeError):
+        p.fwhm_rt = 1.0
+    with pytest.raises(TypeError):
+        p.update({"kernel_size": 2.5})
+    with pytest.raises(TypeError):
+        p.update({"mass_tolerance": "x"})
+    p.update({"kernel_size": 30, "mass_tolerance": 10})
+    assert p.kernel_size == 30 and p.mass_tolerance == 10.0
+
+
+def test_candidate_collection_kats():
+    import alphadia_search_rs as ads
+    row = GOLD["candidates_to_arrays"]["row"]
+    c = ads.CandidateCollection.from_arrays(*[[v] for v in row])
+    arrs = c.to_arrays()
+    assert len(arrs) == 9 and c.len() == 1 and not c.is_empty()
+    for i, (a, v) in enumerate(zip(arrs, row)):
+        assert a.dtype == (np.float32 if i == 2 else np.uint64)
+        assert a[0] == np.float32(v) if i == 2 else a[0] == v
+    cols = GOLD["candidates_roundtrip"]["columns"]
+    c = ads.CandidateCollection.from_arrays(*cols)
+    assert c.len() == 2
+    for a, want in zip(c.to_arrays(), cols):
+        assert np.array_equal(a, np.array(want, dtype=a.dtype))
+    with pytest.raises(ValueError):
+        ads.CandidateCollection.from_arrays([1], [1, 2], [0.1], [1], [1], [1], [1], [1], [1])
+    empty = ads.CandidateCollection()
+    assert empty.len() == 0 and empty.is_empty() and all(len(a) == 0 for a in empty.to_arrays())
+
+
+def test_speclib_surface_and_sorting():
+    import alphadia_search_rs as ads
+    case = GOLD["speclib_sort"]
+    n, nf = 4, 12
+    u8 = lambda v: np.full(nf, v, np.uint8)
+    lib = ads.SpecLibFlat.from_arrays(
+        np.array(case["precursor_idx"], np.uint64), np.array(case["precursor_mz"], np.float32),
+        np.array(case["precursor_mz"], np.float32), np.array(case["precursor_rt"], np.float32),
+        np.array(case["precursor_rt"], np.float32), np.array(case["precursor_naa"], np.uint8),
+        np.array(case["flat_frag_start_idx"], np.uint64), np.array(case["flat_frag_stop_idx"], np.uint64),
+        np.array(case["fragment_mz"], np.float32), np.array(case["fragment_mz"], np.float32),
+        np.array(case["fragment_intensity"], np.float32), u8(1), u8(1), u8(0), u8(1), u8(1), u8(98))
+    assert lib.num_precursors == n and lib.num_fragments == nf
+    assert ads.SpecLibFlat().num_precursors == 0
+
+
+def test_argument_errors():
+    import alphadia_search_rs as ads
+    case = GOLD["builder_three_windows"]
+    arrays = list(_dia_arrays(case["arrays"]))
+    bad = list(arrays)
+    bad[0] = bad[0].astype(np.int32)
+    with pytest.raises(TypeError):
+        ads.DIAData.from_arrays(*bad)
+    bad = list(arrays)
+    bad[1] = bad[1][:-1]
+    with pytest.raises(ValueError):
+        ads.DIAData.from_arrays(*bad)
+    bad = list(arrays)
+    bad[4] = bad[4].copy()
+    bad[4][-1] = 10 ** 6  # peak_stop beyond the peak arrays
+    with pytest.raises(ValueError):
+        ads.DIAData.from_arrays(*bad)
+    empty = [np.zeros(0, a.dtype) for a in arrays[:9]] + [arrays[9]]
+    with pytest.raises(ValueError):
+        ads.DIAData.from_arrays(*empty)
+    with pytest.raises(TypeError):
+        ads.PeakGroupSelection("not params")
+
+
+def test_search_fails_loudly_without_gpu(run0):
+    import alphadia_search_rs as ads
+    import alphadia_search_rs._native as N
+    if N.device_available():
+        pytest.skip("a gfx950 device is present")
+    d = ads.DIAData.from_arrays(*run0["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*run0["lib"])
+    with pytest.raises(N.AdsrError, match="gfx950"):
+        ads.PeakGroupSelection(ads.SelectionParameters()).search(d, lib)
+    with pytest.raises(ValueError):
+        ads.set_simd_backend("scalar")
+    with pytest.raises(N.AdsrError):
+        ads.set_simd_backend("hip")
