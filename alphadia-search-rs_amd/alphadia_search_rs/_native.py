@@ -21,6 +21,17 @@ if not os.path.exists(LIB_PATH):
         "(or __graft_entry__.build())"
     )
 
+# One HIP runtime per process.  libadsr.so needs libamdhip64.so.7; PyTorch
+# ships its own copy under that soname.  Loading torch first makes the
+# dynamic loader bind libadsr.so to torch's runtime, so device pointers,
+# streams and device initialisation are shared.  Loaded the other way round,
+# a process holds two runtimes, and the second one to initialise can find
+# no device.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = C.CDLL(LIB_PATH)
 
 c_size = C.c_size_t
@@ -97,6 +108,11 @@ _sig("adsr_select_scores", I, vp, vp, C.POINTER(SelectionParamsC), c_size,
      c_size, c_size, vp, vp, vp)
 _sig("adsr_select_plan", I, vp, vp, C.POINTER(SelectionParamsC), c_size, c_size,
      vp, vp, vp, vp, c_size)
+_sig("adsr_diff_create", I, vp, vp, C.POINTER(SelectionParamsC), C.POINTER(vp))
+_sig("adsr_diff_forward", I, vp, vp, vp, vp, vp)
+_sig("adsr_diff_backward", I, vp, vp, vp, vp)
+_sig("adsr_diff_timings", I, vp, C.POINTER(C.c_double), C.POINTER(C.c_double))
+_sig("adsr_diff_free", None, vp)
 _sig("adsr_candidates_len", c_size, vp)
 _sig("adsr_candidates_export", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
 _sig("adsr_candidates_from_arrays", I, c_size, vp, vp, vp, vp, vp, vp, vp, vp,

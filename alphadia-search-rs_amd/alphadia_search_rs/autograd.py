@@ -1,0 +1,145 @@
+"""Differentiable candidate selection: gradients of each precursor's rank-0
+selection score with respect to a Gaussian sigma per quadrupole window and
+the library fragment intensities.
+
+The reference has no backward pass; this module extends
+``PeakGroupSelection.search`` (src/peak_group_selection/mod.rs:99-123).  The
+forward pass is that search with ``sigma_per_window`` and the weights in
+place of ``fragment_intensity``; its output is s* per precursor, the score of
+the rank-0 candidate (0 for a precursor without candidates).  The backward
+holds each precursor's apex and its selected fragments fixed (both are
+piecewise constant in the inputs), recomputes the XIC window around the
+apex on the device and differentiates
+
+    s* = sum_f w_f * ln(1 + Y_f),  Y_f = sum_d k_d(sigma) * X_f[apex + d]
+
+with k the normalised Gaussian of src/kernel/mod.rs:32-56.  Both passes run
+in libadsr.so (adsr_diff_forward / adsr_diff_backward); there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import DIAData, SelectionParameters, SpecLibFlat
+from . import _native as N
+from ._native import check
+
+
+def _dev_ptr(t: torch.Tensor):
+    return C.c_void_p(t.data_ptr())
+
+
+class SelectionGradState:
+    """Device state of one (DIAData, SpecLibFlat, SelectionParameters)
+    triple for repeated forward/backward passes.
+
+    Requirements: a gfx950 device, and fragment ranges that no two
+    precursors share (per-fragment gradients are scattered without atomics).
+    Rows of s* follow ascending ``precursor_idx`` (stable), the order of the
+    candidate table.
+    """
+
+    def __init__(self, dia: DIAData, lib: SpecLibFlat,
+                 params: SelectionParameters):
+        if not N.device_available():
+            raise RuntimeError("differentiable selection needs a gfx950 device")
+        if not dia._h or not lib._h:
+            raise ValueError("empty DIAData or SpecLibFlat")
+        self._h = None
+        self.num_observations = dia.num_observations
+        self.num_precursors = lib.num_precursors
+        self.num_fragments = lib.num_fragments
+        self._dia, self._lib = dia, lib  # keep the handles alive
+        c = params._c()
+        out = C.c_void_p()
+        check(N.lib.adsr_diff_create(dia._h, lib._h, C.byref(c), C.byref(out)))
+        self._h = out.value
+        self._fwd_done = False
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib.adsr_diff_free(h)
+            self._h = None
+
+    def _check_inputs(self, sigma: torch.Tensor, weights: torch.Tensor):
+        if sigma.dtype != torch.float32 or sigma.numel() != self.num_observations:
+            raise ValueError(
+                f"sigma must be float32 with {self.num_observations} values")
+        if (weights.dtype != torch.float32 or not weights.is_cuda
+                or weights.numel() != self.num_fragments):
+            raise ValueError(
+                f"weights must be a float32 device tensor with "
+                f"{self.num_fragments} values")
+
+    def forward(self, sigma: torch.Tensor, weights: torch.Tensor):
+        """Return (s_star f32[P], has_candidate bool[P]) on the device."""
+        self._check_inputs(sigma, weights)
+        w = weights.detach().contiguous()
+        sig = np.ascontiguousarray(
+            sigma.detach().to("cpu", torch.float32).numpy().reshape(-1))
+        dev = w.device
+        s_star = torch.empty(self.num_precursors, dtype=torch.float32, device=dev)
+        has = torch.empty(self.num_precursors, dtype=torch.uint8, device=dev)
+        torch.cuda.synchronize(dev)  # the library runs on its own stream
+        check(N.lib.adsr_diff_forward(self._h, N.ptr(sig), _dev_ptr(w),
+                                      _dev_ptr(s_star), _dev_ptr(has)))
+        self._fwd_done = True
+        return s_star, has.bool()
+
+    def backward(self, grad_s: torch.Tensor):
+        """Return (grad_weights f32[n_fragments] on the device,
+        grad_sigma f64[n_observations] on the host) for dLoss/ds* = grad_s."""
+        if not self._fwd_done:
+            raise RuntimeError("backward before forward")
+        g = grad_s.detach().to(torch.float32).contiguous()
+        if not g.is_cuda or g.numel() != self.num_precursors:
+            raise ValueError("grad_s must be a device tensor with one value "
+                             "per precursor")
+        gw = torch.empty(self.num_fragments, dtype=torch.float32, device=g.device)
+        gs = np.zeros(self.num_observations, dtype=np.float64)
+        torch.cuda.synchronize(g.device)
+        check(N.lib.adsr_diff_backward(self._h, _dev_ptr(g), _dev_ptr(gw),
+                                       N.ptr(gs)))
+        return gw, torch.from_numpy(gs)
+
+    def timings(self):
+        """Device milliseconds of the last forward and backward pass."""
+        f, b = C.c_double(), C.c_double()
+        check(N.lib.adsr_diff_timings(self._h, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+
+class RankZeroScore(torch.autograd.Function):
+    """s* = RankZeroScore.apply(state, sigma, weights).
+
+    ``sigma``: float32[n_observations] (host or device); ``weights``:
+    float32[n_fragments] on the device.  Gradients flow to both; the sigma
+    gradient is accumulated in float64 and returned in sigma's dtype."""
+
+    @staticmethod
+    def forward(ctx, state: SelectionGradState, sigma, weights):
+        s_star, _ = state.forward(sigma, weights)
+        ctx.state = state
+        ctx.sigma_meta = (sigma.dtype, sigma.device)
+        return s_star
+
+    @staticmethod
+    def backward(ctx, grad_s):
+        gw, gs = ctx.state.backward(grad_s)
+        dt, dev = ctx.sigma_meta
+        return None, gs.to(device=dev, dtype=dt), gw
+
+
+def rank_zero_scores(state: SelectionGradState, sigma, weights):
+    """s* per precursor (rows in candidate-table order), differentiable."""
+    return RankZeroScore.apply(state, sigma, weights)
+
+
+def hinge_loss(s_star: torch.Tensor, targets: torch.Tensor,
+               decoys: torch.Tensor, margin: float = 1.0) -> torch.Tensor:
+    """sum over pairs of max(0, margin - (s*_target - s*_decoy))."""
+    return torch.clamp(margin - (s_star[targets] - s_star[decoys]), min=0.0).sum()

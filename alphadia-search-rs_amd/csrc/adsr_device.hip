@@ -504,13 +504,13 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) {
 // followed, within its bin, by another entry with cycle == c0: the
 // reference's binary_search(c0) starts at the last such entry
 // (src/quadrupole_observation/mod.rs:92-94).  Rare; reads global memory.
-__device__ __forceinline__ bool skipped_duplicate(const SelArgs &a, uint32_t o,
+__device__ __forceinline__ bool skipped_duplicate(const KDia &dia, uint32_t o,
                                                   uint64_t base, uint32_t b0,
                                                   uint32_t e, uint32_t c0) {
-  const uint32_t *ss = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
+  const uint32_t *ss = dia.ss + (size_t)o * (dia.n_grid + 1);
   uint32_t b = b0;
   while (ss[b + 1] <= e) ++b;
-  return e + 1 < ss[b + 1] && a.dia.cycles[base + e + 1] == c0;
+  return e + 1 < ss[b + 1] && dia.cycles[base + e + 1] == c0;
 }
 
 // One dilation step of a 192-bit window (prev, cur, next) by +-s.
@@ -603,16 +603,18 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
       const uint4 rec = a.plan.rec[row];
       const uint32_t c0 = uni(rec.x), c1 = uni(rec.y);
       const uint32_t nsel = uni(rec.z & 0xffffu), nobs = uni(rec.z >> 16);
+      // Stores next to a `continue` or the loop's back edge are made by all
+      // lanes (same value, same address): a lane-0-only region there lets
+      // the compiler thread it into the lane-0-only queue grab, after which
+      // lanes 1..63 no longer follow lane 0 through the loop.
       if (c1 <= c0) {  // mod.rs:146-148 -> None
-        if (lane == 0) a.cand_count[row] = 0;
+        a.cand_count[row] = 0;
         continue;
       }
       const uint32_t L = c1 - c0;
       if (L > a.lcap) {  // excluded by the host bound; flagged if it happens
-        if (lane == 0) {
-          a.cand_count[row] = 0;
-          atomicOr(a.status, (uint32_t)kStWindowTooLong);
-        }
+        a.cand_count[row] = 0;
+        atomicOr(a.status, (uint32_t)kStWindowTooLong);
         continue;
       }
       const uint32_t nw = (L + 63) / 64;  // 64-cycle words per row
@@ -685,7 +687,7 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
               bool hit = c >= c0 && c < c1;
               if (hit && c == c0) {
                 const uint32_t e = rdlane(rx, tf[u]) + tk[u] * kWave + lane;
-                if (skipped_duplicate(a, o, base, rdlane(rb0, tf[u]), e, c0))
+                if (skipped_duplicate(a.dia, o, base, rdlane(rb0, tf[u]), e, c0))
                   hit = false;
               }
               hitm |= (hit ? 1u : 0u) << u;
@@ -879,7 +881,7 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
         }
         ++ncand;
       }
-      if (lane == 0) a.cand_count[row] = ncand;
+      a.cand_count[row] = ncand;
       __builtin_amdgcn_wave_barrier();
     }
   }
@@ -1004,6 +1006,177 @@ __global__ __launch_bounds__(256) void traffic_kernel(KDia dia, KLib lib,
   atomicAdd(&part[2], hits);
   __syncthreads();
   if (threadIdx.x < 3) atomicAdd(&acc[threadIdx.x], part[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// differentiable selection: backward through the convolution and the
+// weighted ln(1+x) score at each precursor's rank-0 apex
+// ---------------------------------------------------------------------------
+struct BwdArgs {
+  KDia dia;
+  KLib lib;
+  KParams prm;
+  KPlan plan;
+  uint32_t n_rows;
+  uint32_t cc;
+  const uint32_t *cand_center;
+  const uint32_t *cand_count;
+  const float *grad_s;  // dLoss/ds_star per row
+  const double *dk;     // d(kernel tap)/d(sigma), (half+1) per kernel index
+  float *grad_w;        // per fragment
+  double *contrib;      // per row: dLoss/ds_star * ds_star/dsigma
+  uint32_t *queue;
+};
+
+constexpr int kWin = 32;  // cycles kept around the apex (2*half+1 <= 31)
+
+__global__ void fill_f32_kernel(float *p, uint64_t n, float v) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+__global__ void sstar_kernel(const float *cs, const uint32_t *cnt, uint32_t cc,
+                             uint32_t n, float *s_star, uint8_t *has) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const bool h = cnt[r] > 0;
+  s_star[r] = h ? cs[(size_t)r * cc] : 0.0f;
+  if (has) has[r] = h ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kWave) void diff_backward_kernel(BwdArgs a) {
+  __shared__ __align__(16) float xw[kMaxSel * kWin];
+  __shared__ uint16_t own[kMaxSel * kWin];
+  __shared__ double part[kMaxSel];
+  const int lane = lane_id();
+  const uint32_t half = a.prm.half;
+  const uint32_t ks = a.plan.ks;
+  for (;;) {
+    uint32_t row = 0;
+    if (lane == 0) row = atomicAdd(a.queue, 1u);
+    row = uni(__shfl(row, 0, kWave));
+    if (row >= a.n_rows) break;
+    const uint4 rec = a.plan.rec[row];
+    const uint32_t c0 = uni(rec.x), c1 = uni(rec.y);
+    const uint32_t nsel = uni(rec.z & 0xffffu), nobs = uni(rec.z >> 16);
+    // every value that steers control flow is read as a wave-uniform
+    // scalar: a skip decided on per-lane loads is a divergent `continue`
+    const float g = __builtin_bit_cast(
+        float, uni(__builtin_bit_cast(uint32_t, a.grad_s[row])));
+    const uint32_t ncand = uni(a.cand_count[row]);
+    // No lane-0-only code and no `continue` in this loop besides the queue
+    // grab: the contribution is written by every lane (same value, same
+    // address), so all 64 lanes stay together around the back edge.
+    double total = 0.0;
+    if (!(ncand == 0 || g == 0.0f || c1 <= c0 || nobs == 0 || nsel == 0)) {
+      const uint32_t istar = uni(a.cand_center[(size_t)row * a.cc]) - c0;
+      const uint32_t hc0 = c0 + istar - half, hc1 = hc0 + 2 * half + 1;
+      for (uint32_t i = lane; i < nsel * kWin; i += kWave) xw[i] = 0.0f;
+      __builtin_amdgcn_wave_barrier();
+      // XIC around the apex, accumulated in the order of the forward pass
+      for (uint32_t j = 0; j < nobs; ++j) {
+        const uint32_t o = uni(a.plan.obs[(size_t)row * kObsMax + j]);
+        const uint64_t base = a.dia.entry_base[o];
+        const size_t ridx = ((size_t)row * kObsMax + j) * ks;
+        uint32_t rx = 0, rlen = 0, rb0 = 0;
+        if (lane < (int)nsel) {
+          const uint2 r = a.plan.rng[ridx + lane];
+          rx = r.x;
+          rlen = r.y - r.x;
+          rb0 = a.plan.b0[ridx + lane];
+        }
+        const uint32_t nt = (rlen + kWave - 1) / kWave;
+        const uint32_t tincl = wave_incl_scan(nt);
+        const uint32_t tp = tincl - nt;
+        const uint32_t NT = rdlane(tincl, nsel - 1);
+        for (uint32_t T = 0; T < NT; ++T) {
+          const uint64_t fm = ballot64(lane < (int)nsel && nt > 0 && tp <= T);
+          const uint32_t f = fm ? 63u - (uint32_t)__builtin_clzll(fm) : 0u;
+          const uint32_t k = (T - rdlane(tp, f)) * kWave + lane;
+          bool hit = false;
+          uint32_t c = 0, e = 0;
+          if (k < rdlane(rlen, f)) {
+            e = rdlane(rx, f) + k;
+            c = a.dia.cycles[base + e];
+            hit = c >= hc0 && c < hc1 && c >= c0 && c < c1;
+            if (hit && c == c0 &&
+                skipped_duplicate(a.dia, o, base, rdlane(rb0, f), e, c0))
+              hit = false;
+          }
+          const uint64_t hm = ballot64(hit);
+          if (hm == 0) continue;
+          const float inten = hit ? a.dia.intens[base + e] : 0.0f;
+          const uint32_t t = hit ? c - hc0 : 0u;
+          if (hit) own[f * kWin + t] = (uint16_t)lane;
+          __builtin_amdgcn_wave_barrier();
+          const bool dup = hit && own[f * kWin + t] != (uint16_t)lane;
+          if (ballot64(dup) == 0) {
+            if (hit) xw[f * kWin + t] = xw[f * kWin + t] + inten;
+          } else {
+            uint64_t it = hm;
+            while (it) {
+              const int j2 = __builtin_ctzll(it);
+              it &= it - 1;
+              if (lane == j2) xw[f * kWin + t] = xw[f * kWin + t] + inten;
+              __builtin_amdgcn_wave_barrier();
+            }
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      // per fragment: Y at the apex, dY/dsigma, ln(Y+1)
+      double cf = 0.0;
+      if (lane < (int)nsel) {
+        const float *x = xw + lane * kWin + half;  // x[0] = apex cycle
+        const float *kk = a.prm.kernels + (size_t)rec.w * a.prm.klen;
+        const double *dk = a.dk + (size_t)rec.w * (half + 1);
+        float y = 0.0f;
+        y = y + x[0] * kk[half];
+        double dy = (double)x[0] * dk[half];
+        for (uint32_t jj = 0; jj < half; ++jj) {
+          const uint32_t d = half - jj;
+          y = y + (x[-(int)d] + x[d]) * kk[jj];
+          dy += ((double)x[-(int)d] + (double)x[d]) * dk[jj];
+        }
+        // d s*/d w_f = ln(1 + Y_f), evaluated as log1p in f64: the f32
+        // logf(Y + 1) of the forward loses the low bits of Y when Y << 1
+        const double lnv = log1p((double)y);
+        const uint32_t sel = a.plan.sel[(size_t)row * ks + lane];
+        const uint64_t fs = a.lib.f_start[row];
+        a.grad_w[fs + sel] = (float)((double)g * lnv);
+        const double wf = (double)a.plan.sel_w[(size_t)row * ks + lane];
+        cf = wf * dy / ((double)y + 1.0);
+        part[lane] = cf;
+      }
+      __builtin_amdgcn_wave_barrier();
+      double acc = 0.0;  // fixed fragment order, identical in every lane
+      for (uint32_t f = 0; f < nsel; ++f) acc += part[f];
+      total = (double)g * acc;
+      __builtin_amdgcn_wave_barrier();
+    }
+    a.contrib[row] = total;
+  }
+}
+
+// grad_sigma[w] = sum of contrib over the rows whose kernel is window w's,
+// in a fixed order (per-thread strided sums, then a fixed tree).
+__global__ __launch_bounds__(256) void sigma_reduce_kernel(const uint4 *rec,
+                                                           const double *contrib,
+                                                           uint32_t n,
+                                                           double *out) {
+  __shared__ double red[256];
+  const uint32_t w = blockIdx.x;
+  double acc = 0.0;
+  for (uint32_t r = threadIdx.x; r < n; r += 256)
+    if (rec[r].w == w + 1) acc += contrib[r];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[w] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1986,6 +2159,221 @@ extern "C" adsr_status adsr_select_plan(const adsr_dia *dc, const adsr_lib *lc,
   ws_free(w);
   return st;
 }
+
+// ---- differentiable selection ---------------------------------------------
+namespace adsr {
+struct DiffState {
+  adsr_dia *d = nullptr;
+  adsr_lib *l = nullptr;
+  adsr_selection_params p{};
+  std::vector<float> sig;
+  Workspace w;
+  SearchPlan sp;
+  double *dk = nullptr;
+  double *contrib = nullptr;
+  double *gsig = nullptr;
+  uint32_t *bq = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float fwd_ms = 0.0f, bwd_ms = 0.0f;
+  bool have_forward = false;
+};
+
+// d k[j] / d sigma for the taps j = 0..half of gaussian_kernel_1d
+// (src/kernel/mod.rs:32-56): k = e / sum(e), e_j = exp(-x_j^2 / (2 sigma^2)),
+// x_j = j - half over the kernel_width taps, so
+// dk_j/dsigma = k_j (x_j^2 - sum_m k_m x_m^2) / sigma^3.
+static void kernel_dsigma(double sigma, size_t width, double *out) {
+  const int half = (int)(width / 2);
+  std::vector<double> e(width);
+  double S = 0.0;
+  for (int j = 0; j < (int)width; ++j) {
+    const double x = j - half;
+    e[j] = std::exp(-x * x / (2.0 * sigma * sigma));
+    S += e[j];
+  }
+  double m2 = 0.0;
+  for (int j = 0; j < (int)width; ++j) {
+    const double x = j - half;
+    m2 += e[j] / S * x * x;
+  }
+  for (int j = 0; j <= half; ++j) {
+    const double x = j - half;
+    out[j] = (e[j] / S) * (x * x - m2) / (sigma * sigma * sigma);
+  }
+}
+}  // namespace adsr
+extern "C" {
+
+adsr_status adsr_diff_create(const adsr_dia *dc, const adsr_lib *lc,
+                             const adsr_selection_params *p, adsr_diff **out) {
+  clear_error();
+  if (!dc || !lc || !p || !out)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  adsr_dia *d = const_cast<adsr_dia *>(dc);
+  adsr_lib *l = const_cast<adsr_lib *>(lc);
+  if (!l->frag_ranges_disjoint)
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "precursors share fragment indices; per-fragment "
+                     "gradients need disjoint fragment ranges");
+  adsr_status st = upload_dia(d);
+  if (!st) st = upload_lib(l);
+  if (st) return st;
+  auto *s = new DiffState();
+  s->d = d;
+  s->l = l;
+  s->p = *p;
+  s->sig.assign(d->n_obs, 2.0f);
+  s->p.sigma_per_window = s->sig.data();
+  s->p.n_sigma_per_window = d->n_obs;
+  s->p.rt_tolerance_per_precursor = nullptr;
+  // the per-precursor tolerances are uploaded into the workspace here; the
+  // caller's array only has to live during this call
+  s->p.rt_tolerance_per_precursor = p->rt_tolerance_per_precursor;
+  st = build_search(d, l, &s->p, s->w, s->sp);
+  s->p.rt_tolerance_per_precursor = nullptr;
+  const size_t nk = 1 + d->n_obs;
+  const size_t half = s->sp.prm.half;
+  auto fail = [&](adsr_status e) {
+    ws_free(s->w);
+    (void)hipFree(s->dk);
+    (void)hipFree(s->contrib);
+    (void)hipFree(s->gsig);
+    (void)hipFree(s->bq);
+    delete s;
+    return e;
+  };
+  if (st) return fail(st);
+  if (hipMalloc(&s->dk, nk * (half + 1) * 8) != hipSuccess ||
+      hipMalloc(&s->contrib, std::max<size_t>(l->n_prec, 1) * 8) != hipSuccess ||
+      hipMalloc(&s->gsig, std::max<size_t>(d->n_obs, 1) * 8) != hipSuccess ||
+      hipMalloc(&s->bq, 64) != hipSuccess)
+    return fail(set_error(ADSR_ERR_OUT_OF_MEMORY, "diff buffers"));
+  for (auto &e : s->ev)
+    if (hipEventCreate(&e) != hipSuccess)
+      return fail(set_error(ADSR_ERR_DEVICE, "event creation failed"));
+  *out = reinterpret_cast<adsr_diff *>(s);
+  return ADSR_OK;
+}
+
+adsr_status adsr_diff_forward(adsr_diff *h, const float *sigma,
+                              const float *weights, float *s_star,
+                              uint8_t *has) {
+  clear_error();
+  auto *s = reinterpret_cast<DiffState *>(h);
+  if (!s || !sigma || !weights || !s_star)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  const adsr_dia *d = s->d;
+  const adsr_lib *l = s->l;
+  const size_t klen = s->sp.prm.klen, half = s->sp.prm.half;
+  const size_t nk = 1 + d->n_obs;
+  s->sig.assign(sigma, sigma + d->n_obs);
+  for (size_t o = 0; o < d->n_obs; ++o)
+    if (!(s->sig[o] > 0.0f) || !std::isfinite(s->sig[o]))
+      return set_error(ADSR_ERR_INVALID_ARGUMENT, "sigma[%zu] must be > 0", o);
+  // taps: index 0 from fwhm_rt, 1 + o from sigma[o]
+  std::vector<float> taps(nk * klen);
+  adsr_gaussian_kernel(s->p.fwhm_rt, 1.0f, s->p.kernel_size, 1.0f, taps.data(), klen);
+  for (size_t o = 0; o < d->n_obs; ++o)
+    adsr_gaussian_kernel_sigma(s->sig[o], klen, taps.data() + (1 + o) * klen, klen);
+  std::vector<double> dk(nk * (half + 1));
+  kernel_dsigma((double)(s->p.fwhm_rt / 2.3548f), klen, dk.data());
+  for (size_t o = 0; o < d->n_obs; ++o)
+    kernel_dsigma((double)s->sig[o], klen, dk.data() + (1 + o) * (half + 1));
+  HIP_TRY(hipMemcpy(s->w.kernels, taps.data(), taps.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->dk, dk.data(), dk.size() * 8, hipMemcpyHostToDevice));
+  s->sp.lib.f_int = weights;
+  LaunchShape shape{4, 2};
+  const uint32_t n = (uint32_t)l->n_prec;
+  HIP_TRY(hipEventRecord(s->ev[0], s->w.stream));
+  adsr_status st = enqueue(s->w, s->sp, n, shape, nullptr, 0, 0, 0, false);
+  if (st) return st;
+  if (n)
+    sstar_kernel<<<(n + 255) / 256, 256, 0, s->w.stream>>>(
+        s->w.cand_score, s->w.cand_count, (uint32_t)s->sp.prm.cand_count, n,
+        s_star, has);
+  HIP_TRY(hipEventRecord(s->ev[1], s->w.stream));
+  st = check_status(s->w);
+  if (st) return st;
+  (void)hipEventElapsedTime(&s->fwd_ms, s->ev[0], s->ev[1]);
+  s->have_forward = true;
+  return ADSR_OK;
+}
+
+adsr_status adsr_diff_backward(adsr_diff *h, const float *grad_s,
+                               float *grad_w, double *grad_sigma) {
+  clear_error();
+  auto *s = reinterpret_cast<DiffState *>(h);
+  if (!s || !grad_s || !grad_w || !grad_sigma)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!s->have_forward)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "backward before forward");
+  const adsr_dia *d = s->d;
+  const adsr_lib *l = s->l;
+  hipStream_t st_ = s->w.stream;
+  const uint32_t n = (uint32_t)l->n_prec;
+  HIP_TRY(hipEventRecord(s->ev[2], st_));
+  if (l->n_frag)
+    fill_f32_kernel<<<(uint32_t)((l->n_frag + 255) / 256), 256, 0, st_>>>(
+        grad_w, l->n_frag, 0.0f);
+  reset_queue_kernel<<<1, 64, 0, st_>>>(s->bq, s->bq + 1);
+  if (n) {
+    BwdArgs a;
+    a.dia = s->sp.dia;
+    a.lib = s->sp.lib;
+    a.prm = s->sp.prm;
+    a.plan = s->sp.plan;
+    a.n_rows = n;
+    a.cc = (uint32_t)s->sp.prm.cand_count;
+    a.cand_center = s->w.cand_center;
+    a.cand_count = s->w.cand_count;
+    a.grad_s = grad_s;
+    a.dk = s->dk;
+    a.grad_w = grad_w;
+    a.contrib = s->contrib;
+    a.queue = s->bq;
+    const int per_cu =
+        occupancy_blocks((const void *)diff_backward_kernel, kWave, 0);
+    const uint32_t blocks =
+        (uint32_t)std::min<uint64_t>((uint64_t)per_cu * s->sp.cus, n);
+    diff_backward_kernel<<<blocks, kWave, 0, st_>>>(a);
+    if (d->n_obs)
+      sigma_reduce_kernel<<<(uint32_t)d->n_obs, 256, 0, st_>>>(
+          s->sp.plan.rec, s->contrib, n, s->gsig);
+  }
+  HIP_TRY(hipEventRecord(s->ev[3], st_));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st_));
+  std::vector<double> g(d->n_obs, 0.0);
+  if (n && d->n_obs)
+    HIP_TRY(hipMemcpy(g.data(), s->gsig, d->n_obs * 8, hipMemcpyDeviceToHost));
+  std::memcpy(grad_sigma, g.data(), d->n_obs * 8);
+  (void)hipEventElapsedTime(&s->bwd_ms, s->ev[2], s->ev[3]);
+  return ADSR_OK;
+}
+
+adsr_status adsr_diff_timings(const adsr_diff *h, double *fwd, double *bwd) {
+  const auto *s = reinterpret_cast<const DiffState *>(h);
+  if (!s) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (fwd) *fwd = s->fwd_ms;
+  if (bwd) *bwd = s->bwd_ms;
+  return ADSR_OK;
+}
+
+void adsr_diff_free(adsr_diff *h) {
+  auto *s = reinterpret_cast<DiffState *>(h);
+  if (!s) return;
+  ws_free(s->w);
+  (void)hipFree(s->dk);
+  (void)hipFree(s->contrib);
+  (void)hipFree(s->gsig);
+  (void)hipFree(s->bq);
+  for (auto &e : s->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete s;
+}
+
+}  // extern "C"
 
 // ---- per-stage operations --------------------------------------------------
 template <class F>

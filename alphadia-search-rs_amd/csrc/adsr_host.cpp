@@ -525,6 +525,16 @@ extern "C" adsr_status adsr_lib_create(
   for (size_t i = 0; i < np; ++i)
     l->max_raw_fragments =
         std::max<uint64_t>(l->max_raw_fragments, l->frag_stop[i] - l->frag_start[i]);
+  {
+    std::vector<std::pair<uint64_t, uint64_t>> r;
+    r.reserve(np);
+    for (size_t i = 0; i < np; ++i)
+      if (l->frag_stop[i] > l->frag_start[i])
+        r.emplace_back(l->frag_start[i], l->frag_stop[i]);
+    std::sort(r.begin(), r.end());
+    for (size_t i = 1; i < r.size(); ++i)
+      if (r[i].first < r[i - 1].second) l->frag_ranges_disjoint = false;
+  }
   *out = l;
   return ADSR_OK;
 }

@@ -107,6 +107,9 @@ struct adsr_lib {
   std::vector<uint8_t> fragment_cardinality, fragment_charge,
       fragment_loss_type, fragment_number, fragment_position, fragment_type;
   uint64_t max_raw_fragments = 0;
+  // no fragment index is shared by two precursors (needed to scatter
+  // per-fragment gradients without atomics)
+  bool frag_ranges_disjoint = true;
   adsr::DeviceLib dev;
   std::mutex mu;
 };

@@ -1,16 +1,23 @@
 """Throughput of PeakGroupSelection.search on one MI355X.
 
-Default workload: 6,000 cycles x 60 windows, 1,500 peaks per spectrum
-(~554M peaks with planted elution profiles), 400k precursors (200k targets +
-200k shifted decoys), 12 fragments, +-150 cycles, 10 ppm, top-3.
+Workloads (--workload, default 1):
+  1  6,000 cycles x 60 windows, 1,500 peaks per spectrum (~554M peaks with
+     planted elution profiles), 400k precursors (200k targets + 200k shifted
+     decoys), 12 fragments, +-150 cycles, 10 ppm, top-3
+  2  as 1 with 6-24 fragments, per-precursor half-windows of 30-200 cycles,
+     clustered precursor m/z and top_k_fragments 24 (launch shape autotuned)
+  3  the data of 2, forward + backward: rank-0 scores, a hinge loss over the
+     target/decoy pairs and its gradient with respect to a Gaussian sigma
+     per window and every fragment intensity (value = fwd+bwd precursors/s)
 
 A step is one full search over the library (plan, bins, select, compaction
-kernels) with the run and the library already resident in HBM.  The value
-is precursors per second of the device pipeline; the host copy of the
-candidate table is reported separately.  The oracle (the C restatement of
-the reference's scalar path) is timed on the host's cores on a sample of the
-same library for cpu_baseline, and its candidates are compared with the
-device's on that sample.
+kernels) with the run and the library already resident in HBM; for workload
+3 it is the forward, the loss and the backward.  The value is precursors per
+second of the device pipeline; the host copy of the candidate table is
+reported separately.  The oracle (the C restatement of the reference's
+scalar path) is timed on the host's cores on a sample of the same library
+for cpu_baseline, and its candidates are compared with the device's on that
+sample.
 
 Prints one JSON line.
 """
@@ -34,6 +41,49 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def fwd_bwd(args, gen, dia, lib, params):
+    """Workload 3: time forward (rank-0 scores) + hinge loss + backward.
+
+    Returns (ms per step, forward device ms, backward device ms, extras)."""
+    import numpy as np
+    import torch
+
+    from alphadia_search_rs.autograd import (SelectionGradState, hinge_loss,
+                                             rank_zero_scores)
+    st = SelectionGradState(dia, lib, params)
+    sigma = torch.full((dia.num_observations,), 2.0, requires_grad=True)
+    w = torch.tensor(gen["lib"][10], device="cuda", requires_grad=True)
+    pairs = torch.tensor(np.asarray(gen["pairs"], np.int64), device="cuda")
+    t_idx, d_idx = pairs[:, 0].contiguous(), pairs[:, 1].contiguous()
+    stats = {}
+
+    def step():
+        sigma.grad = None
+        w.grad = None
+        s = rank_zero_scores(st, sigma, w)
+        loss = hinge_loss(s, t_idx, d_idx, args.margin)
+        loss.backward()
+        return s, loss
+
+    for _ in range(max(args.warmup, 1)):
+        s, loss = step()
+    torch.cuda.synchronize()
+    fwd_ms = bwd_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s, loss = step()
+        f, b = st.timings()
+        fwd_ms += f
+        bwd_ms += b
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    stats["loss"] = float(loss.detach().cpu())
+    stats["active_pairs"] = int(((args.margin - (s[t_idx] - s[d_idx])) > 0).sum().cpu())
+    stats["grad_sigma_norm"] = float(sigma.grad.norm())
+    stats["grad_weight_nonzero"] = int((w.grad != 0).sum().cpu())
+    return step_ms, fwd_ms / args.steps, bwd_ms / args.steps, stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,7 +96,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads-per-block", type=int, default=0)
     ap.add_argument("--precursors-per-wave", type=int, default=0)
+    ap.add_argument("--margin", type=float, default=1.0,
+                    help="hinge margin of the workload-3 loss")
     args = ap.parse_args()
+    if args.workload not in (1, 2, 3):
+        print("bench.py: --workload is 1, 2 or 3", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU split is described in DESIGN.md", file=sys.stderr)
@@ -62,7 +117,8 @@ def main():
         print("bench.py: no gfx950 device visible", file=sys.stderr)
         sys.exit(1)
     os.environ.setdefault("ADSR_QUIET", "1")
-    spec = synthetic.config(args.workload)
+    # workload 3 runs on the data of workload 2 (same generator seeds)
+    spec = synthetic.config(2 if args.workload == 3 else args.workload)
     t0 = time.time()
     last = [0.0]
 
@@ -112,6 +168,13 @@ def main():
     api_s = (time.perf_counter() - t2) / api_reps
     log(f"search() incl. candidate table on host: {api_s * 1e3:.1f} ms")
 
+    fb = None
+    if args.workload == 3:
+        log("timing forward + hinge loss + backward")
+        fb = fwd_bwd(args, gen, dia, lib, params)
+        log(f"fwd+bwd step {fb[0]:.3f} ms (forward {fb[1]:.3f} ms, backward "
+            f"{fb[2]:.3f} ms on the device), {fb[3]}")
+
     abytes = C.c_double()
     ents = C.c_double()
     hits = C.c_double()
@@ -119,6 +182,16 @@ def main():
                                                 C.byref(abytes), C.byref(ents),
                                                 C.byref(hits)))
     achieved = abytes.value / (kern_ms.value / 1e3) / 1e9
+    fwd_rate = value
+    ms = step_ms.value
+    kind = "plain selection"
+    if args.workload == 2:
+        kind = "uneven batched selection (per-precursor RT windows)"
+    if fb is not None:
+        ms = fb[0]
+        value = n_prec / (ms / 1e3)
+        kind = ("forward + hinge loss + backward (d/dsigma per window, "
+                "d/dintensity per fragment) on the uneven batched selection")
     result = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -126,18 +199,18 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(step_ms.value, 4),
+        "ms_per_step": round(ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator, planted elution profiles)",
         "config": {
-            "workload": (f"plain selection, {spec.n_cycles} cycles x "
+            "workload": (f"{kind}, {spec.n_cycles} cycles x "
                          f"{spec.n_windows} windows, {gen['n_peaks']} peaks, "
                          f"{n_prec} precursors ({spec.n_targets} targets + "
                          f"decoys), {spec.frag_min}-{spec.frag_max} fragments, "
-                         f"+-{spec.half_window_max:g} cycles, "
+                         f"+-{spec.half_window_min:g}..{spec.half_window_max:g} cycles, "
                          f"{spec.mass_tolerance:g} ppm, top-{spec.candidate_count}"),
             "candidates": cand.len(),
             "select_kernel_ms": round(kern_ms.value, 4),
@@ -159,9 +232,20 @@ def main():
         },
     }
 
+    if fb is not None:
+        result["config"].update({
+            "forward_only_precursors_per_s": round(fwd_rate, 1),
+            "fwd_bwd_over_forward": round(value / fwd_rate, 3),
+            "diff_forward_device_ms": round(fb[1], 4),
+            "diff_backward_device_ms": round(fb[2], 4),
+            "hinge_margin": args.margin,
+            **{k: v for k, v in fb[3].items()},
+        })
+
     if not args.no_cpu_baseline:
         import oracle
-        nt = os.cpu_count() or 1
+        # the host share of one GPU is 16 cores, whatever os.cpu_count() says
+        nt = max(1, min(os.cpu_count() or 1, 16))
         t3 = time.time()
         odia = oracle.Dia(gen["dia"], nthreads=nt)
         olib = oracle.Lib(gen["lib"])
@@ -204,7 +288,10 @@ def main():
                        f"precursors, targets and decoys) of the same run; C "
                        f"restatement of the reference scalar path, dynamic "
                        f"chunks of 64 precursors over {nt} threads; "
-                       f"{cpu_s:.1f}s"),
+                       f"{cpu_s:.1f}s"
+                       + ("; forward only (the reference has no backward "
+                          "pass), so an upper bound on a CPU fwd+bwd rate"
+                          if fb is not None else "")),
         }
         result["config"]["cpu_sample_candidates_identical"] = identical
         result["config"]["gpu_over_cpu"] = round(value / cpu_rate, 1)

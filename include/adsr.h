@@ -189,6 +189,37 @@ adsr_status adsr_select_plan(const adsr_dia *d, const adsr_lib *l,
                              uint64_t *cycle_stop, uint32_t *n_selected,
                              uint64_t *selected, size_t stride);
 
+/* ---- differentiable selection (no counterpart in the reference) ---------
+ * Gradient of the rank-0 selection score with respect to a Gaussian sigma
+ * per quadrupole observation and the library fragment intensities used as
+ * score weights.  The forward runs PeakGroupSelection.search with
+ * sigma_per_window and the weights in place of fragment_intensity (also for
+ * the top-k fragment filter), and records each precursor's rank-0 apex; the
+ * backward holds the apex and fragment selection fixed, recomputes the XIC
+ * around the apex and differentiates the convolution and the weighted
+ * ln(1+x) sum.  Device pointers, single GPU; calls on one state are not
+ * concurrent.                                                             */
+typedef struct adsr_diff adsr_diff;
+adsr_status adsr_diff_create(const adsr_dia *d, const adsr_lib *l,
+                             const adsr_selection_params *p,
+                             adsr_diff **out);
+/* sigma: host f32[num_observations]; weights: device f32[num_fragments];
+ * s_star: device f32[num_precursors], rows in ascending precursor_idx
+ * (stable), the order of the candidate table; 0 without a candidate;
+ * has_candidate: optional device u8[num_precursors].                      */
+adsr_status adsr_diff_forward(adsr_diff *s, const float *sigma,
+                              const float *weights, float *s_star,
+                              uint8_t *has_candidate);
+/* grad_s: device f32[num_precursors] = dLoss/ds_star; outputs: device
+ * f32[num_fragments] (fragments that were not selected get 0) and host
+ * f64[num_observations].                                                  */
+adsr_status adsr_diff_backward(adsr_diff *s, const float *grad_s,
+                               float *grad_weights, double *grad_sigma);
+/* Device time of the last forward / backward call (ms). */
+adsr_status adsr_diff_timings(const adsr_diff *s, double *forward_ms,
+                              double *backward_ms);
+void adsr_diff_free(adsr_diff *s);
+
 /* ---- CandidateCollection (src/candidate/entry.rs:41-193) -------------- */
 size_t adsr_candidates_len(const adsr_candidates *c);
 /* to_arrays (entry.rs:124-173): nine columns in reference order. */

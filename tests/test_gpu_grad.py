@@ -1,0 +1,259 @@
+"""Differentiable selection (adsr_diff_* through alphadia_search_rs.autograd)
+against an FP64 restatement of the rank-0 score.
+
+Model, per precursor p with rank-0 apex a (fixed by the forward pass),
+selected fragments f, XIC X (exact f32 sums from the oracle) and the kernel
+of its first valid quadrupole window o(p):
+
+    s*_p = sum_f w_f ln(1 + Y_f),  Y_f = sum_j k_j(sigma_o) X_f[a - half + j]
+
+with k_j = exp(-x_j^2 / 2 sigma^2) / sum_m exp(-x_m^2 / 2 sigma^2) over
+x = -half .. half-1 (src/kernel/mod.rs:32-56; the normalisation constant
+cancels) and the symmetric tap use of src/convolution/scalar.rs:5-56
+(taps 0..half on both sides).  The loss is the hinge over target/decoy
+pairs, sum max(0, m - (s*_t - s*_d)).
+
+Checks (tolerance 1e-4 relative, elementwise, with an absolute floor of
+1e-4 * 1e-6 * max|reference| for entries that are zero or nearly so):
+  * the device forward equals the oracle's rank-0 score bit for bit, at
+    sigma = 2.0 equals plain search() bit for bit, and the FP64 model's s*
+    agrees with it to 1e-5 (pins apex, XIC, window and fragment mapping);
+  * the FP64 analytic gradient agrees with FP64 central finite differences
+    (h = 1e-4) of the FP64 model, for every sigma and a sample of weights;
+  * the device gradients agree with the FP64 gradient.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import synthetic
+
+pytestmark = pytest.mark.gpu
+
+SPEC = synthetic.config(3, n_cycles=1200, n_windows=20, peaks_per_spectrum=500,
+                        n_targets=500)
+H = 1e-4
+RTOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    import alphadia_search_rs as ads
+    import alphadia_search_rs._native as N
+    if not N.device_available():
+        pytest.fail("gfx950 device not available for a gpu-marked test")
+    gen = synthetic.generate(SPEC)
+    lib = gen["lib"]
+    assert np.all(np.diff(lib[0].astype(np.int64)) > 0)  # rows = input order
+    rng = np.random.default_rng(7)
+    W = SPEC.n_windows
+    sigma = np.linspace(1.6, 2.8, W).astype(np.float32)
+    weights = (lib[10] * rng.uniform(0.5, 1.5, lib[10].shape)).astype(np.float32)
+    return dict(torch=torch, ads=ads, gen=gen, sigma=sigma, weights=weights)
+
+
+def _params(gen, **extra):
+    from conftest import params_for
+    p = params_for(gen["spec"], **extra)
+    p.update({"rt_tolerance_per_precursor": gen["rt_tolerance_per_precursor"]})
+    return p
+
+
+def _oracle_state(gen, weights, sigma):
+    lib = list(gen["lib"])
+    lib[10] = weights
+    od, ol = oracle.Dia(gen["dia"]), oracle.Lib(tuple(lib))
+    cfg = gen["spec"].selection_config()
+    prm = oracle.params(**cfg, rt_tolerance_per_precursor=gen["rt_tolerance_per_precursor"],
+                        sigma_per_window=sigma)
+    return od, ol, prm
+
+
+def _rank0(table, n):
+    pid, rank, score, cc = table[0], table[1], table[2], table[6]
+    s = np.zeros(n, np.float32)
+    c = np.full(n, -1, np.int64)
+    m = rank == 0
+    s[pid[m].astype(np.int64)] = score[m]
+    c[pid[m].astype(np.int64)] = cc[m].astype(np.int64)
+    return s, c
+
+
+class F64Model:
+    """The FP64 restatement described in the module docstring."""
+
+    def __init__(self, gen, weights, sigma, apex):
+        spec = gen["spec"]
+        od, ol, prm = _oracle_state(gen, weights, sigma)
+        lib = gen["lib"]
+        self.klen = int(np.ceil(spec.kernel_size / 2.0) * 2)
+        self.half = self.klen // 2
+        self.rows = []  # (row, window, fragment indices, X window f64)
+        fs, fe = lib[6].astype(np.int64), lib[7].astype(np.int64)
+        for r in np.flatnonzero(apex >= 0):
+            one = oracle.select_one(od, ol, prm, int(r), max_cells=64 * 4096)
+            assert one is not None
+            c0 = one["cycle_start"]
+            a = int(apex[r]) - c0
+            assert self.half <= a < one["xic"].shape[1] - self.half
+            sel = oracle.filter_sort_fragments(
+                lib[9][fs[r]:fe[r]], weights[fs[r]:fe[r]], lib[14][fs[r]:fe[r]],
+                lib[16][fs[r]:fe[r]], top_k=spec.top_k_fragments)
+            assert np.array_equal(lib[9][fs[r]:fe[r]][sel], one["fragment_mz"])
+            obs = od.valid_observations(float(lib[2][r]))
+            X = one["xic"][:, a - self.half:a + self.half + 1].astype(np.float64)
+            self.rows.append((int(r), int(obs[0]), fs[r] + sel, X))
+        self.n_prec = len(lib[0])
+        self.n_frag = len(lib[10])
+
+    def taps(self, s):
+        x = np.arange(-self.half, self.half, dtype=np.float64)
+        e = np.exp(-x * x / (2.0 * s * s))
+        k = e / e.sum()
+        dk = k * (x * x - (k * x * x).sum()) / (s ** 3)
+        return k[: self.half + 1], dk[: self.half + 1]
+
+    def _y(self, X, k):
+        h = self.half
+        # Y = X[a] k[h] + sum_d (X[a-d] + X[a+d]) k[h-d]; X column h is the apex
+        sym = np.concatenate([X[:, :h] + X[:, h + 1:][:, ::-1], X[:, h:h + 1]], axis=1)
+        return sym @ k
+
+    def s_star(self, sigma, w):
+        s = np.zeros(self.n_prec)
+        for r, o, fi, X in self.rows:
+            k, _ = self.taps(float(sigma[o]))
+            s[r] = (w[fi] * np.log1p(self._y(X, k))).sum()
+        return s
+
+    def grads(self, sigma, w, g):
+        gw = np.zeros(self.n_frag)
+        gs = np.zeros(len(sigma))
+        for r, o, fi, X in self.rows:
+            k, dk = self.taps(float(sigma[o]))
+            y = self._y(X, k)
+            gw[fi] = g[r] * np.log1p(y)
+            gs[o] += g[r] * (w[fi] * self._y(X, dk) / (1.0 + y)).sum()
+        return gw, gs
+
+
+def _hinge64(s, pairs, m):
+    return np.maximum(0.0, m - (s[pairs[:, 0]] - s[pairs[:, 1]])).sum()
+
+
+def _close(got, ref, what):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    floor = RTOL * 1e-6 * max(np.abs(ref).max(), 1e-30)
+    err = np.abs(got - ref)
+    bad = np.flatnonzero(err > RTOL * np.abs(ref) + floor)
+    rel = (err / np.maximum(np.abs(ref), floor)).max()
+    print(f"{what}: max rel err {rel:.3e} over {len(ref)} values")
+    assert bad.size == 0, (f"{what}: {bad.size} values off, first {bad[:5]}: "
+                           f"got {got[bad[:5]]} ref {ref[bad[:5]]}")
+
+
+def test_diff_forward_at_init_equals_search(env):
+    """sigma = 2.0 everywhere and the library intensities as weights: s* is
+    the rank-0 score of plain search() (fwhm 4.7096 gives sigma 2.0f)."""
+    torch, ads, gen = env["torch"], env["ads"], env["gen"]
+    from alphadia_search_rs.autograd import SelectionGradState
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    p = _params(gen)
+    table = ads.PeakGroupSelection(p).search(d, lib).to_arrays()
+    want, _ = _rank0(table, lib.num_precursors)
+    st = SelectionGradState(d, lib, p)
+    w = torch.tensor(gen["lib"][10], device="cuda")
+    s, has = st.forward(torch.full((d.num_observations,), 2.0), w)
+    s = s.cpu().numpy()
+    assert has.cpu().numpy().sum() > 0.5 * lib.num_precursors
+    np.testing.assert_array_equal(s.view(np.uint32), want.view(np.uint32))
+
+
+def test_diff_gradients_match_fp64(env):
+    torch, ads, gen = env["torch"], env["ads"], env["gen"]
+    from alphadia_search_rs.autograd import (SelectionGradState, hinge_loss,
+                                             rank_zero_scores)
+    sigma, weights = env["sigma"], env["weights"]
+    lib_arrays = list(gen["lib"])
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*lib_arrays)
+    st = SelectionGradState(d, lib, _params(gen))
+    P = lib.num_precursors
+
+    # forward: bitwise against the oracle with the same sigma and weights
+    od, ol, prm = _oracle_state(gen, weights, sigma)
+    want_s, apex = _rank0(oracle.select(od, ol, prm), P)
+    sig_t = torch.tensor(sigma, requires_grad=True)
+    w_t = torch.tensor(weights, device="cuda", requires_grad=True)
+    s_t = rank_zero_scores(st, sig_t, w_t)
+    s_gpu = s_t.detach().cpu().numpy()
+    np.testing.assert_array_equal(s_gpu.view(np.uint32), want_s.view(np.uint32))
+
+    model = F64Model(gen, weights, sigma, apex)
+    s64 = model.s_star(sigma.astype(np.float64), weights.astype(np.float64))
+    _close(s64, s_gpu, "FP64 model s* vs device s*")
+
+    # hinge margin away from every kink
+    pairs = gen["pairs"]
+    diff = s64[pairs[:, 0]] - s64[pairs[:, 1]]
+    grid = np.linspace(np.quantile(diff, 0.3), np.quantile(diff, 0.7), 201)
+    m = float(grid[np.argmax([np.abs(diff - x).min() for x in grid])])
+    assert np.abs(diff - m).min() > 1e-3
+    active = (m - diff) > 0
+    assert 0 < active.sum() < len(pairs)
+
+    loss = hinge_loss(s_t, torch.tensor(pairs[:, 0], device="cuda"),
+                      torch.tensor(pairs[:, 1], device="cuda"), m)
+    loss.backward()
+    gw_gpu = w_t.grad.cpu().numpy()
+    gs_gpu = sig_t.grad.numpy()
+
+    g = np.zeros(P)
+    g[pairs[active, 0]] = -1.0
+    g[pairs[active, 1]] = 1.0
+    sig64, w64 = sigma.astype(np.float64), weights.astype(np.float64)
+    gw_ref, gs_ref = model.grads(sig64, w64, g)
+
+    # FP64 finite differences of the FP64 model pin the analytic gradient
+    def L(sg, ww):
+        return _hinge64(model.s_star(sg, ww), pairs, m)
+
+    fd_s = np.zeros_like(sig64)
+    for o in range(len(sig64)):
+        e = np.zeros_like(sig64)
+        e[o] = H
+        fd_s[o] = (L(sig64 + e, w64) - L(sig64 - e, w64)) / (2 * H)
+    _close(gs_ref, fd_s, "FP64 d/dsigma analytic vs central differences")
+    rng = np.random.default_rng(3)
+    nz = np.flatnonzero(gw_ref)
+    for j in rng.choice(nz, size=min(60, nz.size), replace=False):
+        e = np.zeros_like(w64)
+        e[j] = H
+        fd = (L(sig64, w64 + e) - L(sig64, w64 - e)) / (2 * H)
+        assert abs(fd - gw_ref[j]) <= RTOL * abs(gw_ref[j]) + 1e-9, (j, fd, gw_ref[j])
+
+    # device gradients against FP64
+    _close(gs_gpu, gs_ref, "device d/dsigma vs FP64")
+    _close(gw_gpu, gw_ref, "device d/dweights vs FP64")
+    assert np.count_nonzero(gw_gpu[gw_ref == 0]) == 0
+
+    # deterministic: a second backward gives identical bits
+    gw2, gs2 = st.backward(torch.tensor(g, dtype=torch.float32, device="cuda"))
+    np.testing.assert_array_equal(gw2.cpu().numpy().view(np.uint32),
+                                  gw_gpu.view(np.uint32))
+    np.testing.assert_array_equal(gs2.numpy().astype(np.float32), gs_gpu)
+
+
+def test_diff_backward_before_forward_raises(env):
+    ads, gen, torch = env["ads"], env["gen"], env["torch"]
+    from alphadia_search_rs.autograd import SelectionGradState
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    st = SelectionGradState(d, lib, _params(gen))
+    with pytest.raises(RuntimeError):
+        st.backward(torch.zeros(lib.num_precursors, device="cuda"))
+    with pytest.raises(ValueError):
+        st.forward(torch.full((3,), 2.0), torch.zeros(lib.num_fragments, device="cuda"))

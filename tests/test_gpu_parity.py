@@ -171,7 +171,7 @@ def test_workload0_candidates_match_oracle(ads, run0):
     assert_tables_equal(got, want, "workload 0")
 
 
-@pytest.mark.parametrize("variant", ["decoys_uneven", "per_window_sigma", "ties"])
+@pytest.mark.parametrize("variant", ["decoys_uneven", "per_window_sigma", "ties", "skips"])
 def test_small_variants_match_oracle(ads, variant):
     if variant == "decoys_uneven":
         spec = synthetic.config(2, n_cycles=600, n_windows=12, peaks_per_spectrum=800,
@@ -186,6 +186,25 @@ def test_small_variants_match_oracle(ads, variant):
         sig = np.linspace(1.5, 3.0, 6).astype(np.float32)
         got = _search(ads, gen, sigma_per_window=sig)[3]
         want = _oracle(gen, sigma_per_window=sig)
+    elif variant == "skips":
+        # rows the kernels skip: empty RT windows (c1 <= c0), no isolation
+        # window containing the precursor, no fragment with intensity > 0
+        spec = synthetic.config(0, n_cycles=400, n_windows=4, peaks_per_spectrum=800,
+                                n_targets=700, half_window_min=30, half_window_max=30)
+        gen = synthetic.generate(spec)
+        lib = [a.copy() for a in gen["lib"]]
+        lib[3][::7] = -1.0e4
+        lib[4][::7] = -1.0e4
+        lib[1][5::11] = 50.0
+        lib[2][5::11] = 50.0
+        fs, fe = lib[6].astype(np.int64), lib[7].astype(np.int64)
+        for r in range(3, len(fs), 13):
+            lib[10][fs[r]:fe[r]] = 0.0
+        gen = dict(gen, lib=tuple(lib))
+        got = _search(ads, gen)[3]
+        want = _oracle(gen)
+        skipped = set(range(0, 700, 7)) | set(range(5, 700, 11)) | set(range(3, 700, 13))
+        assert not skipped & set(got[0].astype(np.int64).tolist())
     else:
         # coarse intensities and many duplicate (bin, cycle) entries
         spec = synthetic.config(0, n_cycles=300, n_windows=4, peaks_per_spectrum=3000,

@@ -251,3 +251,29 @@ def test_search_fails_loudly_without_gpu(run0):
         ads.set_simd_backend("scalar")
     with pytest.raises(N.AdsrError):
         ads.set_simd_backend("hip")
+
+
+def test_diff_rejects_shared_fragment_ranges_and_needs_gpu(run0):
+    import alphadia_search_rs as ads
+    import alphadia_search_rs._native as N
+    d = ads.DIAData.from_arrays(*run0["dia"])
+    lib = list(run0["lib"])
+    # precursor 1 reuses precursor 0's fragments: per-fragment gradients
+    # would need atomics, so the differentiable path refuses the library
+    start, stop = lib[6].copy(), lib[7].copy()
+    start[1], stop[1] = start[0], stop[0]
+    lib[6], lib[7] = start, stop
+    shared = ads.SpecLibFlat.from_arrays(*lib)
+    prm = ads.SelectionParameters()._c()
+    out = C.c_void_p()
+    st = N.lib.adsr_diff_create(d._h, shared._h, C.byref(prm), C.byref(out))
+    assert st == N.ADSR_ERR_UNSUPPORTED and not out.value
+    assert "disjoint" in N.lib.adsr_last_error().decode()
+    if N.device_available():
+        return
+    ok = ads.SpecLibFlat.from_arrays(*run0["lib"])
+    st = N.lib.adsr_diff_create(d._h, ok._h, C.byref(prm), C.byref(out))
+    assert st != N.ADSR_OK and not out.value
+    from alphadia_search_rs.autograd import SelectionGradState
+    with pytest.raises(RuntimeError, match="gfx950"):
+        SelectionGradState(d, ok, ads.SelectionParameters())
