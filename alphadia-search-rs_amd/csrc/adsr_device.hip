@@ -1881,7 +1881,14 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
                        hipGetErrorString(hipGetLastError()));
     return ADSR_OK;
   };
-  LaunchShape shape;
+  // untimed searches use the shape tuned for this (rows, window, fragments)
+  // key when there is one, else 256 threads x 2 precursors per grab
+  LaunchShape shape{4, 2};
+  {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    auto it = g_tuned.find(std::make_tuple((size_t)n, sp.lcap, w.ks));
+    if (it != g_tuned.end()) shape = it->second;
+  }
   const int want_tpb = tpb ? *tpb : 0, want_ppw = ppw ? *ppw : 0;
   if (want_tpb == 64 || want_tpb == 128 || want_tpb == 256) {
     shape.waves = want_tpb / 64;

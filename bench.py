@@ -84,6 +84,45 @@ def fwd_bwd(args, gen, dia, lib, params):
     return step_ms, fwd_ms / args.steps, bwd_ms / args.steps, stats
 
 
+def measured_traffic(workload):
+    """HBM bytes per select_kernel launch from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy); None
+    when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", f"select_workload{workload}_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if (len(parts) == 4 and parts[0] == "traffic_bytes_per_launch"
+                        and parts[1] == f"workload={workload}"
+                        and parts[2] == "kernel=select_kernel"):
+                    return int(parts[3])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(mib):
+    """A plain copy of known size, for scaling HBM byte counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    assert bool((dst[:: max(1, n // 1000)] == src[:: max(1, n // 1000)]).all())
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
+                      "ms_per_copy": round(ms, 4),
+                      "GB_per_s": round(2 * n * 4 / (ms / 1e3) / 1e9, 1)}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,7 +137,14 @@ def main():
     ap.add_argument("--precursors-per-wave", type=int, default=0)
     ap.add_argument("--margin", type=float, default=1.0,
                     help="hinge margin of the workload-3 loss")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB "
+                         "(5 times) and print its size and time: the "
+                         "calibration for FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.workload not in (1, 2, 3):
         print("bench.py: --workload is 1, 2 or 3", file=sys.stderr)
         sys.exit(2)
@@ -228,7 +274,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.workload),
         },
     }
 

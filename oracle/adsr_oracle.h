@@ -11,6 +11,38 @@
  * operation and no FMA contraction (Rust semantics; build with
  * -ffp-contract=off).  ln/exp are glibc logf/expf, the libm the Rust binary
  * calls on x86-64 Linux.
+ *
+ * How each part is pinned (tests/test_oracle.py unless stated):
+ *   grid, nearest bin, bin range      reference KATs (length 2,995,974 at
+ *                                     100..2000, clamp, nearest/range cases)
+ *                                     + brute force over random queries
+ *   RT index, cycle limits            reference KATs (src/rt_index/tests.rs)
+ *   CSR build, fill_xic_slice,        reference KATs + brute force on tiny
+ *   dense XIC                         runs (snap every peak, sum), including
+ *                                     the duplicate-entry case below
+ *   Gaussian kernel                   reference KATs (length, sum 1 +- 1e-6,
+ *                                     centre max) + closed form + symmetry
+ *   convolution                       direct 21-tap definition on random
+ *                                     rows, constant-row closed form,
+ *                                     L < kernel width -> zeros
+ *   log-dot score                     reference KAT + closed form
+ *   local maxima                      reference KATs + brute force
+ *   fragment filter/top-k/m/z sort    reference KATs (multiset at ties)
+ *   candidates                        reference KATs (to_arrays order)
+ *   logf                              identical to glibc logf by definition
+ *                                     (it calls it)
+ * Parity unpinned (no fixture or closed form fixes the bits; the oracle
+ * follows the reference's published algorithms):
+ *   - kernel-weight bits: ndarray 0.16.1 sum() order, restated as an
+ *     8-accumulator unrolled fold (orc_ndarray_sum)
+ *   - which of several equal cycle entries Rust 1.88 binary_search returns
+ *     (restated as the last one; orc_rust_bsearch_u16)
+ *   - select_nth_unstable_by among tied intensities at the top-k cut:
+ *     restated exactly for up to 16 fragments (insertion sort, the
+ *     k == 0 and k == len-1 special cases); beyond 16 the introselect is
+ *     replaced by a stable sort, which picks the same set whenever the
+ *     intensities at the cut differ
+ *   - rayon collect order (documented as order preserving)
  */
 #ifndef ADSR_ORACLE_H
 #define ADSR_ORACLE_H
