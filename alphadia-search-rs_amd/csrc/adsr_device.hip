@@ -419,45 +419,47 @@ struct SelArgs {
 
 // Per-wave LDS.  XIC rows have kXPad zero floats on both sides so that a
 // block of 4 outputs reads its 21-tap neighbourhood with aligned 16-byte
-// loads; M holds one hit bit per cycle (64-bit words, one zero word before
-// and after each row); BD one bit per active 4-cycle output block.
+// loads.  BD holds one bit per 4-cycle output block that can be non-zero.
+// hits is the extraction's hit list (entry or intensity, key), items the
+// (fragment, block) work list of the convolution, own the duplicate
+// detector of the hit accumulation (one byte per fragment and cycle).
 constexpr int kXPad = 16;
-constexpr int kItemCap = 1024;
-constexpr int kTilesInFlight = 8;
+constexpr int kItemCap = 512;   // u16 (fragment, block) items
+constexpr int kHitCap = 256;    // (u32 entry or intensity, u32 key) hits
+constexpr int kTileHits = 4 * 64;  // hits one extraction tile can add
+constexpr int kGroupTiles = 8;     // extraction tiles in flight
 
 struct LdsShape {
   uint32_t lp;     // XIC row stride (floats)
-  uint32_t wm;     // M row stride (u64 words)
   uint32_t wb;     // BD row stride (u64 words)
 };
 
 __host__ __device__ inline LdsShape lds_shape(uint32_t lcap) {
   LdsShape s;
   s.lp = lcap + 2 * kXPad;
-  s.wm = lcap / 64 + 2;
-  s.wb = (lcap / 64 + 3) / 4 * 4 / 4 + 1;
+  s.wb = (lcap / 4 + 63) / 64;
   return s;
 }
 
 __host__ __device__ inline uint32_t lds_bytes(uint32_t lcap, uint32_t gcap) {
   const LdsShape s = lds_shape(lcap);
-  uint32_t b = lcap * 4                 // R
+  uint32_t b = (lcap + 12) * 4          // R with 4 + 8 pad floats
                + gcap * s.lp * 4        // X
-               + gcap * s.wm * 8        // M
                + gcap * s.wb * 8        // BD
-               + kItemCap * 2           // items (u16)
-               + lcap                   // own
+               + kHitCap * 8            // hits
+               + kItemCap * 2           // items
+               + 32 * 4                 // kernel taps
                + kMaxSel * 4            // w
-               + 32 * 4;                // kernel taps
+               + gcap * lcap;           // own
   return (b + 15) / 16 * 16;
 }
 
 struct WaveLds {
   float *R;
   float *X;
-  uint64_t *M;
   uint64_t *BD;
   uint16_t *items;
+  uint2 *hits;
   uint8_t *own;
   float *w;
   float *kern;
@@ -467,14 +469,14 @@ __device__ __forceinline__ WaveLds carve(char *base, const SelArgs &a) {
   const LdsShape sh = lds_shape(a.lcap);
   WaveLds l;
   char *p = base;
-  l.R = (float *)p;
-  p += a.lcap * 4;
+  l.R = (float *)p + 4;
+  p += (a.lcap + 12) * 4;
   l.X = (float *)p;
   p += a.gcap * sh.lp * 4;
-  l.M = (uint64_t *)p;
-  p += a.gcap * sh.wm * 8;
   l.BD = (uint64_t *)p;
   p += a.gcap * sh.wb * 8;
+  l.hits = (uint2 *)p;
+  p += kHitCap * 8;
   l.items = (uint16_t *)p;
   p += kItemCap * 2;
   l.kern = (float *)p;
@@ -513,30 +515,11 @@ __device__ __forceinline__ bool skipped_duplicate(const KDia &dia, uint32_t o,
   return e + 1 < ss[b + 1] && dia.cycles[base + e + 1] == c0;
 }
 
-// One dilation step of a 192-bit window (prev, cur, next) by +-s.
-__device__ __forceinline__ void dilate_step(uint64_t &p, uint64_t &c,
-                                            uint64_t &n, uint32_t s) {
-  const uint64_t p2 = p | (p << s) | (p >> s) | (c << (64 - s));
-  const uint64_t c2 = c | (c << s) | (c >> s) | (p >> (64 - s)) | (n << (64 - s));
-  const uint64_t n2 = n | (n << s) | (n >> s) | (c >> (64 - s));
-  p = p2;
-  c = c2;
-  n = n2;
-}
-
-// Bits 0,4,...,60 of x packed into bits 0..15.
-__device__ __forceinline__ uint32_t pack_nibbles(uint64_t x) {
-  x &= 0x1111111111111111ull;
-  x = (x | (x >> 3)) & 0x0303030303030303ull;
-  x = (x | (x >> 6)) & 0x000f000f000f000full;
-  x = (x | (x >> 12)) & 0x000000ff000000ffull;
-  x = (x | (x >> 24)) & 0xffffull;
-  return (uint32_t)x;
-}
-
 // Four consecutive outputs i0..i0+3 of src/convolution/scalar.rs:14-36
 // (s = 0 + x[i]*k[h], then s += (x[i-d] + x[i+d]) * k[h-d] for d = h..1),
 // from the zero-padded row: xs points at X[i0 - 12] (16-byte aligned).
+// The leading 0 + is dropped: it only turns a -0 product into +0, and an
+// output that ends up +-0 scores ln(+-0 + 1) = 0 either way.
 __device__ __forceinline__ void conv4_h10(const float *xs, const float (&k)[11],
                                           float (&y)[4]) {
   float x[28];
@@ -551,8 +534,7 @@ __device__ __forceinline__ void conv4_h10(const float *xs, const float (&k)[11],
 #pragma unroll
   for (int o = 0; o < 4; ++o) {
     const int c = 12 + o;
-    float s = 0.0f;
-    s = s + x[c] * k[10];
+    float s = x[c] * k[10];
 #pragma unroll
     for (int j = 0; j < 10; ++j) {
       const int d = 10 - j;
@@ -569,14 +551,45 @@ __device__ __forceinline__ void conv4_any(const float *row, uint32_t i0,
 #pragma unroll
   for (int o = 0; o < 4; ++o) {
     const float *xc = row + i0 + o;
-    float s = 0.0f;
-    s = s + xc[0] * k[half];
+    float s = xc[0] * k[half];
     for (uint32_t j = 0; j < half; ++j) {
       const uint32_t d = half - j;
       s = s + (xc[-(int)d] + xc[d]) * k[j];
     }
     y[o] = s;
   }
+}
+
+// One step of a wave-wide arg-max over (value, position) pairs through a
+// DPP lane permutation; lanes without a source keep their own pair.
+// Order: larger value first, then the lower position; ~0u marks no pair.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void argmax_dpp(float &bv, uint32_t &bi) {
+  const int vb = __builtin_bit_cast(int, bv);
+  const float ov = __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(vb, vb, CTRL, ROWMASK, 0xf, false));
+  const uint32_t oi = (uint32_t)__builtin_amdgcn_update_dpp(
+      (int)bi, (int)bi, CTRL, ROWMASK, 0xf, false);
+  const bool take = oi != 0xffffffffu &&
+                    (bi == 0xffffffffu || ov > bv || (ov == bv && oi < bi));
+  if (take) {
+    bv = ov;
+    bi = oi;
+  }
+}
+
+// Wave-wide arg-max: row_shr 1, 2, 4, 8 reduce each row of 16 lanes into
+// its lane 15; row_bcast 15 and 31 carry rows 0..2 into lane 63.  The
+// result is returned in every lane.
+__device__ __forceinline__ void wave_argmax(float &bv, uint32_t &bi) {
+  argmax_dpp<0x111, 0xf>(bv, bi);
+  argmax_dpp<0x112, 0xf>(bv, bi);
+  argmax_dpp<0x114, 0xf>(bv, bi);
+  argmax_dpp<0x118, 0xf>(bv, bi);
+  argmax_dpp<0x142, 0xa>(bv, bi);
+  argmax_dpp<0x143, 0xc>(bv, bi);
+  bv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, bv), 63));
+  bi = (uint32_t)__builtin_amdgcn_readlane((int)bi, 63);
 }
 
 template <int WAVES, int HALF>
@@ -593,14 +606,74 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
   const uint32_t cc = a.prm.cand_count;
   const uint32_t ks = a.plan.ks;
 
-  for (;;) {
-    uint32_t row0 = 0;
-    if (lane == 0) row0 = atomicAdd(a.queue, a.chunk);
-    row0 = uni(__shfl(row0, 0, kWave));
-    if (row0 >= a.n_rows) break;
-    const uint32_t row_end = min(row0 + a.chunk, a.n_rows);
-    for (uint32_t row = row0; row < row_end; ++row) {
-      const uint4 rec = a.plan.rec[row];
+  // Row schedule: the queue hands out `chunk` rows at a time.  The next
+  // row's plan data is loaded before the current row is processed (stage
+  // 1: record, weights, windows, first-window CSR ranges) and its dependent
+  // values half-way through (stage 2: kernel taps, window entry bases), so
+  // that their latency overlaps the current row's work.
+  // The queue is grabbed one chunk ahead: `ahead` is the start of the
+  // chunk after the current one, requested when the current one began.
+  uint32_t nxt = 0, cend = 0, ahead = 0;
+  auto request = [&]() {
+    uint32_t r0 = 0;
+    if (lane == 0) r0 = atomicAdd(a.queue, a.chunk);
+    return r0;
+  };
+  auto grab = [&]() {
+    nxt = uni(__shfl(ahead, 0, kWave));
+    cend = min(nxt + a.chunk, a.n_rows);
+    ahead = nxt < a.n_rows ? request() : 0xffffffffu;
+  };
+  ahead = request();
+  // Outputs of the last finished row are stored after the next row's first
+  // loads are issued, so that those loads do not wait for the stores.
+  uint32_t out_row = 0xffffffffu, out_n = 0, out_c = 0;
+  float out_s = 0.0f;
+  auto flush_out = [&]() {
+    if (out_row != 0xffffffffu) {
+      if ((uint32_t)lane < min(out_n, (uint32_t)kWave)) {
+        a.cand_score[(size_t)out_row * cc + lane] = out_s;
+        a.cand_center[(size_t)out_row * cc + lane] = out_c;
+      }
+      a.cand_count[out_row] = out_n;
+      out_row = 0xffffffffu;
+    }
+  };
+  uint4 p_rec = make_uint4(0u, 0u, 0u, 0u);
+  float p_sw = 0.0f, p_tap = 0.0f;
+  uint32_t p_obs = 0, p_b0 = 0;
+  uint2 p_rng = make_uint2(0u, 0u);
+  uint64_t p_eb = 0;
+  auto stage1 = [&](uint32_t r) {
+    p_rec = a.plan.rec[r];
+    p_sw = lane < (int)ks ? a.plan.sel_w[(size_t)r * ks + lane] : 0.0f;
+    p_obs = lane < kObsMax ? a.plan.obs[(size_t)r * kObsMax + lane] : 0u;
+    p_rng = lane < (int)ks ? a.plan.rng[(size_t)r * kObsMax * ks + lane]
+                           : make_uint2(0u, 0u);
+    p_b0 = lane < (int)ks ? a.plan.b0[(size_t)r * kObsMax * ks + lane] : 0u;
+  };
+  auto stage2 = [&]() {
+    const uint32_t kidx = uni(p_rec.w), pobs = uni(p_rec.z >> 16);
+    p_tap = lane < (int)a.prm.klen ? a.prm.kernels[(size_t)kidx * a.prm.klen + lane]
+                                   : 0.0f;
+    p_eb = lane < (int)min(pobs, (uint32_t)kObsMax) ? a.dia.entry_base[p_obs] : 0ull;
+  };
+  grab();
+  if (nxt < a.n_rows) {
+    stage1(nxt);
+    stage2();
+  }
+  while (nxt < a.n_rows) {
+    {
+      const uint32_t row = nxt;
+      const uint4 rec = p_rec;
+      const float sw = p_sw, tap = p_tap;
+      const uint32_t obs_l = p_obs, b00 = p_b0;
+      const uint2 rng0 = p_rng;
+      const uint64_t eb_l = p_eb;
+      if (++nxt >= cend) grab();
+      const bool more = nxt < a.n_rows;
+      if (more) stage1(nxt);
       const uint32_t c0 = uni(rec.x), c1 = uni(rec.y);
       const uint32_t nsel = uni(rec.z & 0xffffu), nobs = uni(rec.z >> 16);
       // Stores next to a `continue` or the loop's back edge are made by all
@@ -609,160 +682,290 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
       // lanes 1..63 no longer follow lane 0 through the loop.
       if (c1 <= c0) {  // mod.rs:146-148 -> None
         a.cand_count[row] = 0;
+        if (more) stage2();
         continue;
       }
       const uint32_t L = c1 - c0;
       if (L > a.lcap) {  // excluded by the host bound; flagged if it happens
         a.cand_count[row] = 0;
         atomicOr(a.status, (uint32_t)kStWindowTooLong);
+        if (more) stage2();
         continue;
       }
       const uint32_t nw = (L + 63) / 64;  // 64-cycle words per row
       for (uint32_t i = 4 * lane; i < L; i += 4 * kWave)
         *reinterpret_cast<float4 *>(l.R + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-      const float *kg = a.prm.kernels + (size_t)uni(rec.w) * a.prm.klen;
       float k10[11];
       if (HALF == 10) {
 #pragma unroll
-        for (int j = 0; j < 11; ++j) k10[j] = kg[j];
+        for (int j = 0; j < 11; ++j)
+          k10[j] = __builtin_bit_cast(
+              float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tap), j));
       } else {
-        if (lane <= (int)half) l.kern[lane] = kg[lane];
+        if (lane <= (int)half) l.kern[lane] = tap;
       }
-      if (lane < (int)nsel) l.w[lane] = a.plan.sel_w[(size_t)row * ks + lane];
+      if (lane < (int)nsel) l.w[lane] = sw;
       __builtin_amdgcn_wave_barrier();
+      // window j of this row: obs index and entry base
+      auto win_obs = [&](uint32_t j) { return (uint32_t)__builtin_amdgcn_readlane((int)obs_l, (int)j); };
+      auto win_base = [&](uint32_t j) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)eb_l, (int)j);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(eb_l >> 32), (int)j);
+        return ((uint64_t)hi << 32) | lo;
+      };
 
+      // ---- extraction: src/dense_xic_observation/mod.rs:42-55 ------------
+      // Windows in order; within a window the fragments' CSR ranges are cut
+      // into 4-entry groups (one 8-byte load of u16 cycles per lane), 64
+      // groups per tile across fragment boundaries, kGroupTiles tiles in
+      // flight.  In-window hits go to an LDS list in entry order (window,
+      // fragment, bin, entry), which is the reference's summation order.
+      // Hit key: t (12 bits) | fragment (6) << 12 | window (2) << 18.
+      uint32_t nh = 0;
+      uint32_t gA = 0, gG = 0;  // fragment group being accumulated
+      // Adds hits [0, nh) of fragments [gA, gA + gG) into X; the list holds
+      // intensities (loaded) or window-relative entries of window `base`.
+      auto accumulate = [&](bool loaded, uint64_t base) {
+        for (uint32_t c = 0; c < nh; c += kWave) {
+          const uint32_t idx = c + lane;
+          uint2 h = make_uint2(0u, 0u);
+          if (idx < nh) h = l.hits[idx];
+          const uint32_t f = (h.y >> 12) & 63u, t = h.y & 0xfffu;
+          const bool v = idx < nh && f >= gA && f < gA + gG;
+          const uint32_t fl = v ? f - gA : 0u;
+          float inten = 0.0f;
+          if (v) inten = loaded ? __builtin_bit_cast(float, h.x) : a.dia.intens[base + h.x];
+          uint8_t *ow = l.own + fl * a.lcap + t;
+          if (v) *ow = (uint8_t)lane;
+          __builtin_amdgcn_wave_barrier();
+          const bool dup = v && *ow != (uint8_t)lane;
+          float *xr = l.X + fl * sh.lp + kXPad;
+          if (ballot64(dup) == 0) {
+            if (v) xr[t] = xr[t] + inten;
+          } else {
+            uint64_t it2 = ballot64(v);
+            while (it2) {
+              const int j2 = __builtin_ctzll(it2);
+              it2 &= it2 - 1;
+              if (lane == j2) xr[t] = xr[t] + inten;
+              __builtin_amdgcn_wave_barrier();
+            }
+          }
+          // blocks of the outputs this hit reaches, within the computed
+          // range [half, L - half) of the convolution
+          if (v) {
+            const int ilo = max((int)t - (int)half, (int)half);
+            const int ihi = min((int)(t + half), (int)L - (int)half - 1);
+            if (ilo <= ihi) {
+              const uint32_t blo = (uint32_t)ilo >> 2, bhi = (uint32_t)ihi >> 2;
+              uint64_t *br = l.BD + fl * sh.wb;
+              const uint32_t wlo = blo >> 6, whi = bhi >> 6;
+              const uint64_t ones = ~0ull;
+              if (wlo == whi) {
+                const uint64_t m = (ones >> (63 - (bhi - blo))) << (blo & 63);
+                atomicOr((unsigned long long *)&br[wlo], m);
+              } else {
+                atomicOr((unsigned long long *)&br[wlo], ones << (blo & 63));
+                atomicOr((unsigned long long *)&br[whi], ones >> (63 - (bhi & 63)));
+              }
+            }
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+      };
+      // Lists the hits of fragments [fa, fb) of every window.  Deferred: the
+      // list must hold the whole row (false when it would overflow).
+      // Direct: the list is accumulated into X whenever it fills and at the
+      // end of each window.
+      auto extract = [&](uint32_t fa, uint32_t fb, bool deferred) -> bool {
+        const uint32_t FN = fb - fa;
+        for (uint32_t j = 0; j < nobs; ++j) {
+          const uint32_t o = win_obs(j);
+          const uint64_t base = win_base(j);
+          const size_t ridx = ((size_t)row * kObsMax + j) * ks + fa;
+          uint32_t rx = 0, ry = 0, rb0 = 0;
+          if (lane < (int)FN) {
+            if (j == 0 && fa == 0) {
+              rx = rng0.x;
+              ry = rng0.y;
+              rb0 = b00;
+            } else {
+              const uint2 r = a.plan.rng[ridx + lane];
+              rx = r.x;
+              ry = r.y;
+              rb0 = a.plan.b0[ridx + lane];
+            }
+          }
+          // Groups of 4 entries are counted from the window's aligned start
+          // gw = base / 4 (group g covers entries 4(gw + g) .. +3), so one
+          // 8-byte load at cw + 4g reads a group.  Fragment fa + lane spans
+          // groups gsr .. gsr + ng - 1; its first and last groups are
+          // partial (entries ko .. 3 and 0 .. kl).
+          const uint32_t bo = (uint32_t)base & 3u;
+          const uint16_t *cw = a.dia.cycles + (base - bo);
+          const uint32_t gsr = (bo + rx) >> 2;
+          const uint32_t ng = ry > rx ? ((bo + ry - 1) >> 2) - gsr + 1 : 0u;
+          // packed: first-group offset, last-group end, group count
+          const uint32_t meta = ((bo + rx) & 3u) | (((bo + ry - 1) & 3u) << 2) | (ng << 4);
+          const uint32_t gincl = wave_incl_scan(ng);
+          const uint32_t gp = gincl - ng;  // first virtual group
+          const uint32_t NG = rdlane(gincl, FN - 1);
+          for (uint32_t vb = 0; vb < NG; vb += kWave * kGroupTiles) {
+            uint2 raw[kGroupTiles];
+            uint32_t tg[kGroupTiles], tfr[kGroupTiles], tkl[kGroupTiles],
+                tkh[kGroupTiles], tb0[kGroupTiles];
+#pragma unroll
+            for (int u = 0; u < kGroupTiles; ++u) {
+              const uint32_t v0 = vb + (uint32_t)(u * kWave);
+              const uint32_t v = v0 + lane;
+              raw[u] = make_uint2(0xffffffffu, 0xffffffffu);
+              tg[u] = 0;
+              tfr[u] = 0;
+              tkl[u] = 4;  // no entry of this group is in range
+              tkh[u] = 0;
+              tb0[u] = 0;
+              if (v0 < NG) {
+                const uint32_t vmax = min(v0 + (uint32_t)kWave - 1, NG - 1);
+                const uint64_t lo_m = ballot64(lane < (int)FN && ng > 0 && gp <= v0);
+                const uint64_t hi_m = ballot64(lane < (int)FN && ng > 0 && gp <= vmax);
+                const uint32_t flo = 63u - (uint32_t)__builtin_clzll(lo_m);
+                const uint32_t fhi = 63u - (uint32_t)__builtin_clzll(hi_m);
+                // fragment of this lane's group: the last one starting at
+                // or before it (empty fragments share the next one's start)
+                uint32_t f = flo, fgp = rdlane(gp, flo), fgs = rdlane(gsr, flo);
+                uint32_t fmeta = rdlane(meta, flo), fb0 = rdlane(rb0, flo);
+                for (uint32_t ff = flo + 1; ff <= fhi; ++ff) {
+                  const uint32_t pp = rdlane(gp, ff);
+                  if (pp <= v) {
+                    f = ff;
+                    fgp = pp;
+                    fgs = rdlane(gsr, ff);
+                    fmeta = rdlane(meta, ff);
+                    fb0 = rdlane(rb0, ff);
+                  }
+                }
+                if (v < NG) {
+                  const uint32_t q = v - fgp;  // group within the fragment
+                  tg[u] = fgs + q;
+                  raw[u] = *reinterpret_cast<const uint2 *>(cw + 4 * (size_t)tg[u]);
+                  tfr[u] = fa + f;
+                  tkl[u] = (q == 0) ? (fmeta & 3u) : 0u;
+                  tkh[u] = (q + 1 == (fmeta >> 4)) ? ((fmeta >> 2) & 3u) : 3u;
+                  tb0[u] = fb0;
+                }
+              }
+            }
+            flush_out();
+#pragma unroll
+            for (int u = 0; u < kGroupTiles; ++u) {
+              if (vb + (uint32_t)(u * kWave) >= NG) break;
+              const uint32_t cyc[4] = {raw[u].x & 0xffffu, raw[u].x >> 16,
+                                       raw[u].y & 0xffffu, raw[u].y >> 16};
+              uint32_t hm = 0;
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                const bool in = (uint32_t)k >= tkl[u] && (uint32_t)k <= tkh[u] &&
+                                cyc[k] >= c0 && cyc[k] < c1;
+                hm |= (in ? 1u : 0u) << k;
+              }
+              // the reference's binary_search(c0) starts at the last entry
+              // equal to c0 in a bin: earlier ones are skipped
+              bool at_c0 = false;
+#pragma unroll
+              for (int k = 0; k < 4; ++k)
+                at_c0 = at_c0 || (((hm >> k) & 1u) && cyc[k] == c0);
+              if (ballot64(at_c0) != 0) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                  if (((hm >> k) & 1u) && cyc[k] == c0) {
+                    const uint32_t e = 4 * tg[u] + k - bo;
+                    if (skipped_duplicate(a.dia, o, base, tb0[u], e, c0))
+                      hm &= ~(1u << k);
+                  }
+                }
+              }
+              // list slots in (lane, k) order = entry order
+              uint32_t pre = 0, tot = 0;
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                const uint64_t bk = ballot64((hm >> k) & 1u);
+                pre += __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(bk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bk, 0u));
+                tot += (uint32_t)__builtin_popcountll(bk);
+              }
+              if (deferred && nh + tot > (uint32_t)kHitCap) return false;
+              uint32_t pos = nh + pre;
+#pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                if ((hm >> k) & 1u) {
+                  const uint32_t e = 4 * tg[u] + k - bo;
+                  l.hits[pos] = make_uint2(e, (j << 18) | (tfr[u] << 12) | (cyc[k] - c0));
+                  ++pos;
+                }
+              }
+              nh += tot;
+              __builtin_amdgcn_wave_barrier();
+              if (!deferred && nh > (uint32_t)(kHitCap - kTileHits)) {
+                accumulate(false, base);
+                nh = 0;
+              }
+            }
+          }
+          if (!deferred && nh) {
+            accumulate(false, base);
+            nh = 0;
+          }
+          __builtin_amdgcn_wave_barrier();
+        }
+        return true;
+      };
+      auto zero_group = [&](uint32_t G) {
+        float4 *xz = reinterpret_cast<float4 *>(l.X);
+        const uint32_t n4 = G * sh.lp / 4;
+        for (uint32_t i = lane; i < n4; i += kWave)
+          xz[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t i = lane; i < G * sh.wb; i += kWave) l.BD[i] = 0ull;
+        __builtin_amdgcn_wave_barrier();
+      };
+      // the whole row's hits in one pass, their intensities in one batch
+      bool listed = nobs > 0 && nsel > 0 && extract(0, nsel, true);
+      if (listed) {
+        uint64_t wb[kObsMax];
+#pragma unroll
+        for (int j = 0; j < kObsMax; ++j) wb[j] = win_base((uint32_t)j);
+        for (uint32_t c = 0; c < nh; c += kWave * 4) {
+          float iv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t idx = c + (uint32_t)(u * kWave) + lane;
+            iv[u] = 0.0f;
+            if (idx < nh) {
+              const uint2 h = l.hits[idx];
+              const uint32_t j = h.y >> 18;
+              const uint64_t base = j == 0 ? wb[0] : j == 1 ? wb[1] : j == 2 ? wb[2] : wb[3];
+              iv[u] = a.dia.intens[base + h.x];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t idx = c + (uint32_t)(u * kWave) + lane;
+            if (idx < nh) l.hits[idx].x = __builtin_bit_cast(uint32_t, iv[u]);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+
+      if (more) stage2();
       for (uint32_t g0 = 0; g0 < nsel && nobs > 0; g0 += a.gcap) {
         const uint32_t G = min(a.gcap, nsel - g0);
-        // zero the XIC rows (with their pads) and the hit masks
-        {
-          float4 *xz = reinterpret_cast<float4 *>(l.X);
-          const uint32_t n4 = G * sh.lp / 4;
-          for (uint32_t i = lane; i < n4; i += kWave)
-            xz[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-          for (uint32_t i = lane; i < G * sh.wm; i += kWave) l.M[i] = 0ull;
-        }
-        __builtin_amdgcn_wave_barrier();
-        // ---- extraction: src/dense_xic_observation/mod.rs:42-55 ----------
-        for (uint32_t j = 0; j < nobs; ++j) {
-          const uint32_t o = uni(a.plan.obs[(size_t)row * kObsMax + j]);
-          const uint64_t base = a.dia.entry_base[o];
-          const size_t ridx = ((size_t)row * kObsMax + j) * ks + g0;
-          uint32_t rx = 0, rlen = 0, rb0 = 0;
-          if (lane < (int)G) {
-            const uint2 r = a.plan.rng[ridx + lane];
-            rx = r.x;
-            rlen = r.y - r.x;
-            rb0 = a.plan.b0[ridx + lane];
-          }
-          // 64-entry tiles; tile T belongs to the last fragment whose first
-          // tile tp is <= T (fragments without entries are skipped)
-          const uint32_t nt = (rlen + kWave - 1) / kWave;
-          const uint32_t tincl = wave_incl_scan(nt);
-          const uint32_t tp = tincl - nt;
-          const uint32_t NT = rdlane(tincl, G - 1);
-          for (uint32_t t0 = 0; t0 < NT; t0 += kTilesInFlight) {
-            uint32_t tf[kTilesInFlight], tk[kTilesInFlight];
-#pragma unroll
-            for (int u = 0; u < kTilesInFlight; ++u) {
-              const uint32_t T = t0 + u;
-              const uint64_t fm = ballot64(lane < (int)G && nt > 0 && tp <= T);
-              const uint32_t f = fm ? 63u - (uint32_t)__builtin_clzll(fm) : 0u;
-              tf[u] = T < NT ? f : 0xffffffffu;
-              tk[u] = T - rdlane(tp, f);
-            }
-            uint32_t cyc[kTilesInFlight];
-#pragma unroll
-            for (int u = 0; u < kTilesInFlight; ++u) {
-              cyc[u] = 0xffffffffu;
-              if (tf[u] != 0xffffffffu) {
-                const uint32_t k = tk[u] * kWave + lane;
-                if (k < rdlane(rlen, tf[u]))
-                  cyc[u] = a.dia.cycles[base + rdlane(rx, tf[u]) + k];
-              }
-            }
-            uint32_t hitm = 0;
-#pragma unroll
-            for (int u = 0; u < kTilesInFlight; ++u) {
-              const uint32_t c = cyc[u];
-              bool hit = c >= c0 && c < c1;
-              if (hit && c == c0) {
-                const uint32_t e = rdlane(rx, tf[u]) + tk[u] * kWave + lane;
-                if (skipped_duplicate(a.dia, o, base, rdlane(rb0, tf[u]), e, c0))
-                  hit = false;
-              }
-              hitm |= (hit ? 1u : 0u) << u;
-            }
-            float inten[kTilesInFlight];
-#pragma unroll
-            for (int u = 0; u < kTilesInFlight; ++u) {
-              inten[u] = 0.0f;
-              if ((hitm >> u) & 1u) {
-                const uint32_t e = rdlane(rx, tf[u]) + tk[u] * kWave + lane;
-                inten[u] = a.dia.intens[base + e];
-              }
-            }
-            // accumulate in entry order (tiles ascending, lanes ascending);
-            // a tile holds one fragment, so hits collide only on equal cycles
-#pragma unroll
-            for (int u = 0; u < kTilesInFlight; ++u) {
-              const bool hit = (hitm >> u) & 1u;
-              const uint64_t hm = ballot64(hit);
-              if (hm == 0) continue;
-              const uint32_t f = tf[u];
-              const uint32_t t = hit ? cyc[u] - c0 : 0u;
-              if (hit) l.own[t] = (uint8_t)lane;
-              __builtin_amdgcn_wave_barrier();
-              const bool dup = hit && l.own[t] != (uint8_t)lane;
-              float *xr = l.X + f * sh.lp + kXPad;
-              uint64_t *mr = l.M + f * sh.wm + 1;
-              if (ballot64(dup) == 0) {
-                if (hit) {
-                  xr[t] = xr[t] + inten[u];
-                  atomicOr((unsigned long long *)&mr[t >> 6], 1ull << (t & 63));
-                }
-              } else {
-                uint64_t it2 = hm;
-                while (it2) {
-                  const int j2 = __builtin_ctzll(it2);
-                  it2 &= it2 - 1;
-                  if (lane == j2) {
-                    xr[t] = xr[t] + inten[u];
-                    atomicOr((unsigned long long *)&mr[t >> 6], 1ull << (t & 63));
-                  }
-                  __builtin_amdgcn_wave_barrier();
-                }
-              }
-              __builtin_amdgcn_wave_barrier();
-            }
-          }
-        }
-        __builtin_amdgcn_wave_barrier();
-        // ---- active 4-cycle output blocks: hits dilated by +-half, within
-        //      the computed range [half, L-half) of the convolution --------
-        for (uint32_t idx = lane; idx < G * sh.wb * 4; idx += kWave) {
-          const uint32_t f = idx / (sh.wb * 4), w = idx % (sh.wb * 4);
-          uint32_t blocks = 0;
-          if (w < nw) {
-            const uint64_t *mr = l.M + f * sh.wm + 1;
-            uint64_t p = mr[(int)w - 1], c = mr[w], n = mr[w + 1];
-            uint32_t rem = half, s = 1;
-            while (rem) {
-              const uint32_t st = min(s, rem);
-              dilate_step(p, c, n, st);
-              rem -= st;
-              s <<= 1;
-            }
-            // valid outputs: half <= pos < L - half
-            const uint32_t lo = 64 * w, hi = lo + 64;
-            uint64_t vm = ~0ull;
-            if (half > lo) vm &= (half - lo >= 64) ? 0ull : (~0ull << (half - lo));
-            const uint32_t end = L >= half ? L - half : 0u;
-            if (end < hi) vm &= (end <= lo) ? 0ull : (~0ull >> (hi - end));
-            c &= vm;
-            const uint64_t nib = c | (c >> 1) | (c >> 2) | (c >> 3);
-            blocks = pack_nibbles(nib);
-          }
-          reinterpret_cast<uint16_t *>(l.BD + f * sh.wb)[w] = (uint16_t)blocks;
+        zero_group(G);
+        gA = g0;
+        gG = G;
+        if (listed) {
+          accumulate(true, 0);
+        } else {
+          nh = 0;
+          extract(g0, g0 + G, false);
         }
         __builtin_amdgcn_wave_barrier();
         // ---- compaction of (fragment, block) items, fragment-major -------
@@ -783,13 +986,23 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
             const float wf = l.w[g0 + f];
             // outputs outside [half, L - half) stay zero (scalar.rs:30-38)
             float t[4];
+            bool act[4];
+            bool odd = false;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-              t[q] = 0.0f;
               const uint32_t pos = i0 + q;
               const bool inside = pos >= half && pos + half < L;
-              if (v && inside && y[q] != 0.0f)
-                t[q] = logf_exact(y[q] + 1.0f, ltab) * wf;
+              act[q] = v && inside && y[q] != 0.0f;
+              odd = odd || (act[q] && !logf_is_normal(y[q] + 1.0f));
+            }
+            if (ballot64(odd) == 0) {
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                t[q] = act[q] ? logf_normal(y[q] + 1.0f, ltab) * wf : 0.0f;
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                t[q] = act[q] ? logf_exact(y[q] + 1.0f, ltab) * wf : 0.0f;
             }
             const uint32_t flo = rdlane(item >> 10, 0);
             const uint32_t last = min(n - c, (uint32_t)kWave) - 1;
@@ -837,54 +1050,65 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
         for (uint32_t i = lane; i < L; i += kWave) out[i] = l.R[i];
       }
       // ---- local maxima and top-k: mod.rs:23-64, 167-192 -----------------
+      // lane l tests positions 256m + 4l + k (k < 4) from three float4
+      // reads; bit 4m + k of ismax marks a strict 5-point maximum
       uint64_t ismax = 0;
-      for (uint32_t m = 0; m * kWave + lane < L && m < 64; ++m) {
-        const uint32_t i = m * kWave + lane;
-        if (i >= 2 && i + 2 < L) {
-          const float v = l.R[i];
-          if (l.R[i - 2] < l.R[i - 1] && l.R[i - 1] < v && v > l.R[i + 1] &&
-              l.R[i + 1] > l.R[i + 2])
-            ismax |= 1ull << m;
+      for (uint32_t m = 0; 256 * m < L; ++m) {
+        const uint32_t i0 = 256 * m + 4 * lane;
+        if (i0 < L) {
+          const float4 pa = *reinterpret_cast<const float4 *>(l.R + i0 - 4);
+          const float4 pb = *reinterpret_cast<const float4 *>(l.R + i0);
+          const float4 pc = *reinterpret_cast<const float4 *>(l.R + i0 + 4);
+          const float w[12] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y,
+                               pb.z, pb.w, pc.x, pc.y, pc.z, pc.w};
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const uint32_t i = i0 + k;
+            const float v = w[4 + k];
+            if (i >= 2 && i + 2 < L && w[2 + k] < w[3 + k] && w[3 + k] < v &&
+                v > w[5 + k] && w[5 + k] > w[6 + k])
+              ismax |= 1ull << (4 * m + k);
+          }
         }
       }
+      // ranks: descending score, ties to the lower cycle (stable sort)
+      flush_out();  // rows without extraction still own pending outputs
       uint32_t ncand = 0;
       for (uint32_t r = 0; r < cc; ++r) {
         float bv = 0.0f;
         uint32_t bi = 0xffffffffu;
         uint64_t it = ismax;
         while (it) {
-          const uint32_t m = __builtin_ctzll(it);
+          const uint32_t b = __builtin_ctzll(it);
           it &= it - 1;
-          const uint32_t i = m * kWave + lane;
+          const uint32_t i = 256 * (b >> 2) + 4 * lane + (b & 3);
           const float v = l.R[i];
           if (bi == 0xffffffffu || v > bv) {
             bv = v;
             bi = i;
           }
         }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-          const float ov = __shfl_xor(bv, o, kWave);
-          const uint32_t oi = __shfl_xor(bi, o, kWave);
-          const bool take = (oi != 0xffffffffu) &&
-                            (bi == 0xffffffffu || ov > bv || (ov == bv && oi < bi));
-          if (take) {
-            bv = ov;
-            bi = oi;
-          }
-        }
+        wave_argmax(bv, bi);
         if (bi == 0xffffffffu) break;
-        if ((bi & (kWave - 1)) == (uint32_t)lane) ismax &= ~(1ull << (bi / kWave));
-        if (lane == 0) {
+        if (((bi & 255u) >> 2) == (uint32_t)lane)
+          ismax &= ~(1ull << (((bi >> 8) << 2) | (bi & 3u)));
+        if (r < (uint32_t)kWave) {
+          if ((uint32_t)lane == r) {
+            out_s = bv;
+            out_c = c0 + bi;
+          }
+        } else if (lane == 0) {  // ranks beyond one per lane: stored now
           a.cand_score[(size_t)row * cc + r] = bv;
           a.cand_center[(size_t)row * cc + r] = c0 + bi;
         }
         ++ncand;
       }
-      a.cand_count[row] = ncand;
+      out_row = row;
+      out_n = ncand;
       __builtin_amdgcn_wave_barrier();
     }
   }
+  flush_out();
 }
 
 // ---------------------------------------------------------------------------
@@ -1322,11 +1546,13 @@ static adsr_status require_device() {
 }
 
 template <class T>
-static adsr_status dmalloc_copy(T **dst, const T *src, size_t n, size_t &acc) {
+static adsr_status dmalloc_copy(T **dst, const T *src, size_t n, size_t &acc,
+                                size_t pad = 0) {
   *dst = nullptr;
-  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  const size_t bytes = std::max<size_t>(n + pad, 1) * sizeof(T);
   HIP_TRY(hipMalloc((void **)dst, bytes));
   if (n) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  if (pad) HIP_TRY(hipMemset(*dst + n, 0xff, pad * sizeof(T)));
   acc += bytes;
   return ADSR_OK;
 }
@@ -1367,9 +1593,9 @@ static adsr_status upload_dia(adsr_dia *d) {
   const Grid &g = global_grid();
   DeviceDia &v = d->dev;
   size_t acc = 0;
-#define UP(dst, src, n)                                  \
+#define UP(dst, src, n, ...)                             \
   do {                                                   \
-    adsr_status s_ = dmalloc_copy(&(dst), (src), (n), acc); \
+    adsr_status s_ = dmalloc_copy(&(dst), (src), (n), acc, ##__VA_ARGS__); \
     if (s_) {                                            \
       device_dia_free(v);                                \
       return s_;                                         \
@@ -1382,7 +1608,8 @@ static adsr_status upload_dia(adsr_dia *d) {
   UP(v.iso_hi, d->iso_hi.data(), d->iso_hi.size());
   UP(v.slice_starts, d->slice_starts.data(), d->slice_starts.size());
   UP(v.entry_base, d->entry_base.data(), d->entry_base.size());
-  UP(v.cycles, d->cycles.data(), d->cycles.size());
+  // 4 spare entries: the select kernel reads cycles in aligned groups of 4
+  UP(v.cycles, d->cycles.data(), d->cycles.size(), 4);
   UP(v.intens, d->intens.data(), d->intens.size());
 #undef UP
   v.bytes = acc;
@@ -1397,9 +1624,9 @@ static adsr_status upload_lib(adsr_lib *l) {
   if (st) return st;
   DeviceLib &v = l->dev;
   size_t acc = 0;
-#define UP(dst, src, n)                                  \
+#define UP(dst, src, n, ...)                             \
   do {                                                   \
-    adsr_status s_ = dmalloc_copy(&(dst), (src), (n), acc); \
+    adsr_status s_ = dmalloc_copy(&(dst), (src), (n), acc, ##__VA_ARGS__); \
     if (s_) {                                            \
       device_lib_free(v);                                \
       return s_;                                         \
@@ -1425,7 +1652,7 @@ static adsr_status upload_lib(adsr_lib *l) {
 // ---------------------------------------------------------------------------
 struct LaunchShape {
   int waves = 1;  // waves per block: 1, 2 or 4 (64/128/256 threads)
-  int chunk = 1;  // precursors per queue grab: 1, 2, 4
+  int chunk = 1;  // precursors per queue grab: 1, 2, 4, 8 (16 on request)
 };
 
 struct Workspace {
@@ -1892,7 +2119,10 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
   const int want_tpb = tpb ? *tpb : 0, want_ppw = ppw ? *ppw : 0;
   if (want_tpb == 64 || want_tpb == 128 || want_tpb == 256) {
     shape.waves = want_tpb / 64;
-    shape.chunk = (want_ppw == 1 || want_ppw == 2 || want_ppw == 4) ? want_ppw : 1;
+    shape.chunk = (want_ppw == 1 || want_ppw == 2 || want_ppw == 4 ||
+                   want_ppw == 8 || want_ppw == 16)
+                      ? want_ppw
+                      : 1;
   } else if (steps > 0) {
     auto key = std::make_tuple((size_t)n, sp.lcap, w.ks);
     bool cached = false;
@@ -1905,11 +2135,11 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
       }
     }
     if (!cached) {
-      // autotuner: {64,128,256} threads x {1,2,4} precursors per queue grab,
-      // each timed on its second launch
+      // autotuner: {64,128,256} threads x {1,2,4,8} precursors per queue
+      // grab, each timed on its second launch
       float best = 1e30f;
       for (int wv : {1, 2, 4})
-        for (int ch : {1, 2, 4}) {
+        for (int ch : {1, 2, 4, 8}) {
           LaunchShape sh{wv, ch};
           for (int rep = 0; rep < 2; ++rep) {
             st = enqueue(w, sp, n, sh, nullptr, 0, 0, 0, true);

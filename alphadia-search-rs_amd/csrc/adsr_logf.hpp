@@ -80,4 +80,29 @@ __host__ __device__ __forceinline__ float logf_exact(float x,
   return res;
 }
 
+// The main path of logf_exact alone: valid for positive normal x
+// (0x00800000 <= bits(x) < 0x7f800000), where it gives the same bits.
+__host__ __device__ __forceinline__ float logf_normal(float x,
+                                                      const LogfEntry *tab) {
+  const uint32_t ix = __builtin_bit_cast(uint32_t, x);
+  const uint32_t tmp = ix - 0x3f330000u;
+  const int i = (int)((tmp >> 19) & 15u);
+  const int k = (int32_t)tmp >> 23;
+  const uint32_t iz = ix - (tmp & 0xff800000u);
+  const double invc = tab[i].invc, logc = tab[i].logc;
+  const double z = (double)__builtin_bit_cast(float, iz);
+  const double r = __builtin_fma(z, invc, -1.0);
+  const double y0 = __builtin_fma((double)k, kLogfLn2, logc);
+  const double r2 = r * r;
+  double y = __builtin_fma(kLogfA1, r, kLogfA2);
+  y = __builtin_fma(kLogfA0, r2, y);
+  y = __builtin_fma(y, r2, y0 + r);
+  return (float)y;
+}
+
+__host__ __device__ __forceinline__ bool logf_is_normal(float x) {
+  const uint32_t ix = __builtin_bit_cast(uint32_t, x);
+  return (ix - 0x00800000u) < (0x7f800000u - 0x00800000u);
+}
+
 }  // namespace adsr

@@ -148,7 +148,7 @@ adsr_status adsr_select(const adsr_dia *d, const adsr_lib *l,
  * duration of the select kernel, from event pairs around each launch.  When
  * graph_step_ms is non-NULL the search is also captured once in a hipGraph
  * and `steps` replays are timed.  threads_per_block in {64,128,256} and
- * precursors_per_wave in {1,2,4} pick the launch shape; 0 = autotuned.
+ * precursors_per_wave in {1,2,4,8,16} pick the launch shape; 0 = autotuned.
  * The chosen shape is written back when the pointers are non-NULL.  The
  * candidate table of the last search is returned.                        */
 adsr_status adsr_select_timed(const adsr_dia *d, const adsr_lib *l,
