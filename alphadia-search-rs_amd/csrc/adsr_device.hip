@@ -52,7 +52,7 @@ constexpr int kHalfMax = 15;     // kernel half width (kernel_size <= 30)
 constexpr int kLCapMax = 4096;   // cycles per precursor window
 constexpr int kStashFrag = 48;   // per-thread LDS stash in the plan kernel
 constexpr size_t kMaxDynLds = 160 * 1024 - 512;  // bytes per block
-constexpr uint32_t kLdsBudget = 12 * 1024;       // preferred bytes per wave
+constexpr uint32_t kLdsBudget = 10 * 1024;       // preferred bytes per wave
 
 enum StatusBits : uint32_t {
   kStTooManyFragments = 1u,
@@ -84,6 +84,8 @@ struct KDia {
   const uint64_t *entry_base;
   const uint16_t *cycles;
   const float *intens;
+  const uint4 *bucket2;  // (count below, grid[count], grid[count + 1], -)
+  float grid_first, grid_last;
 };
 
 struct KLib {
@@ -156,6 +158,23 @@ __device__ __forceinline__ uint32_t grid_ub(const KDia &d, float x) {
   uint32_t b = d.bucket[(__float_as_uint(x) >> kBucketShift) - d.base_key];
   while (b < n && m[b] <= x) ++b;
   return b;
+}
+
+// Bin bounds from one bucket2 entry.  A bucket spans 16 f32 ulps and a
+// 1-ppm bin at least 8.39, so at most two grid points lie in a bucket and
+// the entry's two values settle the bound exactly; a value past the bucket
+// compares false.  Valid for grid_first < x <= grid_last (lower) and
+// grid_first <= x < grid_last (upper).
+__device__ __forceinline__ uint32_t bucket_key(const KDia &d, float x) {
+  return (__float_as_uint(x) >> kBucketShift) - d.base_key;
+}
+__device__ __forceinline__ uint32_t lb_from(const uint4 &e, float x) {
+  return e.x + (__uint_as_float(e.y) < x ? 1u : 0u) +
+         (__uint_as_float(e.z) < x ? 1u : 0u);
+}
+__device__ __forceinline__ uint32_t ub_from(const uint4 &e, float x) {
+  return e.x + (__uint_as_float(e.y) <= x ? 1u : 0u) +
+         (__uint_as_float(e.z) <= x ? 1u : 0u);
 }
 
 // src/mz_index/mod.rs:95-127 -> [start, end)
@@ -295,13 +314,17 @@ struct StashFragAcc {
   __device__ void set_key(uint32_t j, uint32_t k) const { key_[j * kWave] = k; }
 };
 
+// STASH: fragments per thread kept in LDS (16, 32 or kStashFrag), chosen
+// from the library's longest fragment list so that short lists do not pay
+// for LDS they never use (occupancy).
+template <int STASH>
 __global__ __launch_bounds__(kWave) void plan_kernel(KDia dia, KLib lib,
                                                      KParams prm, KPlan plan,
                                                      uint32_t *key_scratch,
                                                      uint32_t *status) {
-  __shared__ float st_mz[kStashFrag * kWave];
-  __shared__ float st_I[kStashFrag * kWave];
-  __shared__ uint32_t st_key[kStashFrag * kWave];
+  __shared__ float st_mz[STASH * kWave];
+  __shared__ float st_I[STASH * kWave];
+  __shared__ uint32_t st_key[STASH * kWave];
   const uint32_t row = blockIdx.x * kWave + threadIdx.x;
   if (row >= lib.n_prec) return;
   uint32_t st = 0;
@@ -341,7 +364,7 @@ __global__ __launch_bounds__(kWave) void plan_kernel(KDia dia, KLib lib,
   float *smz = plan.sel_mz + (size_t)row * plan.ks;
   float *sw = plan.sel_w + (size_t)row * plan.ks;
   uint32_t nsel;
-  if (nraw <= (uint32_t)kStashFrag) {
+  if (nraw <= (uint32_t)STASH) {
     for (uint32_t j = 0; j < nraw; ++j) {
       const float v = lib.f_int[fs + j];
       const bool keep =
@@ -427,7 +450,7 @@ constexpr int kXPad = 16;
 constexpr int kItemCap = 512;   // u16 (fragment, block) items
 constexpr int kHitCap = 256;    // (u32 entry or intensity, u32 key) hits
 constexpr int kTileHits = 4 * 64;  // hits one extraction tile can add
-constexpr int kGroupTiles = 8;     // extraction tiles in flight
+constexpr int kGroupTiles = 2;     // extraction tiles in flight (8 entries per lane)
 
 struct LdsShape {
   uint32_t lp;     // XIC row stride (floats)
@@ -592,8 +615,12 @@ __device__ __forceinline__ void wave_argmax(float &bv, uint32_t &bi) {
   bi = (uint32_t)__builtin_amdgcn_readlane((int)bi, 63);
 }
 
+// Occupancy: at most 128 VGPRs (4 waves per SIMD) and, with the default
+// 10 KB of LDS per wave, 16 waves per CU; the kernel is latency-bound, and
+// measured faster there than with more registers or more LDS per wave.
 template <int WAVES, int HALF>
-__global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
+__global__ __launch_bounds__(WAVES *kWave) __attribute__((amdgpu_waves_per_eu(4)))
+void select_kernel(SelArgs a) {
   extern __shared__ __align__(16) char smem[];
   __shared__ LogfEntry ltab[16];
   if (threadIdx.x < 16) ltab[threadIdx.x] = kDevLogfTab[threadIdx.x];
@@ -640,35 +667,70 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
     }
   };
   uint4 p_rec = make_uint4(0u, 0u, 0u, 0u);
-  float p_sw = 0.0f, p_tap = 0.0f;
-  uint32_t p_obs = 0, p_b0 = 0;
+  float p_sw = 0.0f, p_tap = 0.0f, p_smz = 0.0f, p_lo = 0.0f, p_hi = 0.0f;
+  uint32_t p_obs = 0, p_bs = 0, p_be = 0;
+  uint4 p_elo = make_uint4(0u, 0u, 0u, 0u), p_ehi = make_uint4(0u, 0u, 0u, 0u);
   uint2 p_rng = make_uint2(0u, 0u);
   uint64_t p_eb = 0;
+  const float mtol = a.prm.mass_tol;
   auto stage1 = [&](uint32_t r) {
     p_rec = a.plan.rec[r];
     p_sw = lane < (int)ks ? a.plan.sel_w[(size_t)r * ks + lane] : 0.0f;
+    p_smz = lane < (int)ks ? a.plan.sel_mz[(size_t)r * ks + lane] : 0.0f;
     p_obs = lane < kObsMax ? a.plan.obs[(size_t)r * kObsMax + lane] : 0u;
-    p_rng = lane < (int)ks ? a.plan.rng[(size_t)r * kObsMax * ks + lane]
-                           : make_uint2(0u, 0u);
-    p_b0 = lane < (int)ks ? a.plan.b0[(size_t)r * kObsMax * ks + lane] : 0u;
   };
+  // src/quadrupole_observation/mod.rs:79-83 and src/mz_index/mod.rs:95-127:
+  // the fragment's ppm window and the bucket entries of its two bounds
   auto stage2 = [&]() {
     const uint32_t kidx = uni(p_rec.w), pobs = uni(p_rec.z >> 16);
+    const uint32_t pnsel = uni(p_rec.z & 0xffffu);
     p_tap = lane < (int)a.prm.klen ? a.prm.kernels[(size_t)kidx * a.prm.klen + lane]
                                    : 0.0f;
     p_eb = lane < (int)min(pobs, (uint32_t)kObsMax) ? a.dia.entry_base[p_obs] : 0ull;
+    const float delta = p_smz * mtol * 1e-6f;
+    p_lo = p_smz - delta;
+    p_hi = p_smz + delta;
+    const bool on = lane < (int)pnsel;
+    if (on && p_lo > a.dia.grid_first && !(p_lo > a.dia.grid_last))
+      p_elo = a.dia.bucket2[bucket_key(a.dia, p_lo)];
+    if (on && p_hi >= a.dia.grid_first && p_hi < a.dia.grid_last)
+      p_ehi = a.dia.bucket2[bucket_key(a.dia, p_hi)];
+  };
+  // bin range [p_bs, p_be) of each fragment, CSR entry range in window 0
+  auto stage3 = [&]() {
+    const uint32_t pobs = uni(p_rec.z >> 16), pnsel = uni(p_rec.z & 0xffffu);
+    const uint32_t n = a.dia.n_grid;
+    uint32_t st = 0, en = 0;
+    if (lane < (int)pnsel) {
+      st = !(p_lo > a.dia.grid_first) ? 0u
+           : (p_lo > a.dia.grid_last) ? n
+                                      : lb_from(p_elo, p_lo);
+      const uint32_t ub = !(p_hi >= a.dia.grid_first) ? 0u
+                          : (p_hi >= a.dia.grid_last) ? n
+                                                      : ub_from(p_ehi, p_hi);
+      en = st < n ? max(st, ub) : st;
+    }
+    p_bs = st;
+    p_be = en;
+    p_rng = make_uint2(0u, 0u);
+    if (pobs > 0 && lane < (int)pnsel) {
+      const uint32_t o0 = (uint32_t)__builtin_amdgcn_readlane((int)p_obs, 0);
+      const uint32_t *ssw = a.dia.ss + (size_t)o0 * (n + 1);
+      p_rng = make_uint2(ssw[st], ssw[en]);
+    }
   };
   grab();
   if (nxt < a.n_rows) {
     stage1(nxt);
     stage2();
+    stage3();
   }
   while (nxt < a.n_rows) {
     {
       const uint32_t row = nxt;
       const uint4 rec = p_rec;
       const float sw = p_sw, tap = p_tap;
-      const uint32_t obs_l = p_obs, b00 = p_b0;
+      const uint32_t obs_l = p_obs, bs_l = p_bs, be_l = p_be;
       const uint2 rng0 = p_rng;
       const uint64_t eb_l = p_eb;
       if (++nxt >= cend) grab();
@@ -682,14 +744,20 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
       // lanes 1..63 no longer follow lane 0 through the loop.
       if (c1 <= c0) {  // mod.rs:146-148 -> None
         a.cand_count[row] = 0;
-        if (more) stage2();
+        if (more) {
+          stage2();
+          stage3();
+        }
         continue;
       }
       const uint32_t L = c1 - c0;
       if (L > a.lcap) {  // excluded by the host bound; flagged if it happens
         a.cand_count[row] = 0;
         atomicOr(a.status, (uint32_t)kStWindowTooLong);
-        if (more) stage2();
+        if (more) {
+          stage2();
+          stage3();
+        }
         continue;
       }
       const uint32_t nw = (L + 63) / 64;  // 64-cycle words per row
@@ -782,47 +850,50 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
         for (uint32_t j = 0; j < nobs; ++j) {
           const uint32_t o = win_obs(j);
           const uint64_t base = win_base(j);
-          const size_t ridx = ((size_t)row * kObsMax + j) * ks + fa;
           uint32_t rx = 0, ry = 0, rb0 = 0;
-          if (lane < (int)FN) {
-            if (j == 0 && fa == 0) {
+          if (j == 0 && fa == 0) {
+            if (lane < (int)FN) {
               rx = rng0.x;
               ry = rng0.y;
-              rb0 = b00;
-            } else {
-              const uint2 r = a.plan.rng[ridx + lane];
-              rx = r.x;
-              ry = r.y;
-              rb0 = a.plan.b0[ridx + lane];
+              rb0 = bs_l;
+            }
+          } else {
+            // bins of fragment fa + lane, entries of window j
+            const uint32_t bs = (uint32_t)__shfl((int)bs_l, (int)min(fa + (uint32_t)lane, 63u), kWave);
+            const uint32_t be = (uint32_t)__shfl((int)be_l, (int)min(fa + (uint32_t)lane, 63u), kWave);
+            if (lane < (int)FN) {
+              const uint32_t *ssw = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
+              rx = ssw[bs];
+              ry = ssw[be];
+              rb0 = bs;
             }
           }
-          // Groups of 4 entries are counted from the window's aligned start
-          // gw = base / 4 (group g covers entries 4(gw + g) .. +3), so one
-          // 8-byte load at cw + 4g reads a group.  Fragment fa + lane spans
-          // groups gsr .. gsr + ng - 1; its first and last groups are
-          // partial (entries ko .. 3 and 0 .. kl).
-          const uint32_t bo = (uint32_t)base & 3u;
+          // Groups of 8 entries are counted from the window's 8-aligned
+          // start (group g covers entries 8g - bo .. 8g - bo + 7 of the
+          // window), so one 16-byte load at cw + 8g reads a group.
+          // Fragment fa + lane spans groups gsr .. gsr + ng - 1; its first
+          // and last groups are partial.
+          const uint32_t bo = (uint32_t)base & 7u;
           const uint16_t *cw = a.dia.cycles + (base - bo);
-          const uint32_t gsr = (bo + rx) >> 2;
-          const uint32_t ng = ry > rx ? ((bo + ry - 1) >> 2) - gsr + 1 : 0u;
-          // packed: first-group offset, last-group end, group count
-          const uint32_t meta = ((bo + rx) & 3u) | (((bo + ry - 1) & 3u) << 2) | (ng << 4);
+          const uint32_t gsr = (bo + rx) >> 3;
+          const uint32_t ng = ry > rx ? ((bo + ry - 1) >> 3) - gsr + 1 : 0u;
+          // packed: first entry in the first group, last in the last, count
+          const uint32_t meta = ((bo + rx) & 7u) | (((bo + ry - 1) & 7u) << 3) | (ng << 6);
           const uint32_t gincl = wave_incl_scan(ng);
           const uint32_t gp = gincl - ng;  // first virtual group
           const uint32_t NG = rdlane(gincl, FN - 1);
           for (uint32_t vb = 0; vb < NG; vb += kWave * kGroupTiles) {
-            uint2 raw[kGroupTiles];
-            uint32_t tg[kGroupTiles], tfr[kGroupTiles], tkl[kGroupTiles],
-                tkh[kGroupTiles], tb0[kGroupTiles];
+            uint4 raw[kGroupTiles];
+            uint32_t tg[kGroupTiles], tfr[kGroupTiles], tvm[kGroupTiles],
+                tb0[kGroupTiles];
 #pragma unroll
             for (int u = 0; u < kGroupTiles; ++u) {
               const uint32_t v0 = vb + (uint32_t)(u * kWave);
               const uint32_t v = v0 + lane;
-              raw[u] = make_uint2(0xffffffffu, 0xffffffffu);
+              raw[u] = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
               tg[u] = 0;
               tfr[u] = 0;
-              tkl[u] = 4;  // no entry of this group is in range
-              tkh[u] = 0;
+              tvm[u] = 0;  // entries of this group that belong to the fragment
               tb0[u] = 0;
               if (v0 < NG) {
                 const uint32_t vmax = min(v0 + (uint32_t)kWave - 1, NG - 1);
@@ -847,10 +918,11 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
                 if (v < NG) {
                   const uint32_t q = v - fgp;  // group within the fragment
                   tg[u] = fgs + q;
-                  raw[u] = *reinterpret_cast<const uint2 *>(cw + 4 * (size_t)tg[u]);
+                  raw[u] = *reinterpret_cast<const uint4 *>(cw + 8 * (size_t)tg[u]);
                   tfr[u] = fa + f;
-                  tkl[u] = (q == 0) ? (fmeta & 3u) : 0u;
-                  tkh[u] = (q + 1 == (fmeta >> 4)) ? ((fmeta >> 2) & 3u) : 3u;
+                  const uint32_t klo = (q == 0) ? (fmeta & 7u) : 0u;
+                  const uint32_t khi = (q + 1 == (fmeta >> 6)) ? ((fmeta >> 3) & 7u) : 7u;
+                  tvm[u] = (0xffu << klo) & (0xffu >> (7u - khi));
                   tb0[u] = fb0;
                 }
               }
@@ -859,55 +931,60 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
 #pragma unroll
             for (int u = 0; u < kGroupTiles; ++u) {
               if (vb + (uint32_t)(u * kWave) >= NG) break;
-              const uint32_t cyc[4] = {raw[u].x & 0xffffu, raw[u].x >> 16,
-                                       raw[u].y & 0xffffu, raw[u].y >> 16};
+              const uint32_t cyc[8] = {raw[u].x & 0xffffu, raw[u].x >> 16,
+                                       raw[u].y & 0xffffu, raw[u].y >> 16,
+                                       raw[u].z & 0xffffu, raw[u].z >> 16,
+                                       raw[u].w & 0xffffu, raw[u].w >> 16};
               uint32_t hm = 0;
 #pragma unroll
-              for (int k = 0; k < 4; ++k) {
-                const bool in = (uint32_t)k >= tkl[u] && (uint32_t)k <= tkh[u] &&
-                                cyc[k] >= c0 && cyc[k] < c1;
-                hm |= (in ? 1u : 0u) << k;
-              }
+              for (int k = 0; k < 8; ++k)
+                hm |= (cyc[k] - c0 < L ? 1u : 0u) << k;
+              hm &= tvm[u];
               // the reference's binary_search(c0) starts at the last entry
               // equal to c0 in a bin: earlier ones are skipped
               bool at_c0 = false;
 #pragma unroll
-              for (int k = 0; k < 4; ++k)
+              for (int k = 0; k < 8; ++k)
                 at_c0 = at_c0 || (((hm >> k) & 1u) && cyc[k] == c0);
               if (ballot64(at_c0) != 0) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < 8; ++k) {
                   if (((hm >> k) & 1u) && cyc[k] == c0) {
-                    const uint32_t e = 4 * tg[u] + k - bo;
+                    const uint32_t e = 8 * tg[u] + k - bo;
                     if (skipped_duplicate(a.dia, o, base, tb0[u], e, c0))
                       hm &= ~(1u << k);
                   }
                 }
               }
-              // list slots in (lane, k) order = entry order
-              uint32_t pre = 0, tot = 0;
+              // list slots in (lane, k) order = entry order, in two halves
+              // of at most 4 x 64 hits each
 #pragma unroll
-              for (int k = 0; k < 4; ++k) {
-                const uint64_t bk = ballot64((hm >> k) & 1u);
-                pre += __builtin_amdgcn_mbcnt_hi(
-                    (uint32_t)(bk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bk, 0u));
-                tot += (uint32_t)__builtin_popcountll(bk);
-              }
-              if (deferred && nh + tot > (uint32_t)kHitCap) return false;
-              uint32_t pos = nh + pre;
+              for (int hh = 0; hh < 2; ++hh) {
+                uint32_t pre = 0, tot = 0;
 #pragma unroll
-              for (int k = 0; k < 4; ++k) {
-                if ((hm >> k) & 1u) {
-                  const uint32_t e = 4 * tg[u] + k - bo;
-                  l.hits[pos] = make_uint2(e, (j << 18) | (tfr[u] << 12) | (cyc[k] - c0));
-                  ++pos;
+                for (int k = 4 * hh; k < 4 * hh + 4; ++k) {
+                  const uint64_t bk = ballot64((hm >> k) & 1u);
+                  pre += __builtin_amdgcn_mbcnt_hi(
+                      (uint32_t)(bk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bk, 0u));
+                  tot += (uint32_t)__builtin_popcountll(bk);
                 }
-              }
-              nh += tot;
-              __builtin_amdgcn_wave_barrier();
-              if (!deferred && nh > (uint32_t)(kHitCap - kTileHits)) {
-                accumulate(false, base);
-                nh = 0;
+                if (tot == 0) continue;
+                if (deferred && nh + tot > (uint32_t)kHitCap) return false;
+                uint32_t pos = nh + pre;
+#pragma unroll
+                for (int k = 4 * hh; k < 4 * hh + 4; ++k) {
+                  if ((hm >> k) & 1u) {
+                    const uint32_t e = 8 * tg[u] + k - bo;
+                    l.hits[pos] = make_uint2(e, (j << 18) | (tfr[u] << 12) | (cyc[k] - c0));
+                    ++pos;
+                  }
+                }
+                nh += tot;
+                __builtin_amdgcn_wave_barrier();
+                if (!deferred && nh > (uint32_t)(kHitCap - kTileHits)) {
+                  accumulate(false, base);
+                  nh = 0;
+                }
               }
             }
           }
@@ -1071,6 +1148,7 @@ __global__ __launch_bounds__(WAVES *kWave) void select_kernel(SelArgs a) {
           }
         }
       }
+      if (more) stage3();
       // ranks: descending score, ties to the lower cycle (stable sort)
       flush_out();  // rows without extraction still own pending outputs
       uint32_t ncand = 0;
@@ -1561,6 +1639,7 @@ void device_dia_free(DeviceDia &d) {
   if (!d.ready && !d.grid) return;
   (void)hipFree(d.grid);
   (void)hipFree(d.bucket);
+  (void)hipFree(d.bucket2);
   (void)hipFree(d.rt);
   (void)hipFree(d.iso_lo);
   (void)hipFree(d.iso_hi);
@@ -1603,18 +1682,55 @@ static adsr_status upload_dia(adsr_dia *d) {
   } while (0)
   UP(v.grid, g.mz.data(), g.mz.size());
   UP(v.bucket, g.bucket.data(), g.bucket.size());
+  {
+    std::vector<uint32_t> b2(4 * g.bucket.size());
+    const size_t n = g.mz.size();
+    const uint32_t inf = 0x7f800000u;
+    for (size_t b = 0; b < g.bucket.size(); ++b) {
+      const uint32_t c = g.bucket[b];
+      b2[4 * b + 0] = c;
+      b2[4 * b + 1] = c < n ? __builtin_bit_cast(uint32_t, g.mz[c]) : inf;
+      b2[4 * b + 2] = c + 1 < n ? __builtin_bit_cast(uint32_t, g.mz[c + 1]) : inf;
+      b2[4 * b + 3] = 0;
+    }
+    UP(v.bucket2, b2.data(), b2.size());
+  }
   UP(v.rt, d->rt_index.data(), d->rt_index.size());
   UP(v.iso_lo, d->iso_lo.data(), d->iso_lo.size());
   UP(v.iso_hi, d->iso_hi.data(), d->iso_hi.size());
   UP(v.slice_starts, d->slice_starts.data(), d->slice_starts.size());
   UP(v.entry_base, d->entry_base.data(), d->entry_base.size());
-  // 4 spare entries: the select kernel reads cycles in aligned groups of 4
-  UP(v.cycles, d->cycles.data(), d->cycles.size(), 4);
+  // 8 spare entries: the select kernel reads cycles in aligned groups of 8
+  UP(v.cycles, d->cycles.data(), d->cycles.size(), 8);
   UP(v.intens, d->intens.data(), d->intens.size());
 #undef UP
   v.bytes = acc;
   v.ready = true;
   return ADSR_OK;
+}
+
+static KDia make_kdia(const adsr_dia *d) {
+  const Grid &g = global_grid();
+  const DeviceDia &dv = d->dev;
+  KDia k;
+  k.grid = dv.grid;
+  k.bucket = dv.bucket;
+  k.n_grid = (uint32_t)g.mz.size();
+  k.base_key = g.base_key;
+  k.rt = dv.rt;
+  k.n_rt = (uint32_t)d->rt_index.size();
+  k.delta_t = d->delta_t;
+  k.n_obs = (uint32_t)d->n_obs;
+  k.iso_lo = dv.iso_lo;
+  k.iso_hi = dv.iso_hi;
+  k.ss = dv.slice_starts;
+  k.entry_base = dv.entry_base;
+  k.cycles = dv.cycles;
+  k.intens = dv.intens;
+  k.bucket2 = reinterpret_cast<const uint4 *>(dv.bucket2);
+  k.grid_first = g.mz.front();
+  k.grid_last = g.mz.back();
+  return k;
 }
 
 static adsr_status upload_lib(adsr_lib *l) {
@@ -1739,6 +1855,7 @@ struct SearchPlan {
   KPlan plan;
   uint32_t lcap = 0, xcap = 0, gcap = 0, lds_per_wave = 0;
   int cus = 256;
+  uint64_t max_raw = 0;  // longest raw fragment list of the library
   std::vector<float> kernels;
 };
 
@@ -1850,19 +1967,33 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
                            LaunchShape shape, float *dbg,
                            uint32_t dbg_stride, uint32_t dbg_row0,
                            uint32_t dbg_rows, bool timed_select,
-                           bool plan_only = false) {
+                           bool plan_only = false, bool with_bins = false) {
   hipStream_t s = w.stream;
   uint32_t *queue = w.small, *status = w.small + 1, *total = w.small + 2;
   reset_queue_kernel<<<1, 64, 0, s>>>(queue, status);
+  auto launch_plan = [&]() {
+    const uint32_t nb = (n_rows + kWave - 1) / kWave;
+    if (sp.max_raw <= 16)
+      plan_kernel<16><<<nb, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm, sp.plan,
+                                            w.key_scratch, status);
+    else if (sp.max_raw <= 32)
+      plan_kernel<32><<<nb, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm, sp.plan,
+                                            w.key_scratch, status);
+    else
+      plan_kernel<kStashFrag><<<nb, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm, sp.plan,
+                                                    w.key_scratch, status);
+  };
   if (n_rows && plan_only) {
-    plan_kernel<<<(n_rows + kWave - 1) / kWave, kWave, 0, s>>>(
-        sp.dia, sp.lib, sp.prm, sp.plan, w.key_scratch, status);
+    launch_plan();
   } else if (n_rows) {
-    plan_kernel<<<(n_rows + kWave - 1) / kWave, kWave, 0, s>>>(
-        sp.dia, sp.lib, sp.prm, sp.plan, w.key_scratch, status);
-    const uint64_t nt = (uint64_t)n_rows * sp.plan.ks;
-    bins_kernel<<<(uint32_t)((nt + 255) / 256), 256, 0, s>>>(sp.dia, sp.lib,
-                                                              sp.prm, sp.plan);
+    launch_plan();
+    // the select kernel finds bin and CSR ranges itself; the per-window
+    // range table is only built for the byte count and the backward pass
+    if (with_bins) {
+      const uint64_t nt = (uint64_t)n_rows * sp.plan.ks;
+      bins_kernel<<<(uint32_t)((nt + 255) / 256), 256, 0, s>>>(sp.dia, sp.lib,
+                                                                sp.prm, sp.plan);
+    }
     SelArgs a;
     a.dia = sp.dia;
     a.lib = sp.lib;
@@ -1941,11 +2072,7 @@ static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
                       l->n_prec * 4, hipMemcpyHostToDevice));
   const Grid &g = global_grid();
   const DeviceDia &dv = d->dev;
-  sp.dia = KDia{dv.grid,          dv.bucket,      (uint32_t)g.mz.size(),
-                g.base_key,       dv.rt,          (uint32_t)d->rt_index.size(),
-                d->delta_t,       (uint32_t)d->n_obs, dv.iso_lo,
-                dv.iso_hi,        dv.slice_starts, dv.entry_base,
-                dv.cycles,        dv.intens};
+  sp.dia = make_kdia(d);
   const DeviceLib &lv = l->dev;
   sp.lib = KLib{(uint32_t)l->n_prec, lv.p_mz,  lv.p_rt, lv.f_start, lv.f_stop,
                 lv.f_mz,             lv.f_int, lv.f_type, lv.f_num};
@@ -1960,6 +2087,7 @@ static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
   sp.prm.klen = (uint32_t)klen;
   sp.prm.per_window = p->sigma_per_window ? 1u : 0u;
   sp.prm.kernels = w.kernels;
+  sp.max_raw = l->max_raw_fragments;
   sp.plan = KPlan{w.rec, w.obs, w.sel, w.sel_mz, w.sel_w, w.rng, w.b0,
                   (uint32_t)ks};
   return ADSR_OK;
@@ -2269,7 +2397,7 @@ extern "C" adsr_status adsr_select_algorithmic_bytes(
   SearchPlan sp;
   st = build_search(d, l, p, w, sp);
   const uint32_t n = (uint32_t)l->n_prec;
-  if (!st) st = enqueue(w, sp, n, LaunchShape(), nullptr, 0, 0, 0, false);
+  if (!st) st = enqueue(w, sp, n, LaunchShape(), nullptr, 0, 0, 0, false, false, true);
   if (!st) st = check_status(w);
   unsigned long long *acc = nullptr;
   if (!st && hipMalloc(&acc, 32) != hipSuccess)
@@ -2523,7 +2651,7 @@ adsr_status adsr_diff_forward(adsr_diff *h, const float *sigma,
   LaunchShape shape{4, 2};
   const uint32_t n = (uint32_t)l->n_prec;
   HIP_TRY(hipEventRecord(s->ev[0], s->w.stream));
-  adsr_status st = enqueue(s->w, s->sp, n, shape, nullptr, 0, 0, 0, false);
+  adsr_status st = enqueue(s->w, s->sp, n, shape, nullptr, 0, 0, 0, false, false, true);
   if (st) return st;
   if (n)
     sstar_kernel<<<(n + 255) / 256, 256, 0, s->w.stream>>>(
@@ -2725,10 +2853,7 @@ extern "C" adsr_status adsr_dense_xic(const adsr_dia *dc, float pmz,
   if (nf == 0 || L == 0) return ADSR_OK;
   const Grid &g = global_grid();
   const DeviceDia &dv = d->dev;
-  KDia kd{dv.grid,       dv.bucket,  (uint32_t)g.mz.size(), g.base_key,
-          dv.rt,         (uint32_t)d->rt_index.size(), d->delta_t,
-          (uint32_t)d->n_obs, dv.iso_lo, dv.iso_hi, dv.slice_starts,
-          dv.entry_base, dv.cycles,  dv.intens};
+  KDia kd = make_kdia(d);
   return with_buffers(nf * 4, nf * L * 4, [&](void *din, void *dout) {
     HIP_TRY(hipMemcpy(din, fmz, nf * 4, hipMemcpyHostToDevice));
     dense_xic_kernel<<<(uint32_t)((nf + 63) / 64), 64>>>(

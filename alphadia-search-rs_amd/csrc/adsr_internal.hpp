@@ -45,6 +45,9 @@ struct DeviceDia {
   int device = -1;
   float *grid = nullptr;
   uint32_t *bucket = nullptr;
+  // per bucket: grid points below it and the first two grid values from it
+  // (u32 count, f32, f32, unused)
+  uint32_t *bucket2 = nullptr;
   float *rt = nullptr;
   float *iso_lo = nullptr, *iso_hi = nullptr;
   uint32_t *slice_starts = nullptr;  // n_obs * (n_grid + 1)

@@ -91,6 +91,17 @@ def lib():
     return _lib
 
 
+def use_library(path):
+    """Load the oracle from another build of adsr_oracle.c from now on
+    (the planted-mistake builds of tests/test_oracle_mutants.py); returns
+    the previous path."""
+    global _lib, LIB_PATH
+    prev = LIB_PATH
+    LIB_PATH = path
+    _lib = None
+    return prev
+
+
 def p(a):
     return None if a is None else a.ctypes.data_as(vp)
 

@@ -16,6 +16,13 @@
  */
 #include "adsr_oracle.h"
 
+/* ORC_MUTANT selects a planted mistake for the tests that show the
+ * oracle's pinning checks catch it (tests/test_oracle_mutants.py); 0 is
+ * the oracle itself. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -132,7 +139,11 @@ int orc_rust_bsearch_u16(const uint16_t *v, size_t n, uint16_t key,
   while (size > 1) {
     size_t half = size / 2;
     size_t mid = base + half;
+#if ORC_MUTANT == 5 /* planted: lower bound (first equal) instead of last */
+    if (v[mid] < key) base = mid;
+#else
     if (!(v[mid] > key)) base = mid;
+#endif
     size -= half;
   }
   if (v[base] == key) {
@@ -462,7 +473,11 @@ size_t orc_gaussian_kernel_sigma(float sigma, size_t size, float *out,
   float normalization = 1.0f / (sigma * sqrtf(2.0f * PI_F));
   for (int x = -half; x < half; x++) {
     float xf = (float)x;
+#if ORC_MUTANT == 6 /* planted: the 1/2 of the exponent dropped */
+    float exponent = (-1.0f * (xf * xf)) / (sigma * sigma);
+#else
     float exponent = (-0.5f * (xf * xf)) / (sigma * sigma);
+#endif
     out[x + half] = normalization * expf(exponent);
   }
   float sum = orc_ndarray_sum(out, size);
@@ -498,12 +513,18 @@ void orc_convolution(const float *k, size_t klen, const float *xic,
     if (start < end) {
       for (size_t i = start; i < end; i++) {
         float sum = 0.0f;
+#if ORC_MUTANT != 2 /* planted (2): the centre term dropped */
         sum = sum + x[i] * k[half];
+#endif
         for (size_t kk = 0; kk < half; kk++) {
           float kv = k[kk];
           float l = x[i - (half - kk)];
           float r = x[i + (half - kk)];
+#if ORC_MUTANT == 1 /* planted: tap index off by one */
+          sum = sum + (l + r) * k[kk + 1];
+#else
           sum = sum + (l + r) * kv;
+#endif
         }
         y[i] = sum;
       }
@@ -520,7 +541,11 @@ void orc_axis_log_dot_product(const float *a, size_t rows, size_t cols,
   for (size_t j = 0; j < cols; j++) out[j] = 0.0f;
   for (size_t i = 0; i < rows; i++)
     for (size_t j = 0; j < cols; j++) {
+#if ORC_MUTANT == 3 /* planted: ln(x) instead of ln(x + 1) */
+      float val = logf(a[i * cols + j]);
+#else
       float val = logf(a[i * cols + j] + 1.0f);
+#endif
       out[j] = out[j] + val * w[i];
     }
 }
@@ -532,8 +557,13 @@ size_t orc_find_local_maxima(const float *a, size_t len, size_t offset,
   size_t n = 0;
   if (len < 5) return 0;
   for (size_t i = 2; i < len - 2; i++) {
+#if ORC_MUTANT == 4 /* planted: non-strict rise into the peak */
+    if (a[i - 2] < a[i - 1] && a[i - 1] <= a[i] && a[i] > a[i + 1] &&
+        a[i + 1] > a[i + 2]) {
+#else
     if (a[i - 2] < a[i - 1] && a[i - 1] < a[i] && a[i] > a[i + 1] &&
         a[i + 1] > a[i + 2]) {
+#endif
       indices[n] = i + offset;
       values[n] = a[i];
       n++;

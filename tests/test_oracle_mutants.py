@@ -1,0 +1,63 @@
+"""The oracle's pinning checks catch plausible mistakes.
+
+adsr_oracle.c built with -DORC_MUTANT=n plants one mistake; each build is
+loaded in place of the oracle and the checks of test_oracle.py that pin
+that function must fail on it, while they pass on the oracle itself.
+"""
+import os
+import subprocess
+
+import pytest
+
+import oracle
+import test_oracle as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(os.path.dirname(HERE), "oracle", "adsr_oracle.c")
+
+# planted mistake -> (what, the checks that pin it)
+MUTANTS = {
+    1: ("convolution tap index off by one",
+        [T.test_convolution_constant_row_closed_form,
+         T.test_convolution_impulse_is_the_mirrored_kernel]),
+    2: ("convolution centre term dropped",
+        [T.test_convolution_constant_row_closed_form,
+         T.test_convolution_impulse_is_the_mirrored_kernel]),
+    3: ("score ln(x) instead of ln(x + 1)", [T.test_score_kat_and_closed_form]),
+    4: ("local maxima with a non-strict rise",
+        [T.test_local_maxima_brute_force_and_stable_order]),
+    5: ("binary search returns the first equal element",
+        [T.test_rust_bsearch_last_equal_and_insertion_point]),
+    6: ("Gaussian exponent without the 1/2", [T.test_kernel_closed_form_and_symmetry]),
+}
+
+
+@pytest.fixture(scope="module")
+def builds(tmp_path_factory):
+    out = tmp_path_factory.mktemp("mutants")
+    paths = {}
+    for m in MUTANTS:
+        so = str(out / f"liboracle_m{m}.so")
+        subprocess.run(
+            ["gcc", "-O1", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-std=c11",
+             "-shared", f"-DORC_MUTANT={m}", "-o", so, SRC, "-lm", "-lpthread"],
+            check=True)
+        paths[m] = so
+    return paths
+
+
+@pytest.mark.parametrize("m", sorted(MUTANTS), ids=lambda m: MUTANTS[m][0])
+def test_planted_mistake_is_caught(builds, m):
+    prev = oracle.use_library(builds[m])
+    try:
+        for check in MUTANTS[m][1]:
+            with pytest.raises(AssertionError):
+                check()
+    finally:
+        oracle.use_library(prev)
+
+
+def test_checks_pass_on_the_oracle():
+    for _, checks in MUTANTS.values():
+        for check in checks:
+            check()
