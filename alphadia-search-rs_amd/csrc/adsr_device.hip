@@ -583,6 +583,25 @@ __device__ __forceinline__ void conv4_any(const float *row, uint32_t i0,
   }
 }
 
+// Slots of each lane's hits in a list ordered by lane, then entry: `pre` is
+// the number of hits (set bits of the 8-bit mask hm) of the active lanes
+// below this one, `tot` the active lanes' total.  The per-lane count (at
+// most 8) is summed bit by bit with one ballot per bit.
+__device__ __forceinline__ void list_slots(uint32_t hm, bool active, uint32_t &pre,
+                                           uint32_t &tot) {
+  const uint32_t h = active ? (uint32_t)__builtin_popcount(hm) : 0u;
+  pre = 0;
+  tot = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const uint64_t bb = ballot64((h >> b) & 1u);
+    pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(bb >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)bb, 0u))
+           << b;
+    tot += (uint32_t)__builtin_popcountll(bb) << b;
+  }
+}
+
 // One step of a wave-wide arg-max over (value, position) pairs through a
 // DPP lane permutation; lanes without a source keep their own pair.
 // Order: larger value first, then the lower position; ~0u marks no pair.
@@ -956,27 +975,23 @@ void select_kernel(SelArgs a) {
                   }
                 }
               }
-              // list slots in (lane, k) order = entry order, in two halves
-              // of at most 4 x 64 hits each
+              // list slots in (lane, k) order = entry order; lanes 0-31
+              // first, then 32-63 (at most 32 x 8 hits each)
 #pragma unroll
               for (int hh = 0; hh < 2; ++hh) {
-                uint32_t pre = 0, tot = 0;
-#pragma unroll
-                for (int k = 4 * hh; k < 4 * hh + 4; ++k) {
-                  const uint64_t bk = ballot64((hm >> k) & 1u);
-                  pre += __builtin_amdgcn_mbcnt_hi(
-                      (uint32_t)(bk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bk, 0u));
-                  tot += (uint32_t)__builtin_popcountll(bk);
-                }
+                uint32_t pre, tot;
+                list_slots(hm, (lane >> 5) == hh, pre, tot);
                 if (tot == 0) continue;
                 if (deferred && nh + tot > (uint32_t)kHitCap) return false;
-                uint32_t pos = nh + pre;
+                if ((lane >> 5) == hh) {
+                  uint32_t pos = nh + pre;
 #pragma unroll
-                for (int k = 4 * hh; k < 4 * hh + 4; ++k) {
-                  if ((hm >> k) & 1u) {
-                    const uint32_t e = 8 * tg[u] + k - bo;
-                    l.hits[pos] = make_uint2(e, (j << 18) | (tfr[u] << 12) | (cyc[k] - c0));
-                    ++pos;
+                  for (int k = 0; k < 8; ++k) {
+                    if ((hm >> k) & 1u) {
+                      const uint32_t e = 8 * tg[u] + k - bo;
+                      l.hits[pos] = make_uint2(e, (j << 18) | (tfr[u] << 12) | (cyc[k] - c0));
+                      ++pos;
+                    }
                   }
                 }
                 nh += tot;
@@ -1346,9 +1361,12 @@ __global__ void sstar_kernel(const float *cs, const uint32_t *cnt, uint32_t cc,
   if (has) has[r] = h ? 1 : 0;
 }
 
+constexpr int kBwdHitCap = 256;
+
 __global__ __launch_bounds__(kWave) void diff_backward_kernel(BwdArgs a) {
   __shared__ __align__(16) float xw[kMaxSel * kWin];
-  __shared__ uint16_t own[kMaxSel * kWin];
+  __shared__ uint8_t own[kMaxSel * kWin];
+  __shared__ uint2 hits[kBwdHitCap];
   __shared__ double part[kMaxSel];
   const int lane = lane_id();
   const uint32_t half = a.prm.half;
@@ -1372,59 +1390,131 @@ __global__ __launch_bounds__(kWave) void diff_backward_kernel(BwdArgs a) {
     double total = 0.0;
     if (!(ncand == 0 || g == 0.0f || c1 <= c0 || nobs == 0 || nsel == 0)) {
       const uint32_t istar = uni(a.cand_center[(size_t)row * a.cc]) - c0;
-      const uint32_t hc0 = c0 + istar - half, hc1 = hc0 + 2 * half + 1;
+      // cycles [hc0, hc1) of the apex window; x[t] = X[hc0 + t]
+      const uint32_t hc0 = c0 + istar - half;
+      const uint32_t wl = max(hc0, c0), wh = min(hc0 + 2 * half + 1, c1);
       for (uint32_t i = lane; i < nsel * kWin; i += kWave) xw[i] = 0.0f;
       __builtin_amdgcn_wave_barrier();
-      // XIC around the apex, accumulated in the order of the forward pass
+      // the XIC window, summed in the forward pass's order: windows, then
+      // fragments' CSR entries in order (8-entry groups, 64 per tile)
       for (uint32_t j = 0; j < nobs; ++j) {
         const uint32_t o = uni(a.plan.obs[(size_t)row * kObsMax + j]);
         const uint64_t base = a.dia.entry_base[o];
         const size_t ridx = ((size_t)row * kObsMax + j) * ks;
-        uint32_t rx = 0, rlen = 0, rb0 = 0;
+        uint32_t rx = 0, ry = 0, rb0 = 0;
         if (lane < (int)nsel) {
           const uint2 r = a.plan.rng[ridx + lane];
           rx = r.x;
-          rlen = r.y - r.x;
+          ry = r.y;
           rb0 = a.plan.b0[ridx + lane];
         }
-        const uint32_t nt = (rlen + kWave - 1) / kWave;
-        const uint32_t tincl = wave_incl_scan(nt);
-        const uint32_t tp = tincl - nt;
-        const uint32_t NT = rdlane(tincl, nsel - 1);
-        for (uint32_t T = 0; T < NT; ++T) {
-          const uint64_t fm = ballot64(lane < (int)nsel && nt > 0 && tp <= T);
-          const uint32_t f = fm ? 63u - (uint32_t)__builtin_clzll(fm) : 0u;
-          const uint32_t k = (T - rdlane(tp, f)) * kWave + lane;
-          bool hit = false;
-          uint32_t c = 0, e = 0;
-          if (k < rdlane(rlen, f)) {
-            e = rdlane(rx, f) + k;
-            c = a.dia.cycles[base + e];
-            hit = c >= hc0 && c < hc1 && c >= c0 && c < c1;
-            if (hit && c == c0 &&
-                skipped_duplicate(a.dia, o, base, rdlane(rb0, f), e, c0))
-              hit = false;
+        const uint32_t bo = (uint32_t)base & 7u;
+        const uint16_t *cw = a.dia.cycles + (base - bo);
+        const uint32_t gsr = (bo + rx) >> 3;
+        const uint32_t ng = ry > rx ? ((bo + ry - 1) >> 3) - gsr + 1 : 0u;
+        const uint32_t meta = ((bo + rx) & 7u) | (((bo + ry - 1) & 7u) << 3) | (ng << 6);
+        const uint32_t gincl = wave_incl_scan(ng);
+        const uint32_t gp = gincl - ng;
+        const uint32_t NG = rdlane(gincl, nsel - 1);
+        uint32_t nh = 0;
+        auto flush = [&]() {
+          for (uint32_t c = 0; c < nh; c += kWave) {
+            const uint32_t idx = c + lane;
+            const bool v = idx < nh;
+            uint2 h = make_uint2(0u, 0u);
+            if (v) h = hits[idx];
+            const float inten = v ? a.dia.intens[base + h.x] : 0.0f;
+            const uint32_t slot = h.y;  // f * kWin + t
+            if (v) own[slot] = (uint8_t)lane;
+            __builtin_amdgcn_wave_barrier();
+            const bool dup = v && own[slot] != (uint8_t)lane;
+            if (ballot64(dup) == 0) {
+              if (v) xw[slot] = xw[slot] + inten;
+            } else {
+              uint64_t it2 = ballot64(v);
+              while (it2) {
+                const int j2 = __builtin_ctzll(it2);
+                it2 &= it2 - 1;
+                if (lane == j2) xw[slot] = xw[slot] + inten;
+                __builtin_amdgcn_wave_barrier();
+              }
+            }
+            __builtin_amdgcn_wave_barrier();
           }
-          const uint64_t hm = ballot64(hit);
-          if (hm == 0) continue;
-          const float inten = hit ? a.dia.intens[base + e] : 0.0f;
-          const uint32_t t = hit ? c - hc0 : 0u;
-          if (hit) own[f * kWin + t] = (uint16_t)lane;
-          __builtin_amdgcn_wave_barrier();
-          const bool dup = hit && own[f * kWin + t] != (uint16_t)lane;
-          if (ballot64(dup) == 0) {
-            if (hit) xw[f * kWin + t] = xw[f * kWin + t] + inten;
-          } else {
-            uint64_t it = hm;
-            while (it) {
-              const int j2 = __builtin_ctzll(it);
-              it &= it - 1;
-              if (lane == j2) xw[f * kWin + t] = xw[f * kWin + t] + inten;
-              __builtin_amdgcn_wave_barrier();
+          nh = 0;
+        };
+        for (uint32_t v0 = 0; v0 < NG; v0 += kWave) {
+          const uint32_t v = v0 + lane;
+          const uint32_t vmax = min(v0 + (uint32_t)kWave - 1, NG - 1);
+          const uint64_t lo_m = ballot64(lane < (int)nsel && ng > 0 && gp <= v0);
+          const uint64_t hi_m = ballot64(lane < (int)nsel && ng > 0 && gp <= vmax);
+          const uint32_t flo = 63u - (uint32_t)__builtin_clzll(lo_m);
+          const uint32_t fhi = 63u - (uint32_t)__builtin_clzll(hi_m);
+          uint32_t f = flo, fgp = rdlane(gp, flo), fgs = rdlane(gsr, flo);
+          uint32_t fmeta = rdlane(meta, flo), fb0 = rdlane(rb0, flo);
+          for (uint32_t ff = flo + 1; ff <= fhi; ++ff) {
+            const uint32_t pp = rdlane(gp, ff);
+            if (pp <= v) {
+              f = ff;
+              fgp = pp;
+              fgs = rdlane(gsr, ff);
+              fmeta = rdlane(meta, ff);
+              fb0 = rdlane(rb0, ff);
             }
           }
-          __builtin_amdgcn_wave_barrier();
+          uint4 raw = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+          uint32_t tg = 0, vm = 0;
+          if (v < NG) {
+            const uint32_t q = v - fgp;
+            tg = fgs + q;
+            raw = *reinterpret_cast<const uint4 *>(cw + 8 * (size_t)tg);
+            const uint32_t klo = (q == 0) ? (fmeta & 7u) : 0u;
+            const uint32_t khi = (q + 1 == (fmeta >> 6)) ? ((fmeta >> 3) & 7u) : 7u;
+            vm = (0xffu << klo) & (0xffu >> (7u - khi));
+          }
+          const uint32_t cyc[8] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu,
+                                   raw.y >> 16,     raw.z & 0xffffu, raw.z >> 16,
+                                   raw.w & 0xffffu, raw.w >> 16};
+          uint32_t hm = 0;
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            hm |= (cyc[k] >= wl && cyc[k] < wh ? 1u : 0u) << k;
+          hm &= vm;
+          bool at_c0 = false;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) at_c0 = at_c0 || (((hm >> k) & 1u) && cyc[k] == c0);
+          if (ballot64(at_c0) != 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              if (((hm >> k) & 1u) && cyc[k] == c0 &&
+                  skipped_duplicate(a.dia, o, base, fb0, 8 * tg + k - bo, c0))
+                hm &= ~(1u << k);
+            }
+          }
+          // list slots in (lane, k) order = entry order; lanes 0-31
+          // first, then 32-63 (at most 32 x 8 hits each)
+#pragma unroll
+          for (int hh = 0; hh < 2; ++hh) {
+            uint32_t pre, tot;
+            list_slots(hm, (lane >> 5) == hh, pre, tot);
+            if (tot == 0) continue;
+            if ((lane >> 5) == hh) {
+              uint32_t pos = nh + pre;
+#pragma unroll
+              for (int k = 0; k < 8; ++k) {
+                if ((hm >> k) & 1u) {
+                  hits[pos] = make_uint2(8 * tg + k - bo, f * kWin + (cyc[k] - hc0));
+                  ++pos;
+                }
+              }
+            }
+            nh += tot;
+            __builtin_amdgcn_wave_barrier();
+            if (nh > (uint32_t)(kBwdHitCap - 4 * kWave)) flush();
+          }
         }
+        if (nh) flush();
+        __builtin_amdgcn_wave_barrier();
       }
       __builtin_amdgcn_wave_barrier();
       // per fragment: Y at the apex, dY/dsigma, ln(Y+1)

@@ -171,7 +171,8 @@ def test_workload0_candidates_match_oracle(ads, run0):
     assert_tables_equal(got, want, "workload 0")
 
 
-@pytest.mark.parametrize("variant", ["decoys_uneven", "per_window_sigma", "ties", "skips"])
+@pytest.mark.parametrize("variant", ["decoys_uneven", "per_window_sigma", "ties", "skips",
+                                     "cross_bin_sums"])
 def test_small_variants_match_oracle(ads, variant):
     if variant == "decoys_uneven":
         spec = synthetic.config(2, n_cycles=600, n_windows=12, peaks_per_spectrum=800,
@@ -205,6 +206,37 @@ def test_small_variants_match_oracle(ads, variant):
         want = _oracle(gen)
         skipped = set(range(0, 700, 7)) | set(range(5, 700, 11)) | set(range(3, 700, 13))
         assert not skipped & set(got[0].astype(np.int64).tolist())
+    elif variant == "cross_bin_sums":
+        # every peak gets two copies at -6 and +6 ppm (other bins, same
+        # spectrum) with very different intensities: XIC cells sum three or
+        # more entries from different bins, whose order changes the bits
+        spec = synthetic.config(0, n_cycles=300, n_windows=4, peaks_per_spectrum=700,
+                                n_targets=500, half_window_min=40, half_window_max=40)
+        gen = synthetic.generate(spec)
+        dia = list(gen["dia"])
+        start, stop, mz, it = dia[3], dia[4], dia[7], dia[8]
+        new_mz, new_it, new_start, new_stop = [], [], [], []
+        pos = 0
+        for s0, s1 in zip(start, stop):
+            m = mz[s0:s1]
+            v = it[s0:s1]
+            mm = np.concatenate([m, (m * np.float32(1 - 6e-6)).astype(np.float32),
+                                 (m * np.float32(1 + 6e-6)).astype(np.float32)])
+            vv = np.concatenate([v, (v * np.float32(0.37)).astype(np.float32),
+                                 (v * np.float32(1.3e-3)).astype(np.float32)])
+            o = np.argsort(mm, kind="stable")
+            new_mz.append(mm[o])
+            new_it.append(vv[o])
+            new_start.append(pos)
+            pos += len(mm)
+            new_stop.append(pos)
+        dia[3] = np.array(new_start, np.int64)
+        dia[4] = np.array(new_stop, np.int64)
+        dia[7] = np.concatenate(new_mz).astype(np.float32)
+        dia[8] = np.concatenate(new_it).astype(np.float32)
+        gen = dict(gen, dia=tuple(dia))
+        got = _search(ads, gen)[3]
+        want = _oracle(gen)
     else:
         # coarse intensities and many duplicate (bin, cycle) entries
         spec = synthetic.config(0, n_cycles=300, n_windows=4, peaks_per_spectrum=3000,
