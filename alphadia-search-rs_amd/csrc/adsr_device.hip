@@ -1363,204 +1363,226 @@ __global__ void sstar_kernel(const float *cs, const uint32_t *cnt, uint32_t cc,
 
 constexpr int kBwdHitCap = 256;
 
-__global__ __launch_bounds__(kWave) void diff_backward_kernel(BwdArgs a) {
-  __shared__ __align__(16) float xw[kMaxSel * kWin];
-  __shared__ uint8_t own[kMaxSel * kWin];
-  __shared__ uint2 hits[kBwdHitCap];
-  __shared__ double part[kMaxSel];
+// Bin range [st, en) of one fragment m/z (src/quadrupole_observation/mod.rs:
+// 79-83, src/mz_index/mod.rs:95-127) from the bucket entries of its bounds.
+__device__ __forceinline__ void bin_range(const KDia &d, float mz, float tol,
+                                          uint32_t &st, uint32_t &en) {
+  const float delta = mz * tol * 1e-6f;
+  const float lo = mz - delta, hi = mz + delta;
+  const uint32_t n = d.n_grid;
+  st = !(lo > d.grid_first) ? 0u
+       : (lo > d.grid_last) ? n
+                            : lb_from(d.bucket2[bucket_key(d, lo)], lo);
+  const uint32_t ub = !(hi >= d.grid_first) ? 0u
+                      : (hi >= d.grid_last) ? n
+                                            : ub_from(d.bucket2[bucket_key(d, hi)], hi);
+  en = st < n ? max(st, ub) : st;
+}
+
+// Per-wave LDS of the backward: XIC window per selected fragment, its
+// duplicate detector, the hit list and the per-fragment sigma terms.
+__host__ __device__ inline uint32_t bwd_lds_bytes(uint32_t ks) {
+  return (ks * kWin * 4 + ks * kWin + kBwdHitCap * 8 + ks * 8 + 15) / 16 * 16;
+}
+
+// One wave per precursor (launched over all rows; rows without a candidate
+// or with a zero loss gradient leave at once).
+__global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
+  extern __shared__ __align__(16) char bsm[];
   const int lane = lane_id();
-  const uint32_t half = a.prm.half;
   const uint32_t ks = a.plan.ks;
-  for (;;) {
-    uint32_t row = 0;
-    if (lane == 0) row = atomicAdd(a.queue, 1u);
-    row = uni(__shfl(row, 0, kWave));
-    if (row >= a.n_rows) break;
-    const uint4 rec = a.plan.rec[row];
-    const uint32_t c0 = uni(rec.x), c1 = uni(rec.y);
-    const uint32_t nsel = uni(rec.z & 0xffffu), nobs = uni(rec.z >> 16);
-    // every value that steers control flow is read as a wave-uniform
-    // scalar: a skip decided on per-lane loads is a divergent `continue`
-    const float g = __builtin_bit_cast(
-        float, uni(__builtin_bit_cast(uint32_t, a.grad_s[row])));
-    const uint32_t ncand = uni(a.cand_count[row]);
-    // No lane-0-only code and no `continue` in this loop besides the queue
-    // grab: the contribution is written by every lane (same value, same
-    // address), so all 64 lanes stay together around the back edge.
-    double total = 0.0;
-    if (!(ncand == 0 || g == 0.0f || c1 <= c0 || nobs == 0 || nsel == 0)) {
-      const uint32_t istar = uni(a.cand_center[(size_t)row * a.cc]) - c0;
-      // cycles [hc0, hc1) of the apex window; x[t] = X[hc0 + t]
-      const uint32_t hc0 = c0 + istar - half;
-      const uint32_t wl = max(hc0, c0), wh = min(hc0 + 2 * half + 1, c1);
-      for (uint32_t i = lane; i < nsel * kWin; i += kWave) xw[i] = 0.0f;
-      __builtin_amdgcn_wave_barrier();
-      // the XIC window, summed in the forward pass's order: windows, then
-      // fragments' CSR entries in order (8-entry groups, 64 per tile)
-      for (uint32_t j = 0; j < nobs; ++j) {
-        const uint32_t o = uni(a.plan.obs[(size_t)row * kObsMax + j]);
-        const uint64_t base = a.dia.entry_base[o];
-        const size_t ridx = ((size_t)row * kObsMax + j) * ks;
-        uint32_t rx = 0, ry = 0, rb0 = 0;
-        if (lane < (int)nsel) {
-          const uint2 r = a.plan.rng[ridx + lane];
-          rx = r.x;
-          ry = r.y;
-          rb0 = a.plan.b0[ridx + lane];
-        }
-        const uint32_t bo = (uint32_t)base & 7u;
-        const uint16_t *cw = a.dia.cycles + (base - bo);
-        const uint32_t gsr = (bo + rx) >> 3;
-        const uint32_t ng = ry > rx ? ((bo + ry - 1) >> 3) - gsr + 1 : 0u;
-        const uint32_t meta = ((bo + rx) & 7u) | (((bo + ry - 1) & 7u) << 3) | (ng << 6);
-        const uint32_t gincl = wave_incl_scan(ng);
-        const uint32_t gp = gincl - ng;
-        const uint32_t NG = rdlane(gincl, nsel - 1);
-        uint32_t nh = 0;
-        auto flush = [&]() {
-          for (uint32_t c = 0; c < nh; c += kWave) {
-            const uint32_t idx = c + lane;
-            const bool v = idx < nh;
-            uint2 h = make_uint2(0u, 0u);
-            if (v) h = hits[idx];
-            const float inten = v ? a.dia.intens[base + h.x] : 0.0f;
-            const uint32_t slot = h.y;  // f * kWin + t
-            if (v) own[slot] = (uint8_t)lane;
+  const uint32_t row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  char *base_lds = bsm + (size_t)(threadIdx.x / kWave) * bwd_lds_bytes(ks);
+  float *xw = reinterpret_cast<float *>(base_lds);
+  double *part = reinterpret_cast<double *>(base_lds + ks * kWin * 4);
+  uint2 *hits = reinterpret_cast<uint2 *>(base_lds + ks * kWin * 4 + ks * 8);
+  uint8_t *own = reinterpret_cast<uint8_t *>(base_lds + ks * kWin * 4 + ks * 8 +
+                                             kBwdHitCap * 8);
+  if (row >= a.n_rows) return;  // whole wave
+  const uint32_t half = a.prm.half;
+  const uint4 rec = a.plan.rec[row];
+  const uint32_t c0 = uni(rec.x), c1 = uni(rec.y);
+  const uint32_t nsel = uni(rec.z & 0xffffu), nobs = uni(rec.z >> 16);
+  const float g = __builtin_bit_cast(float, uni(__builtin_bit_cast(uint32_t, a.grad_s[row])));
+  const uint32_t ncand = uni(a.cand_count[row]);
+  if (ncand == 0 || g == 0.0f || c1 <= c0 || nobs == 0 || nsel == 0) {
+    if (lane == 0) a.contrib[row] = 0.0;
+    return;
+  }
+  const uint32_t istar = uni(a.cand_center[(size_t)row * a.cc]) - c0;
+  // cycles [hc0, hc1) of the apex window; x[t] = X[hc0 + t]
+  const uint32_t hc0 = c0 + istar - half;
+  const uint32_t wl = max(hc0, c0), wh = min(hc0 + 2 * half + 1, c1);
+  // bin range of fragment `lane`
+  uint32_t bst = 0, ben = 0;
+  if (lane < (int)nsel)
+    bin_range(a.dia, a.plan.sel_mz[(size_t)row * ks + lane], a.prm.mass_tol, bst, ben);
+  for (uint32_t i = lane; i < nsel * kWin; i += kWave) xw[i] = 0.0f;
+  __builtin_amdgcn_wave_barrier();
+  // the XIC window, summed in the forward pass's order: windows, then the
+  // fragments' CSR entries in order (8-entry groups, 64 per tile)
+  for (uint32_t j = 0; j < nobs; ++j) {
+    const uint32_t o = uni(a.plan.obs[(size_t)row * kObsMax + j]);
+    const uint64_t base = a.dia.entry_base[o];
+    uint32_t rx = 0, ry = 0;
+    if (lane < (int)nsel) {
+      const uint32_t *ssw = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
+      rx = ssw[bst];
+      ry = ssw[ben];
+    }
+    const uint32_t bo = (uint32_t)base & 7u;
+    const uint16_t *cw = a.dia.cycles + (base - bo);
+    const uint32_t gsr = (bo + rx) >> 3;
+    const uint32_t ng = ry > rx ? ((bo + ry - 1) >> 3) - gsr + 1 : 0u;
+    const uint32_t meta = ((bo + rx) & 7u) | (((bo + ry - 1) & 7u) << 3) | (ng << 6);
+    const uint32_t gincl = wave_incl_scan(ng);
+    const uint32_t gp = gincl - ng;
+    const uint32_t NG = rdlane(gincl, nsel - 1);
+    uint32_t nh = 0;
+    auto flush = [&]() {
+      for (uint32_t c = 0; c < nh; c += kWave) {
+        const uint32_t idx = c + lane;
+        const bool v = idx < nh;
+        uint2 h = make_uint2(0u, 0u);
+        if (v) h = hits[idx];
+        const float inten = v ? a.dia.intens[base + h.x] : 0.0f;
+        const uint32_t slot = h.y;  // f * kWin + t
+        if (v) own[slot] = (uint8_t)lane;
+        __builtin_amdgcn_wave_barrier();
+        const bool dup = v && own[slot] != (uint8_t)lane;
+        if (ballot64(dup) == 0) {
+          if (v) xw[slot] = xw[slot] + inten;
+        } else {
+          uint64_t it2 = ballot64(v);
+          while (it2) {
+            const int j2 = __builtin_ctzll(it2);
+            it2 &= it2 - 1;
+            if (lane == j2) xw[slot] = xw[slot] + inten;
             __builtin_amdgcn_wave_barrier();
-            const bool dup = v && own[slot] != (uint8_t)lane;
-            if (ballot64(dup) == 0) {
-              if (v) xw[slot] = xw[slot] + inten;
-            } else {
-              uint64_t it2 = ballot64(v);
-              while (it2) {
-                const int j2 = __builtin_ctzll(it2);
-                it2 &= it2 - 1;
-                if (lane == j2) xw[slot] = xw[slot] + inten;
-                __builtin_amdgcn_wave_barrier();
-              }
-            }
-            __builtin_amdgcn_wave_barrier();
-          }
-          nh = 0;
-        };
-        for (uint32_t v0 = 0; v0 < NG; v0 += kWave) {
-          const uint32_t v = v0 + lane;
-          const uint32_t vmax = min(v0 + (uint32_t)kWave - 1, NG - 1);
-          const uint64_t lo_m = ballot64(lane < (int)nsel && ng > 0 && gp <= v0);
-          const uint64_t hi_m = ballot64(lane < (int)nsel && ng > 0 && gp <= vmax);
-          const uint32_t flo = 63u - (uint32_t)__builtin_clzll(lo_m);
-          const uint32_t fhi = 63u - (uint32_t)__builtin_clzll(hi_m);
-          uint32_t f = flo, fgp = rdlane(gp, flo), fgs = rdlane(gsr, flo);
-          uint32_t fmeta = rdlane(meta, flo), fb0 = rdlane(rb0, flo);
-          for (uint32_t ff = flo + 1; ff <= fhi; ++ff) {
-            const uint32_t pp = rdlane(gp, ff);
-            if (pp <= v) {
-              f = ff;
-              fgp = pp;
-              fgs = rdlane(gsr, ff);
-              fmeta = rdlane(meta, ff);
-              fb0 = rdlane(rb0, ff);
-            }
-          }
-          uint4 raw = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
-          uint32_t tg = 0, vm = 0;
-          if (v < NG) {
-            const uint32_t q = v - fgp;
-            tg = fgs + q;
-            raw = *reinterpret_cast<const uint4 *>(cw + 8 * (size_t)tg);
-            const uint32_t klo = (q == 0) ? (fmeta & 7u) : 0u;
-            const uint32_t khi = (q + 1 == (fmeta >> 6)) ? ((fmeta >> 3) & 7u) : 7u;
-            vm = (0xffu << klo) & (0xffu >> (7u - khi));
-          }
-          const uint32_t cyc[8] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu,
-                                   raw.y >> 16,     raw.z & 0xffffu, raw.z >> 16,
-                                   raw.w & 0xffffu, raw.w >> 16};
-          uint32_t hm = 0;
-#pragma unroll
-          for (int k = 0; k < 8; ++k)
-            hm |= (cyc[k] >= wl && cyc[k] < wh ? 1u : 0u) << k;
-          hm &= vm;
-          bool at_c0 = false;
-#pragma unroll
-          for (int k = 0; k < 8; ++k) at_c0 = at_c0 || (((hm >> k) & 1u) && cyc[k] == c0);
-          if (ballot64(at_c0) != 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-              if (((hm >> k) & 1u) && cyc[k] == c0 &&
-                  skipped_duplicate(a.dia, o, base, fb0, 8 * tg + k - bo, c0))
-                hm &= ~(1u << k);
-            }
-          }
-          // list slots in (lane, k) order = entry order; lanes 0-31
-          // first, then 32-63 (at most 32 x 8 hits each)
-#pragma unroll
-          for (int hh = 0; hh < 2; ++hh) {
-            uint32_t pre, tot;
-            list_slots(hm, (lane >> 5) == hh, pre, tot);
-            if (tot == 0) continue;
-            if ((lane >> 5) == hh) {
-              uint32_t pos = nh + pre;
-#pragma unroll
-              for (int k = 0; k < 8; ++k) {
-                if ((hm >> k) & 1u) {
-                  hits[pos] = make_uint2(8 * tg + k - bo, f * kWin + (cyc[k] - hc0));
-                  ++pos;
-                }
-              }
-            }
-            nh += tot;
-            __builtin_amdgcn_wave_barrier();
-            if (nh > (uint32_t)(kBwdHitCap - 4 * kWave)) flush();
           }
         }
-        if (nh) flush();
         __builtin_amdgcn_wave_barrier();
       }
-      __builtin_amdgcn_wave_barrier();
-      // per fragment: Y at the apex, dY/dsigma, ln(Y+1)
-      double cf = 0.0;
-      if (lane < (int)nsel) {
-        const float *x = xw + lane * kWin + half;  // x[0] = apex cycle
-        const float *kk = a.prm.kernels + (size_t)rec.w * a.prm.klen;
-        const double *dk = a.dk + (size_t)rec.w * (half + 1);
-        float y = 0.0f;
-        y = y + x[0] * kk[half];
-        double dy = (double)x[0] * dk[half];
-        for (uint32_t jj = 0; jj < half; ++jj) {
-          const uint32_t d = half - jj;
-          y = y + (x[-(int)d] + x[d]) * kk[jj];
-          dy += ((double)x[-(int)d] + (double)x[d]) * dk[jj];
+      nh = 0;
+    };
+    for (uint32_t v0 = 0; v0 < NG; v0 += kWave) {
+      const uint32_t v = v0 + lane;
+      const uint32_t vmax = min(v0 + (uint32_t)kWave - 1, NG - 1);
+      const uint64_t lo_m = ballot64(lane < (int)nsel && ng > 0 && gp <= v0);
+      const uint64_t hi_m = ballot64(lane < (int)nsel && ng > 0 && gp <= vmax);
+      const uint32_t flo = 63u - (uint32_t)__builtin_clzll(lo_m);
+      const uint32_t fhi = 63u - (uint32_t)__builtin_clzll(hi_m);
+      uint32_t f = flo, fgp = rdlane(gp, flo), fgs = rdlane(gsr, flo);
+      uint32_t fmeta = rdlane(meta, flo), fb0 = rdlane(bst, flo);
+      for (uint32_t ff = flo + 1; ff <= fhi; ++ff) {
+        const uint32_t pp = rdlane(gp, ff);
+        if (pp <= v) {
+          f = ff;
+          fgp = pp;
+          fgs = rdlane(gsr, ff);
+          fmeta = rdlane(meta, ff);
+          fb0 = rdlane(bst, ff);
         }
-        // d s*/d w_f = ln(1 + Y_f), evaluated as log1p in f64: the f32
-        // logf(Y + 1) of the forward loses the low bits of Y when Y << 1
-        const double lnv = log1p((double)y);
-        const uint32_t sel = a.plan.sel[(size_t)row * ks + lane];
-        const uint64_t fs = a.lib.f_start[row];
-        a.grad_w[fs + sel] = (float)((double)g * lnv);
-        const double wf = (double)a.plan.sel_w[(size_t)row * ks + lane];
-        cf = wf * dy / ((double)y + 1.0);
-        part[lane] = cf;
       }
-      __builtin_amdgcn_wave_barrier();
-      double acc = 0.0;  // fixed fragment order, identical in every lane
-      for (uint32_t f = 0; f < nsel; ++f) acc += part[f];
-      total = (double)g * acc;
-      __builtin_amdgcn_wave_barrier();
+      uint4 raw = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+      uint32_t tg = 0, vm = 0;
+      if (v < NG) {
+        const uint32_t q = v - fgp;
+        tg = fgs + q;
+        raw = *reinterpret_cast<const uint4 *>(cw + 8 * (size_t)tg);
+        const uint32_t klo = (q == 0) ? (fmeta & 7u) : 0u;
+        const uint32_t khi = (q + 1 == (fmeta >> 6)) ? ((fmeta >> 3) & 7u) : 7u;
+        vm = (0xffu << klo) & (0xffu >> (7u - khi));
+      }
+      const uint32_t cyc[8] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu,
+                               raw.y >> 16,     raw.z & 0xffffu, raw.z >> 16,
+                               raw.w & 0xffffu, raw.w >> 16};
+      uint32_t hm = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) hm |= (cyc[k] >= wl && cyc[k] < wh ? 1u : 0u) << k;
+      hm &= vm;
+      bool at_c0 = false;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) at_c0 = at_c0 || (((hm >> k) & 1u) && cyc[k] == c0);
+      if (ballot64(at_c0) != 0) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (((hm >> k) & 1u) && cyc[k] == c0 &&
+              skipped_duplicate(a.dia, o, base, fb0, 8 * tg + k - bo, c0))
+            hm &= ~(1u << k);
+        }
+      }
+      // list slots in (lane, k) order = entry order; lanes 0-31 first,
+      // then 32-63 (at most 32 x 8 hits each)
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        uint32_t pre, tot;
+        list_slots(hm, (lane >> 5) == hh, pre, tot);
+        if (tot == 0) continue;
+        if ((lane >> 5) == hh) {
+          uint32_t pos = nh + pre;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            if ((hm >> k) & 1u) {
+              hits[pos] = make_uint2(8 * tg + k - bo, f * kWin + (cyc[k] - hc0));
+              ++pos;
+            }
+          }
+        }
+        nh += tot;
+        __builtin_amdgcn_wave_barrier();
+        if (nh > (uint32_t)(kBwdHitCap - 4 * kWave)) flush();
+      }
     }
-    a.contrib[row] = total;
+    if (nh) flush();
+    __builtin_amdgcn_wave_barrier();
+  }
+  __builtin_amdgcn_wave_barrier();
+  // per fragment: Y at the apex, dY/dsigma, ln(Y+1)
+  if (lane < (int)nsel) {
+    const float *x = xw + lane * kWin + half;  // x[0] = apex cycle
+    const float *kk = a.prm.kernels + (size_t)rec.w * a.prm.klen;
+    const double *dk = a.dk + (size_t)rec.w * (half + 1);
+    float y = 0.0f;
+    y = y + x[0] * kk[half];
+    double dy = (double)x[0] * dk[half];
+    for (uint32_t jj = 0; jj < half; ++jj) {
+      const uint32_t d = half - jj;
+      y = y + (x[-(int)d] + x[d]) * kk[jj];
+      dy += ((double)x[-(int)d] + (double)x[d]) * dk[jj];
+    }
+    // d s*/d w_f = ln(1 + Y_f), evaluated as log1p in f64: the f32
+    // logf(Y + 1) of the forward loses the low bits of Y when Y << 1
+    const double lnv = log1p((double)y);
+    const uint32_t sel = a.plan.sel[(size_t)row * ks + lane];
+    const uint64_t fs = a.lib.f_start[row];
+    a.grad_w[fs + sel] = (float)((double)g * lnv);
+    const double wf = (double)a.plan.sel_w[(size_t)row * ks + lane];
+    part[lane] = wf * dy / ((double)y + 1.0);
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) {
+    double acc = 0.0;  // fixed fragment order
+    for (uint32_t f = 0; f < nsel; ++f) acc += part[f];
+    a.contrib[row] = (double)g * acc;
   }
 }
 
 // grad_sigma[w] = sum of contrib over the rows whose kernel is window w's,
-// in a fixed order (per-thread strided sums, then a fixed tree).
-__global__ __launch_bounds__(256) void sigma_reduce_kernel(const uint4 *rec,
-                                                           const double *contrib,
-                                                           uint32_t n,
-                                                           double *out) {
+// in a fixed order: block (c, w) sums rows [c * kSigmaChunk, ...) with
+// per-thread strided sums and a fixed tree; the chunks are then added in
+// chunk order.  Deterministic for a given row count.
+constexpr uint32_t kSigmaChunk = 8192;
+
+__global__ __launch_bounds__(256) void sigma_partial_kernel(const uint4 *rec,
+                                                            const double *contrib,
+                                                            uint32_t n,
+                                                            double *part) {
   __shared__ double red[256];
-  const uint32_t w = blockIdx.x;
+  const uint32_t c = blockIdx.x, w = blockIdx.y;
+  const uint32_t r0 = c * kSigmaChunk, r1 = min(r0 + kSigmaChunk, n);
   double acc = 0.0;
-  for (uint32_t r = threadIdx.x; r < n; r += 256)
+  for (uint32_t r = r0 + threadIdx.x; r < r1; r += 256)
     if (rec[r].w == w + 1) acc += contrib[r];
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -1568,7 +1590,17 @@ __global__ __launch_bounds__(256) void sigma_reduce_kernel(const uint4 *rec,
     if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[w] = red[0];
+  if (threadIdx.x == 0) part[(size_t)w * gridDim.x + c] = red[0];
+}
+
+__global__ __launch_bounds__(64) void sigma_final_kernel(const double *part,
+                                                         uint32_t nch,
+                                                         double *out) {
+  if (threadIdx.x != 0) return;
+  const uint32_t w = blockIdx.x;
+  double acc = 0.0;
+  for (uint32_t c = 0; c < nch; ++c) acc += part[(size_t)w * nch + c];
+  out[w] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -2627,6 +2659,7 @@ struct DiffState {
   double *dk = nullptr;
   double *contrib = nullptr;
   double *gsig = nullptr;
+  double *gpart = nullptr;  // per-window, per-chunk sigma partial sums
   uint32_t *bq = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float fwd_ms = 0.0f, bwd_ms = 0.0f;
@@ -2694,6 +2727,7 @@ adsr_status adsr_diff_create(const adsr_dia *dc, const adsr_lib *lc,
     (void)hipFree(s->dk);
     (void)hipFree(s->contrib);
     (void)hipFree(s->gsig);
+    (void)hipFree(s->gpart);
     (void)hipFree(s->bq);
     delete s;
     return e;
@@ -2702,6 +2736,9 @@ adsr_status adsr_diff_create(const adsr_dia *dc, const adsr_lib *lc,
   if (hipMalloc(&s->dk, nk * (half + 1) * 8) != hipSuccess ||
       hipMalloc(&s->contrib, std::max<size_t>(l->n_prec, 1) * 8) != hipSuccess ||
       hipMalloc(&s->gsig, std::max<size_t>(d->n_obs, 1) * 8) != hipSuccess ||
+      hipMalloc(&s->gpart, std::max<size_t>(d->n_obs, 1) *
+                               ((l->n_prec + kSigmaChunk - 1) / kSigmaChunk + 1) * 8) !=
+          hipSuccess ||
       hipMalloc(&s->bq, 64) != hipSuccess)
     return fail(set_error(ADSR_ERR_OUT_OF_MEMORY, "diff buffers"));
   for (auto &e : s->ev)
@@ -2741,7 +2778,7 @@ adsr_status adsr_diff_forward(adsr_diff *h, const float *sigma,
   LaunchShape shape{4, 2};
   const uint32_t n = (uint32_t)l->n_prec;
   HIP_TRY(hipEventRecord(s->ev[0], s->w.stream));
-  adsr_status st = enqueue(s->w, s->sp, n, shape, nullptr, 0, 0, 0, false, false, true);
+  adsr_status st = enqueue(s->w, s->sp, n, shape, nullptr, 0, 0, 0, false);
   if (st) return st;
   if (n)
     sstar_kernel<<<(n + 255) / 256, 256, 0, s->w.stream>>>(
@@ -2771,7 +2808,6 @@ adsr_status adsr_diff_backward(adsr_diff *h, const float *grad_s,
   if (l->n_frag)
     fill_f32_kernel<<<(uint32_t)((l->n_frag + 255) / 256), 256, 0, st_>>>(
         grad_w, l->n_frag, 0.0f);
-  reset_queue_kernel<<<1, 64, 0, st_>>>(s->bq, s->bq + 1);
   if (n) {
     BwdArgs a;
     a.dia = s->sp.dia;
@@ -2786,15 +2822,16 @@ adsr_status adsr_diff_backward(adsr_diff *h, const float *grad_s,
     a.dk = s->dk;
     a.grad_w = grad_w;
     a.contrib = s->contrib;
-    a.queue = s->bq;
-    const int per_cu =
-        occupancy_blocks((const void *)diff_backward_kernel, kWave, 0);
-    const uint32_t blocks =
-        (uint32_t)std::min<uint64_t>((uint64_t)per_cu * s->sp.cus, n);
-    diff_backward_kernel<<<blocks, kWave, 0, st_>>>(a);
-    if (d->n_obs)
-      sigma_reduce_kernel<<<(uint32_t)d->n_obs, 256, 0, st_>>>(
-          s->sp.plan.rec, s->contrib, n, s->gsig);
+    a.queue = nullptr;
+    // four rows (waves) per block
+    const size_t lds = 4 * (size_t)bwd_lds_bytes(s->sp.plan.ks);
+    diff_backward_kernel<<<(n + 3) / 4, 256, lds, st_>>>(a);
+    if (d->n_obs) {
+      const uint32_t nch = (n + kSigmaChunk - 1) / kSigmaChunk;
+      sigma_partial_kernel<<<dim3(nch, (uint32_t)d->n_obs), 256, 0, st_>>>(
+          s->sp.plan.rec, s->contrib, n, s->gpart);
+      sigma_final_kernel<<<(uint32_t)d->n_obs, 64, 0, st_>>>(s->gpart, nch, s->gsig);
+    }
   }
   HIP_TRY(hipEventRecord(s->ev[3], st_));
   HIP_TRY(hipGetLastError());
@@ -2822,6 +2859,7 @@ void adsr_diff_free(adsr_diff *h) {
   (void)hipFree(s->dk);
   (void)hipFree(s->contrib);
   (void)hipFree(s->gsig);
+  (void)hipFree(s->gpart);
   (void)hipFree(s->bq);
   for (auto &e : s->ev)
     if (e) (void)hipEventDestroy(e);
