@@ -391,6 +391,143 @@ __global__ __launch_bounds__(kWave) void plan_kernel(KDia dia, KLib lib,
 }
 
 // ---------------------------------------------------------------------------
+// plan with LP lanes per precursor row (LP = 16, 32 or 64; every raw
+// fragment list of the library has at most LP entries): the same RT
+// window, valid windows and fragment filter/order as plan_kernel, one
+// fragment per lane, so the O(n^2) orderings cost O(n) per lane.
+// ---------------------------------------------------------------------------
+template <int LP>
+__global__ __launch_bounds__(kWave) void plan_group_kernel(KDia dia, KLib lib,
+                                                           KParams prm, KPlan plan,
+                                                           uint32_t *status) {
+  __shared__ float g_mz[kWave], g_I[kWave];
+  __shared__ uint32_t g_key[kWave];
+  const int lane = lane_id();
+  const int gl = lane % LP;           // lane within the row's group
+  const int gb = lane - gl;           // first lane of the group
+  const uint64_t gmask = (LP == 64) ? ~0ull : (((1ull << LP) - 1) << gb);
+  const uint32_t row = blockIdx.x * (kWave / LP) + (uint32_t)(lane / LP);
+  const bool live = row < lib.n_prec;
+  uint32_t st = 0;
+  // src/peak_group_selection/mod.rs:139-148 -> rt_index/mod.rs:89-138
+  uint32_t c0 = 0, c1 = 0;
+  if (live) {
+    const float rt_tol = prm.rt_tol_pp ? prm.rt_tol_pp[row] : prm.rt_tol;
+    const float prt = lib.p_rt[row];
+    const uint32_t n = dia.n_rt;
+    if (n > 0) {
+      const float min_tol = (10.0f + prm.pad_cycles) * dia.delta_t;
+      const float eff = fmaxf(rt_tol, min_tol);
+      const float lo = prt - eff, hi = prt + eff;
+      if (hi < dia.rt[0]) {
+        c0 = c1 = 0;
+      } else if (lo > dia.rt[n - 1]) {
+        c0 = c1 = n;
+      } else {
+        c0 = (lo <= dia.rt[0]) ? 0u : rust_bsearch_f32(dia.rt, n, lo);
+        c1 = (hi >= dia.rt[n - 1]) ? n : rust_bsearch_f32(dia.rt, n, hi);
+      }
+    }
+  }
+  // src/dia_data/mod.rs:79-87 valid observations (inclusive, ascending)
+  const float pmz = live ? lib.p_mz[row] : 0.0f;
+  uint32_t nobs = 0, first = 0xffffffffu;
+  for (uint32_t w0 = 0; w0 < dia.n_obs; w0 += LP) {
+    const uint32_t w = w0 + (uint32_t)gl;
+    const bool ok = live && w < dia.n_obs && dia.iso_lo[w] <= pmz && dia.iso_hi[w] >= pmz;
+    const uint64_t m = ballot64(ok) & gmask;
+    if (ok) {
+      const uint32_t rank = nobs + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1));
+      if (rank < (uint32_t)kObsMax) plan.obs[(size_t)row * kObsMax + rank] = (uint16_t)w;
+    }
+    if (m && first == 0xffffffffu) first = w0 + (uint32_t)(__builtin_ctzll(m) - gb);
+    nobs += (uint32_t)__builtin_popcountll(m);
+  }
+  if (nobs > (uint32_t)kObsMax) st |= kStTooManyWindows;
+  // src/speclib_flat/mod.rs:388-456 get_precursor_filtered(i, true, true, k)
+  const uint64_t fs = live ? lib.f_start[row] : 0;
+  const uint32_t nraw = live ? (uint32_t)(lib.f_stop[row] - fs) : 0u;
+  const bool mine = gl < (int)nraw;
+  float I = -__builtin_inff(), mz = 0.0f;
+  if (mine) {
+    const float v = lib.f_int[fs + gl];
+    const bool keep = v > 0.0f && !(lib.f_type[fs + gl] == 121 && lib.f_num[fs + gl] == 1);
+    I = keep ? v : -__builtin_inff();
+    mz = lib.f_mz[fs + gl];
+  }
+  g_I[lane] = I;
+  g_mz[lane] = mz;
+  const bool kept = I > 0.0f;
+  const uint64_t km = ballot64(kept) & gmask;
+  const uint32_t nkept = (uint32_t)__builtin_popcountll(km);
+  const uint32_t pos_kept = (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1));
+  const uint32_t k = min(prm.top_k, nkept);
+  uint32_t nsel = 0;
+  if (k > 0 && (k > (uint32_t)kMaxSel || k > plan.ks)) {
+    st |= kStTooManyFragments;
+  } else if (k > 0) {
+    nsel = k;
+  }
+  const int mode = (k >= nkept) ? 0 : (k == nkept - 1 ? 1 : 2);
+  // mode 1: the first fragment of minimum intensity is dropped and the
+  // last kept one takes its place
+  uint32_t drop = 0xffffffffu, drop_pos = 0;
+  const uint32_t last = km ? (uint32_t)(63 - __builtin_clzll(km) - gb) : 0u;
+  {
+    float mv = kept ? I : __builtin_inff();
+    uint32_t mi = kept ? (uint32_t)gl : 0xffffffffu;
+#pragma unroll
+    for (int o = LP / 2; o >= 1; o >>= 1) {
+      const float ov = __shfl_xor(mv, o, kWave);
+      const uint32_t oi = __shfl_xor(mi, o, kWave);
+      if (oi != 0xffffffffu && (mi == 0xffffffffu || ov < mv || (ov == mv && oi < mi))) {
+        mv = ov;
+        mi = oi;
+      }
+    }
+    drop = mi;
+    drop_pos = (uint32_t)__shfl((int)pos_kept, (int)(gb + (mi & (LP - 1))), kWave);
+  }
+  __builtin_amdgcn_wave_barrier();
+  uint32_t key = 0xffffffffu;
+  if (kept && nsel > 0) {
+    if (mode == 2) {
+      uint32_t r = 0;
+      for (uint32_t i = 0; i < nraw; ++i) {
+        const float u = g_I[gb + i];
+        if (u > I || (u == I && i < (uint32_t)gl)) ++r;
+      }
+      key = r < k ? r : 0xffffffffu;
+    } else if (mode == 1) {
+      key = ((uint32_t)gl == drop) ? 0xffffffffu
+            : (((uint32_t)gl == last && last != drop) ? drop_pos : pos_kept);
+    } else {
+      key = pos_kept;
+    }
+  }
+  g_key[lane] = key;
+  __builtin_amdgcn_wave_barrier();
+  if (key != 0xffffffffu) {
+    uint32_t pos = 0;
+    for (uint32_t i = 0; i < nraw; ++i) {
+      const uint32_t ki = g_key[gb + i];
+      if (i == (uint32_t)gl || ki == 0xffffffffu) continue;
+      const float mi = g_mz[gb + i];
+      if (mi < mz || (!(mi > mz) && !(mi < mz) && ki < key)) ++pos;
+    }
+    plan.sel[(size_t)row * plan.ks + pos] = (uint16_t)gl;
+    plan.sel_mz[(size_t)row * plan.ks + pos] = mz;
+    plan.sel_w[(size_t)row * plan.ks + pos] = lib.f_int[fs + gl];
+  }
+  if (live && gl == 0) {
+    const uint32_t kidx = (prm.per_window && first != 0xffffffffu) ? 1u + first : 0u;
+    plan.rec[row] = make_uint4(c0, c1, nsel | (min(nobs, (uint32_t)kObsMax) << 16), kidx);
+    if (c1 > c0 && c1 - c0 > (uint32_t)kLCapMax) st |= kStWindowTooLong;
+    if (st) atomicOr(status, st);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // bins: one thread per (row, selected-fragment slot)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void bins_kernel(KDia dia, KLib lib,
@@ -2093,17 +2230,21 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
   hipStream_t s = w.stream;
   uint32_t *queue = w.small, *status = w.small + 1, *total = w.small + 2;
   reset_queue_kernel<<<1, 64, 0, s>>>(queue, status);
+  // fragment lists of at most 64 entries: a lane group per row; longer
+  // ones: one thread per row with an LDS stash / global scratch
   auto launch_plan = [&]() {
-    const uint32_t nb = (n_rows + kWave - 1) / kWave;
     if (sp.max_raw <= 16)
-      plan_kernel<16><<<nb, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm, sp.plan,
-                                            w.key_scratch, status);
+      plan_group_kernel<16><<<(n_rows + 3) / 4, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm,
+                                                              sp.plan, status);
     else if (sp.max_raw <= 32)
-      plan_kernel<32><<<nb, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm, sp.plan,
-                                            w.key_scratch, status);
+      plan_group_kernel<32><<<(n_rows + 1) / 2, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm,
+                                                              sp.plan, status);
+    else if (sp.max_raw <= 64)
+      plan_group_kernel<64><<<n_rows, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm, sp.plan,
+                                                      status);
     else
-      plan_kernel<kStashFrag><<<nb, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm, sp.plan,
-                                                    w.key_scratch, status);
+      plan_kernel<kStashFrag><<<(n_rows + kWave - 1) / kWave, kWave, 0, s>>>(
+          sp.dia, sp.lib, sp.prm, sp.plan, w.key_scratch, status);
   };
   if (n_rows && plan_only) {
     launch_plan();

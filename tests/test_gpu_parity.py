@@ -172,7 +172,7 @@ def test_workload0_candidates_match_oracle(ads, run0):
 
 
 @pytest.mark.parametrize("variant", ["decoys_uneven", "per_window_sigma", "ties", "skips",
-                                     "cross_bin_sums"])
+                                     "cross_bin_sums", "top_k_drop_one", "long_fragment_lists"])
 def test_small_variants_match_oracle(ads, variant):
     if variant == "decoys_uneven":
         spec = synthetic.config(2, n_cycles=600, n_windows=12, peaks_per_spectrum=800,
@@ -206,6 +206,23 @@ def test_small_variants_match_oracle(ads, variant):
         want = _oracle(gen)
         skipped = set(range(0, 700, 7)) | set(range(5, 700, 11)) | set(range(3, 700, 13))
         assert not skipped & set(got[0].astype(np.int64).tolist())
+    elif variant == "top_k_drop_one":
+        # top_k = kept - 1: the reference drops the first minimum-intensity
+        # fragment and moves the last kept one into its place
+        spec = synthetic.config(0, n_cycles=300, n_windows=4, peaks_per_spectrum=800,
+                                n_targets=600, half_window_min=40, half_window_max=40)
+        gen = synthetic.generate(spec)
+        got = _search(ads, gen, top_k_fragments=11)[3]
+        want = _oracle(gen, top_k_fragments=11)
+    elif variant == "long_fragment_lists":
+        # 40-60 raw fragments per precursor (one lane group of 64 per row),
+        # top 16 by intensity
+        spec = synthetic.config(2, n_cycles=300, n_windows=6, peaks_per_spectrum=600,
+                                n_targets=400, frag_min=40, frag_max=60,
+                                half_window_min=30, half_window_max=60)
+        gen = synthetic.generate(spec)
+        got = _search(ads, gen, top_k_fragments=16)[3]
+        want = _oracle(gen, top_k_fragments=16)
     elif variant == "cross_bin_sums":
         # every peak gets two copies at -6 and +6 ppm (other bins, same
         # spectrum) with very different intensities: XIC cells sum three or
