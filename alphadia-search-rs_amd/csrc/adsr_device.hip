@@ -568,7 +568,7 @@ struct SelArgs {
   uint32_t lds_per_wave;  // bytes
   uint32_t lcap;          // max cycles per window (multiple of 64)
   uint32_t xcap;          // floats in the XIC tile
-  uint32_t gcap;          // fragment rows in the XIC tile
+  uint32_t gcap;          // fragment rows of the longest row (host side)
   float *cand_score;      // [row][cand_count]
   uint32_t *cand_center;  // [row][cand_count]
   uint32_t *cand_count;   // [row]
@@ -601,16 +601,21 @@ __host__ __device__ inline LdsShape lds_shape(uint32_t lcap) {
   return s;
 }
 
+// Bytes of a wave's LDS that do not depend on the number of fragment rows
+// (R with 4 + 8 pad floats, hits, items, kernel taps, weights), and the
+// bytes of one fragment row (X row, BD row, own row), for rows of up to
+// lrow cycles.
+__host__ __device__ inline uint32_t lds_fixed(uint32_t lrow) {
+  return (lrow + 12) * 4 + kHitCap * 8 + kItemCap * 2 + 32 * 4 + kMaxSel * 4;
+}
+
+__host__ __device__ inline uint32_t lds_per_frag(uint32_t lrow) {
+  const LdsShape s = lds_shape(lrow);
+  return s.lp * 4 + s.wb * 8 + lrow;
+}
+
 __host__ __device__ inline uint32_t lds_bytes(uint32_t lcap, uint32_t gcap) {
-  const LdsShape s = lds_shape(lcap);
-  uint32_t b = (lcap + 12) * 4          // R with 4 + 8 pad floats
-               + gcap * s.lp * 4        // X
-               + gcap * s.wb * 8        // BD
-               + kHitCap * 8            // hits
-               + kItemCap * 2           // items
-               + 32 * 4                 // kernel taps
-               + kMaxSel * 4            // w
-               + gcap * lcap;           // own
+  const uint32_t b = lds_fixed(lcap) + gcap * lds_per_frag(lcap);
   return (b + 15) / 16 * 16;
 }
 
@@ -623,18 +628,31 @@ struct WaveLds {
   uint8_t *own;
   float *w;
   float *kern;
+  uint32_t lrow;  // row capacity in cycles (multiple of 64, >= L)
+  uint32_t lp;    // XIC row stride (floats)
+  uint32_t wb;    // BD row stride (u64 words)
+  uint32_t gcap;  // fragment rows in the XIC tile
 };
 
-__device__ __forceinline__ WaveLds carve(char *base, const SelArgs &a) {
-  const LdsShape sh = lds_shape(a.lcap);
+// A wave's LDS for one row of L <= lcap cycles and nsel fragments, carved
+// from the row's own length: the XIC tile holds as many fragment rows as
+// `bytes` leaves, so rows with short RT windows take more fragments per
+// group.  `bytes` >= lds_bytes(lcap, 1) guarantees at least one.
+__device__ __forceinline__ WaveLds carve(char *base, uint32_t L, uint32_t nsel,
+                                         uint32_t bytes) {
   WaveLds l;
+  l.lrow = max(64u, (L + 63) / 64 * 64);
+  const LdsShape sh = lds_shape(l.lrow);
+  l.lp = sh.lp;
+  l.wb = sh.wb;
+  l.gcap = max(1u, min(nsel, (bytes - lds_fixed(l.lrow)) / lds_per_frag(l.lrow)));
   char *p = base;
   l.R = (float *)p + 4;
-  p += (a.lcap + 12) * 4;
+  p += (l.lrow + 12) * 4;
   l.X = (float *)p;
-  p += a.gcap * sh.lp * 4;
+  p += l.gcap * l.lp * 4;
   l.BD = (uint64_t *)p;
-  p += a.gcap * sh.wb * 8;
+  p += l.gcap * l.wb * 8;
   l.hits = (uint2 *)p;
   p += kHitCap * 8;
   l.items = (uint16_t *)p;
@@ -774,6 +792,9 @@ __device__ __forceinline__ void wave_argmax(float &bv, uint32_t &bi) {
 // Occupancy: at most 128 VGPRs (4 waves per SIMD) and, with the default
 // 10 KB of LDS per wave, 16 waves per CU; the kernel is latency-bound, and
 // measured faster there than with more registers or more LDS per wave.
+// static LDS of select_kernel per block (the logf table)
+constexpr uint32_t kSelectStaticLds = 16 * sizeof(LogfEntry);
+
 template <int WAVES, int HALF>
 __global__ __launch_bounds__(WAVES *kWave) __attribute__((amdgpu_waves_per_eu(4)))
 void select_kernel(SelArgs a) {
@@ -783,8 +804,7 @@ void select_kernel(SelArgs a) {
   __syncthreads();
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
-  const WaveLds l = carve(smem + (size_t)wv * a.lds_per_wave, a);
-  const LdsShape sh = lds_shape(a.lcap);
+  char *const wave_lds = smem + (size_t)wv * a.lds_per_wave;
   const uint32_t half = HALF ? (uint32_t)HALF : a.prm.half;
   const uint32_t cc = a.prm.cand_count;
   const uint32_t ks = a.plan.ks;
@@ -916,6 +936,7 @@ void select_kernel(SelArgs a) {
         }
         continue;
       }
+      const WaveLds l = carve(wave_lds, L, nsel, a.lds_per_wave);
       const uint32_t nw = (L + 63) / 64;  // 64-cycle words per row
       for (uint32_t i = 4 * lane; i < L; i += 4 * kWave)
         *reinterpret_cast<float4 *>(l.R + i) = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -959,11 +980,11 @@ void select_kernel(SelArgs a) {
           const uint32_t fl = v ? f - gA : 0u;
           float inten = 0.0f;
           if (v) inten = loaded ? __builtin_bit_cast(float, h.x) : a.dia.intens[base + h.x];
-          uint8_t *ow = l.own + fl * a.lcap + t;
+          uint8_t *ow = l.own + fl * l.lrow + t;
           if (v) *ow = (uint8_t)lane;
           __builtin_amdgcn_wave_barrier();
           const bool dup = v && *ow != (uint8_t)lane;
-          float *xr = l.X + fl * sh.lp + kXPad;
+          float *xr = l.X + fl * l.lp + kXPad;
           if (ballot64(dup) == 0) {
             if (v) xr[t] = xr[t] + inten;
           } else {
@@ -982,7 +1003,7 @@ void select_kernel(SelArgs a) {
             const int ihi = min((int)(t + half), (int)L - (int)half - 1);
             if (ilo <= ihi) {
               const uint32_t blo = (uint32_t)ilo >> 2, bhi = (uint32_t)ihi >> 2;
-              uint64_t *br = l.BD + fl * sh.wb;
+              uint64_t *br = l.BD + fl * l.wb;
               const uint32_t wlo = blo >> 6, whi = bhi >> 6;
               const uint64_t ones = ~0ull;
               if (wlo == whi) {
@@ -1150,10 +1171,10 @@ void select_kernel(SelArgs a) {
       };
       auto zero_group = [&](uint32_t G) {
         float4 *xz = reinterpret_cast<float4 *>(l.X);
-        const uint32_t n4 = G * sh.lp / 4;
+        const uint32_t n4 = G * l.lp / 4;
         for (uint32_t i = lane; i < n4; i += kWave)
           xz[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (uint32_t i = lane; i < G * sh.wb; i += kWave) l.BD[i] = 0ull;
+        for (uint32_t i = lane; i < G * l.wb; i += kWave) l.BD[i] = 0ull;
         __builtin_amdgcn_wave_barrier();
       };
       // the whole row's hits in one pass, their intensities in one batch
@@ -1185,8 +1206,8 @@ void select_kernel(SelArgs a) {
       }
 
       if (more) stage2();
-      for (uint32_t g0 = 0; g0 < nsel && nobs > 0; g0 += a.gcap) {
-        const uint32_t G = min(a.gcap, nsel - g0);
+      for (uint32_t g0 = 0; g0 < nsel && nobs > 0; g0 += l.gcap) {
+        const uint32_t G = min(l.gcap, nsel - g0);
         zero_group(G);
         gA = g0;
         gG = G;
@@ -1208,7 +1229,7 @@ void select_kernel(SelArgs a) {
             const bool v = idx < n;
             const uint32_t item = v ? l.items[idx] : l.items[c];
             const uint32_t f = item >> 10, i0 = (item & 0x3ffu) * 4;
-            const float *row = l.X + f * sh.lp + kXPad;
+            const float *row = l.X + f * l.lp + kXPad;
             float y[4];
             if (HALF == 10) conv4_h10(row + i0 - 12, k10, y);
             else conv4_any(row, i0, half, l.kern, y);
@@ -1253,7 +1274,7 @@ void select_kernel(SelArgs a) {
         };
         for (uint32_t f = 0; f < G; ++f) {
           for (uint32_t q = 0; q < nbw; ++q) {
-            const uint64_t bw = l.BD[f * sh.wb + q];
+            const uint64_t bw = l.BD[f * l.wb + q];
             const uint32_t blo = uni((uint32_t)bw), bhi = uni((uint32_t)(bw >> 32));
             if ((blo | bhi) == 0u) continue;
             const uint32_t before =
@@ -2113,6 +2134,8 @@ struct SearchPlan {
   KParams prm;
   KPlan plan;
   uint32_t lcap = 0, xcap = 0, gcap = 0, lds_per_wave = 0;
+  uint32_t lds_budget = 0;  // per-wave LDS share (incl. static) to fill
+  bool lds_grow = false;    // rows shorter than lcap can use more LDS
   int cus = 256;
   uint64_t max_raw = 0;  // longest raw fragment list of the library
   std::vector<float> kernels;
@@ -2191,6 +2214,11 @@ static adsr_status prepare(const adsr_dia *d, const adsr_lib *l,
   sp.gcap = g;
   sp.xcap = g * lds_shape(sp.lcap).lp;
   sp.lds_per_wave = lds_bytes(sp.lcap, g);
+  // rows shorter than lcap carve their own tile from the wave's share: when
+  // the longest row cannot take every fragment at once, each wave gets the
+  // whole budget at launch so that shorter rows take more fragments per group
+  sp.lds_budget = budget;
+  sp.lds_grow = g < ks;
   if (sp.lds_per_wave > kMaxDynLds)
     return set_error(ADSR_ERR_UNSUPPORTED,
                      "per-precursor LDS tile of %u bytes exceeds the device",
@@ -2265,7 +2293,6 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
     a.n_rows = n_rows;
     a.queue = queue;
     a.chunk = (uint32_t)shape.chunk;
-    a.lds_per_wave = sp.lds_per_wave;
     a.lcap = sp.lcap;
     a.xcap = sp.xcap;
     a.gcap = sp.gcap;
@@ -2280,7 +2307,15 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
     while (shape.waves > 1 &&
            (size_t)sp.lds_per_wave * shape.waves > kMaxDynLds)
       shape.waves >>= 1;
-    const size_t lds = (size_t)sp.lds_per_wave * shape.waves;
+    // the wave's share of the budget, less its part of the block's static
+    // LDS (logf table), so that the budget's occupancy is kept
+    a.lds_per_wave = sp.lds_per_wave;
+    if (sp.lds_grow) {
+      const uint32_t wv = (uint32_t)shape.waves;
+      const uint32_t fit = (sp.lds_budget * wv - kSelectStaticLds) / wv / 16 * 16;
+      if ((size_t)fit * wv <= kMaxDynLds) a.lds_per_wave = std::max(a.lds_per_wave, fit);
+    }
+    const size_t lds = (size_t)a.lds_per_wave * shape.waves;
     const bool h10 = sp.prm.half == 10;
     const void *fn =
         h10 ? (shape.waves == 1   ? (const void *)select_kernel<1, 10>
