@@ -115,8 +115,15 @@ class DIAData:
         peak_mz,
         peak_intensity,
         cycle,
+        device_build=None,
     ) -> "DIAData":
-        """src/dia_data/mod.rs:43-72."""
+        """src/dia_data/mod.rs:43-72.
+
+        ``device_build`` picks where the binned CSR is built: True on the
+        gfx950 device (the CSR then lives in HBM only), False on host
+        threads, None (default) on the device when one is present.  Both
+        give the same CSR, bit for bit.
+        """
         d = _arr("spectrum_delta_scan_idx", spectrum_delta_scan_idx, np.int64)
         lo = _arr("isolation_lower_mz", isolation_lower_mz, np.float32)
         hi = _arr("isolation_upper_mz", isolation_upper_mz, np.float32)
@@ -134,15 +141,24 @@ class DIAData:
         npk = _same_len(["peak_mz", "peak_intensity"], [mz, it])
         shape = (C.c_size_t * 4)(*cyc.shape)
         out = C.c_void_p()
-        check(N.lib.adsr_dia_create(ptr(d), ptr(lo), ptr(hi), ptr(ps), ptr(pe),
-                                    ptr(cy), ptr(rt), ns, ptr(mz), ptr(it), npk,
-                                    ptr(cyc), shape, C.byref(out)))
+        on_device = N.device_available() if device_build is None else bool(device_build)
+        create = N.lib.adsr_dia_create_device if on_device else N.lib.adsr_dia_create
+        check(create(ptr(d), ptr(lo), ptr(hi), ptr(ps), ptr(pe), ptr(cy), ptr(rt),
+                     ns, ptr(mz), ptr(it), npk, ptr(cyc), shape, C.byref(out)))
         obj = DIAData()
         obj._h = out.value
         obj._n_obs = N.lib.adsr_dia_num_observations(obj._h)
-        if N.device_available():
+        if not on_device and N.device_available():
             check(N.lib.adsr_dia_upload(obj._h))
         return obj
+
+    def build_timings(self) -> dict:
+        """Milliseconds of a device build: host preparation, peak copy to
+        the device, device kernels (all 0 for a host-built CSR)."""
+        v = [C.c_double(0.0) for _ in range(3)]
+        check(N.lib.adsr_dia_build_timings(self._h, *(C.byref(x) for x in v)))
+        return {"prepare_ms": v[0].value, "copy_ms": v[1].value,
+                "kernels_ms": v[2].value}
 
     @property
     def num_observations(self) -> int:

@@ -80,6 +80,10 @@ _sig("adsr_backend_current", I, C.c_char_p, c_size)
 _sig("adsr_backend_optimal", I, C.c_char_p, c_size)
 _sig("adsr_dia_create", I, vp, vp, vp, vp, vp, vp, vp, c_size, vp, vp, c_size,
      vp, vp, C.POINTER(vp))
+_sig("adsr_dia_create_device", I, vp, vp, vp, vp, vp, vp, vp, c_size, vp, vp,
+     c_size, vp, vp, C.POINTER(vp))
+_sig("adsr_dia_build_timings", I, vp, C.POINTER(C.c_double),
+     C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("adsr_dia_upload", I, vp)
 _sig("adsr_dia_free", None, vp)
 _sig("adsr_dia_num_observations", c_size, vp)

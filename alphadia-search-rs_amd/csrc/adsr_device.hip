@@ -1944,22 +1944,20 @@ void device_lib_free(DeviceLib &l) {
   l = DeviceLib();
 }
 
-static adsr_status upload_dia(adsr_dia *d) {
-  std::lock_guard<std::mutex> lk(d->mu);
-  if (d->dev.ready) return ADSR_OK;
-  adsr_status st = require_device();
-  if (st) return st;
+#define UP(dst, src, n, ...)                                              \
+  do {                                                                    \
+    adsr_status s_ = dmalloc_copy(&(dst), (src), (n), acc, ##__VA_ARGS__); \
+    if (s_) {                                                             \
+      device_dia_free(v);                                                 \
+      return s_;                                                          \
+    }                                                                     \
+  } while (0)
+
+// Grid tables, RT axis, window bounds and entry bases: everything of a
+// DIAData on the device except the CSR arrays.
+static adsr_status upload_dia_meta(adsr_dia *d, size_t &acc) {
   const Grid &g = global_grid();
   DeviceDia &v = d->dev;
-  size_t acc = 0;
-#define UP(dst, src, n, ...)                             \
-  do {                                                   \
-    adsr_status s_ = dmalloc_copy(&(dst), (src), (n), acc, ##__VA_ARGS__); \
-    if (s_) {                                            \
-      device_dia_free(v);                                \
-      return s_;                                         \
-    }                                                    \
-  } while (0)
   UP(v.grid, g.mz.data(), g.mz.size());
   UP(v.bucket, g.bucket.data(), g.bucket.size());
   {
@@ -1978,16 +1976,31 @@ static adsr_status upload_dia(adsr_dia *d) {
   UP(v.rt, d->rt_index.data(), d->rt_index.size());
   UP(v.iso_lo, d->iso_lo.data(), d->iso_lo.size());
   UP(v.iso_hi, d->iso_hi.data(), d->iso_hi.size());
-  UP(v.slice_starts, d->slice_starts.data(), d->slice_starts.size());
   UP(v.entry_base, d->entry_base.data(), d->entry_base.size());
+  return ADSR_OK;
+}
+
+static adsr_status upload_dia(adsr_dia *d) {
+  std::lock_guard<std::mutex> lk(d->mu);
+  if (d->dev.ready) return ADSR_OK;
+  adsr_status st = require_device();
+  if (st) return st;
+  if (!d->host_csr)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT,
+                     "device-built DIAData has no device copy");
+  DeviceDia &v = d->dev;
+  size_t acc = 0;
+  st = upload_dia_meta(d, acc);
+  if (st) return st;
+  UP(v.slice_starts, d->slice_starts.data(), d->slice_starts.size());
   // 8 spare entries: the select kernel reads cycles in aligned groups of 8
   UP(v.cycles, d->cycles.data(), d->cycles.size(), 8);
   UP(v.intens, d->intens.data(), d->intens.size());
-#undef UP
   v.bytes = acc;
   v.ready = true;
   return ADSR_OK;
 }
+#undef UP
 
 static KDia make_kdia(const adsr_dia *d) {
   const Grid &g = global_grid();
@@ -3172,3 +3185,590 @@ extern "C" adsr_status adsr_dense_xic(const adsr_dia *dc, float pmz,
 // Host instance of the device ln; used to check the device code path
 // against the platform libm on machines without a GPU.
 extern "C" float adsr_logf_host(float x) { return logf_exact(x, kHostLogfTab); }
+
+// ===========================================================================
+// DIAData CSR built in HBM: src/dia_data_builder/mod.rs:52-139
+// ===========================================================================
+// The reference pushes, per observation, one (bin, cycle, intensity) triple
+// per peak in spectrum order, stable-sorts the triples by (bin, cycle) and
+// turns the bins into slice_starts.  Here the host orders each
+// observation's spectra by (u16 cycle, spectrum index) -- the order a stable
+// sort by cycle gives -- and the device
+//   1. snaps every peak to its 1-ppm bin (find_closest_index) and writes
+//      (bin, cycle << 32 | intensity bits) in that order,
+//   2. stable-sorts each observation's entries by bin: three 8-bit LSD
+//      radix passes over 4096-entry tiles that never straddle two
+//      observations (tile histograms, one exclusive scan, stable ranking
+//      with wave ballots, scatter through LDS),
+//   3. marks the end of every bin's run and takes a per-observation prefix
+//      maximum, which is slice_starts.
+// The result equals the host builder's CSR bit for bit.
+
+struct SnapItem {
+  uint64_t src;    // first peak
+  uint64_t dst;    // first entry (global, observation-major)
+  uint32_t n;      // peaks
+  uint32_t cycle;  // u16 cycle index of the spectrum
+};
+constexpr uint32_t kSnapPeaks = 2048;  // peaks per snap item
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // entries per tile
+constexpr int kSortWaves = kSortThreads / kWave;
+constexpr int kSortRun = kSortTile / kSortWaves;  // entries per wave
+constexpr int kMaxScanChunk = 8192;               // prefix-max chunk
+
+// src/mz_index/mod.rs:55-90: nearest grid point, ties to the lower index,
+// exact grid values returned as themselves, NaN and values at or below the
+// first point to 0, values beyond the last point to n - 1.
+__device__ __forceinline__ uint32_t find_closest_bin(const KDia &dia, float x) {
+  const uint32_t n = dia.n_grid;
+  uint32_t lb;
+  if (!(x > dia.grid_first)) lb = 0;
+  else if (x > dia.grid_last) lb = n;
+  else lb = lb_from(dia.bucket2[bucket_key(dia, x)], x);
+  if (lb < n && dia.grid[lb] == x) return lb;
+  if (lb == 0) return 0;
+  if (lb == n) return n - 1;
+  const float left = fabsf(dia.grid[lb] - x);
+  const float right = fabsf(dia.grid[lb - 1] - x);
+  return left < right ? lb : lb - 1;
+}
+
+__global__ __launch_bounds__(256) void snap_kernel(KDia dia, const SnapItem *items,
+                                                   const float *mz, const float *inten,
+                                                   uint32_t *keys, uint64_t *vals) {
+  const SnapItem it = items[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < it.n; i += blockDim.x) {
+    const float x = mz[it.src + i];
+    const uint32_t bits = __float_as_uint(inten[it.src + i]);
+    keys[it.dst + i] = find_closest_bin(dia, x);
+    vals[it.dst + i] = ((uint64_t)it.cycle << 32) | bits;
+  }
+}
+
+struct SortGeo {
+  const uint64_t *ebase;   // n_obs + 1 entry bases
+  const uint32_t *tfirst;  // n_obs + 1 first tiles
+  uint32_t n_obs;
+};
+
+struct SortTile {
+  uint32_t o;      // observation
+  uint32_t tl;     // tile within the observation
+  uint32_t nt;     // tiles of the observation
+  uint32_t cnt;    // entries in this tile
+  uint32_t pos0;   // first entry of the tile, relative to the observation
+  uint64_t eb;     // entry base of the observation
+  uint64_t n_o;    // entries of the observation
+};
+
+// Observation o with tfirst[o] <= tile < tfirst[o + 1] (never an empty one).
+__device__ __forceinline__ SortTile sort_tile(const SortGeo &g, uint32_t tile) {
+  uint32_t lo = 0, hi = g.n_obs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (g.tfirst[mid] <= tile) lo = mid;
+    else hi = mid;
+  }
+  SortTile t;
+  t.o = lo;
+  t.tl = tile - g.tfirst[lo];
+  t.nt = g.tfirst[lo + 1] - g.tfirst[lo];
+  t.eb = g.ebase[lo];
+  t.n_o = g.ebase[lo + 1] - t.eb;
+  t.pos0 = t.tl * (uint32_t)kSortTile;
+  t.cnt = (uint32_t)min((uint64_t)kSortTile, t.n_o - t.pos0);
+  return t;
+}
+
+// Digit histogram of one tile, stored digit-major within the observation:
+// H[tfirst[o] * 256 + d * nt + tl], so that one exclusive scan over H gives
+// every (observation, digit, tile) its output offset.
+__global__ __launch_bounds__(kSortThreads) void radix_upsweep_kernel(
+    SortGeo g, const uint32_t *keys, uint32_t shift, uint32_t *H) {
+  __shared__ uint32_t hist[kSortWaves][256];
+  for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortThreads)
+    (&hist[0][0])[i] = 0;
+  __syncthreads();
+  const SortTile t = sort_tile(g, blockIdx.x);
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  uint32_t k[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t p = (uint32_t)(w * kSortRun + j * kWave + lane);
+    k[j] = p < t.cnt ? keys[t.eb + t.pos0 + p] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t p = (uint32_t)(w * kSortRun + j * kWave + lane);
+    if (p < t.cnt) atomicAdd(&hist[w][(k[j] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  const uint32_t d = threadIdx.x;
+  uint32_t c = 0;
+#pragma unroll
+  for (int q = 0; q < kSortWaves; ++q) c += hist[q][d];
+  H[(size_t)g.tfirst[t.o] * 256 + (size_t)d * t.nt + t.tl] = c;
+}
+
+// Stable scatter of one tile by digit.  Wave w ranks entries
+// [w * kSortRun, (w + 1) * kSortRun) of the tile 64 at a time: lanes with
+// equal digits find each other with 8 ballots, and a per-wave running count
+// per digit in LDS gives each entry its rank among the earlier entries of
+// its digit.  Entries are then placed in LDS in digit order and written out
+// in runs per digit.  The scan offsets are taken modulo 2^32 and made
+// relative to the observation, which holds fewer than 2^32 entries.
+template <bool FINAL>
+__global__ __launch_bounds__(kSortThreads) void radix_downsweep_kernel(
+    SortGeo g, const uint32_t *kin, const uint64_t *vin, uint32_t shift,
+    const uint32_t *hscan, const uint32_t *hbsum, uint32_t *kout,
+    uint64_t *vout, uint16_t *cout, float *iout) {
+  __shared__ uint32_t whist[kSortWaves][256];
+  __shared__ uint32_t goff[256];
+  __shared__ uint32_t wsum[kSortWaves];
+  __shared__ uint64_t stage[kSortTile];
+  for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortThreads)
+    (&whist[0][0])[i] = 0;
+  __syncthreads();
+  const SortTile t = sort_tile(g, blockIdx.x);
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  const uint64_t src0 = t.eb + t.pos0;
+  uint32_t key[kSortItems], rk[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t p = (uint32_t)(w * kSortRun + j * kWave + lane);
+    key[j] = p < t.cnt ? kin[src0 + p] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t p = (uint32_t)(w * kSortRun + j * kWave + lane);
+    const bool valid = p < t.cnt;
+    const uint32_t d = (key[j] >> shift) & 255u;
+    uint64_t m = ballot64(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = ballot64(bit);
+      m &= bit ? bb : ~bb;
+    }
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    uint32_t pre = 0;
+    if (valid) pre = whist[w][d];
+    if (valid && below == 0) whist[w][d] = pre + (uint32_t)__builtin_popcountll(m);
+    rk[j] = pre + below;
+  }
+  __syncthreads();
+  // thread d owns digit d: tile-local start of the digit, start of each
+  // wave's entries within it, and the digit's output offset
+  {
+    const uint32_t d = threadIdx.x;
+    uint32_t c[kSortWaves], tot = 0;
+#pragma unroll
+    for (int q = 0; q < kSortWaves; ++q) {
+      c[q] = whist[q][d];
+      tot += c[q];
+    }
+    const uint32_t incl = wave_incl_scan(tot);
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    uint32_t dstart = incl - tot;
+    for (int q = 0; q < w; ++q) dstart += wsum[q];
+    uint32_t run = dstart;
+#pragma unroll
+    for (int q = 0; q < kSortWaves; ++q) {
+      whist[q][d] = run;
+      run += c[q];
+    }
+    const size_t hidx = (size_t)g.tfirst[t.o] * 256 + (size_t)d * t.nt + t.tl;
+    const uint32_t rel = hscan[hidx] + hbsum[hidx / kScanBlock] - (uint32_t)t.eb;
+    goff[d] = rel - dstart;
+  }
+  __syncthreads();
+  uint32_t *skey = reinterpret_cast<uint32_t *>(stage);
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t p = (uint32_t)(w * kSortRun + j * kWave + lane);
+    if (p < t.cnt) {
+      const uint32_t d = (key[j] >> shift) & 255u;
+      rk[j] += whist[w][d];  // slot in the digit-ordered tile
+      skey[rk[j]] = key[j];
+    }
+  }
+  __syncthreads();
+  uint32_t dst[kSortItems];  // observation-relative destination of slot q
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t q = (uint32_t)(j * kSortThreads) + threadIdx.x;
+    dst[j] = 0;
+    if (q < t.cnt) {
+      const uint32_t k = skey[q];
+      dst[j] = goff[(k >> shift) & 255u] + q;
+      kout[t.eb + dst[j]] = k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t p = (uint32_t)(w * kSortRun + j * kWave + lane);
+    if (p < t.cnt) stage[rk[j]] = vin[src0 + p];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t q = (uint32_t)(j * kSortThreads) + threadIdx.x;
+    if (q < t.cnt) {
+      const uint64_t v = stage[q];
+      if (FINAL) {
+        cout[t.eb + dst[j]] = (uint16_t)(v >> 32);
+        iout[t.eb + dst[j]] = __uint_as_float((uint32_t)v);
+      } else {
+        vout[t.eb + dst[j]] = v;
+      }
+    }
+  }
+}
+
+// slice_starts[o][b + 1] = end of bin b's run in the sorted observation;
+// the other entries stay 0 until the prefix maximum fills them.
+__global__ __launch_bounds__(kSortThreads) void run_end_kernel(
+    SortGeo g, const uint32_t *keys, uint32_t n_grid, uint32_t *ss) {
+  const SortTile t = sort_tile(g, blockIdx.x);
+  uint32_t *sso = ss + (size_t)t.o * (n_grid + 1);
+  for (uint32_t p = threadIdx.x; p < t.cnt; p += kSortThreads) {
+    const uint64_t r = (uint64_t)t.pos0 + p;  // position in the observation
+    const uint32_t k = keys[t.eb + r];
+    const bool last = r + 1 == t.n_o || keys[t.eb + r + 1] != k;
+    if (last) sso[k + 1] = (uint32_t)(r + 1);
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, kWave);
+    if (lane >= o) v = max(v, t);
+  }
+  return v;
+}
+
+// Per (observation, chunk of kMaxScanChunk entries of slice_starts): max.
+__global__ __launch_bounds__(256) void chunk_max_kernel(const uint32_t *ss,
+                                                        uint32_t len,
+                                                        uint32_t *cm) {
+  __shared__ uint32_t wm[4];
+  const uint32_t o = blockIdx.y, c = blockIdx.x;
+  const uint32_t *row = ss + (size_t)o * len;
+  uint32_t m = 0;
+  for (uint32_t i = c * kMaxScanChunk + threadIdx.x;
+       i < min(len, (c + 1) * (uint32_t)kMaxScanChunk); i += 256)
+    m = max(m, row[i]);
+  m = wave_incl_max(m);
+  if (lane_id() == kWave - 1) wm[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    cm[(size_t)o * gridDim.x + c] = max(max(wm[0], wm[1]), max(wm[2], wm[3]));
+}
+
+// Exclusive prefix maximum over the chunk maxima of each observation.
+__global__ __launch_bounds__(256) void chunk_scan_kernel(uint32_t *cm,
+                                                         uint32_t nc) {
+  __shared__ uint32_t wm[4];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  uint32_t *row = cm + (size_t)blockIdx.x * nc;
+  for (uint32_t c0 = 0; c0 < nc; c0 += 256) {
+    const uint32_t i = c0 + threadIdx.x;
+    const uint32_t v = i < nc ? row[i] : 0u;
+    const uint32_t incl = wave_incl_max(v);
+    const int wv = threadIdx.x / kWave;
+    if (lane_id() == kWave - 1) wm[wv] = incl;
+    __syncthreads();
+    uint32_t before = carry;
+    for (int q = 0; q < wv; ++q) before = max(before, wm[q]);
+    uint32_t excl = __shfl_up(incl, 1, kWave);
+    if (lane_id() == 0) excl = 0;
+    if (i < nc) row[i] = max(before, excl);
+    __syncthreads();
+    if (threadIdx.x == 255) carry = max(before, incl);
+    __syncthreads();
+  }
+}
+
+// In-place inclusive prefix maximum of each chunk, seeded with the maximum
+// of the chunks before it: slice_starts.
+__global__ __launch_bounds__(256) void chunk_fill_kernel(uint32_t *ss,
+                                                         uint32_t len,
+                                                         const uint32_t *cmx) {
+  constexpr int kPer = kMaxScanChunk / 256;
+  __shared__ uint32_t wm[4];
+  const uint32_t o = blockIdx.y, c = blockIdx.x;
+  uint32_t *row = ss + (size_t)o * len;
+  const uint32_t i0 = c * kMaxScanChunk + threadIdx.x * kPer;
+  uint32_t v[kPer], m = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    v[j] = i0 + j < len ? row[i0 + j] : 0u;
+    m = max(m, v[j]);
+    v[j] = m;
+  }
+  const uint32_t incl = wave_incl_max(m);
+  const int wv = threadIdx.x / kWave;
+  if (lane_id() == kWave - 1) wm[wv] = incl;
+  __syncthreads();
+  uint32_t before = cmx[(size_t)o * gridDim.x + c];
+  for (int q = 0; q < wv; ++q) before = max(before, wm[q]);
+  uint32_t excl = __shfl_up(incl, 1, kWave);
+  if (lane_id() == 0) excl = 0;
+  before = max(before, excl);
+#pragma unroll
+  for (int j = 0; j < kPer; ++j)
+    if (i0 + j < len) row[i0 + j] = max(before, v[j]);
+}
+
+namespace {
+// Device buffers of one build, freed together.
+struct BuildBufs {
+  std::vector<void *> p;
+  ~BuildBufs() {
+    for (void *q : p) (void)hipFree(q);
+  }
+  template <class T>
+  adsr_status alloc(T **out, size_t n) {
+    *out = nullptr;
+    HIP_TRY(hipMalloc((void **)out, std::max<size_t>(n, 1) * sizeof(T)));
+    p.push_back(*out);
+    return ADSR_OK;
+  }
+  void release(void *q) {
+    for (auto &x : p)
+      if (x == q) {
+        (void)hipFree(x);
+        x = nullptr;
+      }
+  }
+};
+}  // namespace
+
+#define BTRY(expr)                   \
+  do {                               \
+    adsr_status s_ = (expr);         \
+    if (s_) return s_;               \
+  } while (0)
+
+// Builds d's CSR on the device from the peaks; d->dev holds the metadata.
+static adsr_status build_csr_device(adsr_dia *d,
+                                    const std::vector<std::vector<size_t>> &spectra,
+                                    const int64_t *pstart, const int64_t *pstop,
+                                    const int64_t *cyc, const float *peak_mz,
+                                    const float *peak_int, size_t n_peaks) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  const Grid &g = global_grid();
+  const uint32_t B = (uint32_t)g.mz.size();
+  const size_t n_obs = d->n_obs;
+  const std::vector<uint64_t> &eb = d->entry_base;
+  const uint64_t total = eb[n_obs];
+  // snap items: each observation's spectra by (u16 cycle, spectrum index)
+  std::vector<SnapItem> items;
+  std::vector<uint32_t> tfirst(n_obs + 1, 0);
+  uint64_t min_src = ~0ull, max_src = 0;
+  for (size_t o = 0; o < n_obs; ++o) {
+    std::vector<size_t> sp = spectra[o];
+    auto by_cycle = [&](size_t a, size_t b) {
+      return (uint16_t)cyc[a] < (uint16_t)cyc[b];
+    };
+    if (!std::is_sorted(sp.begin(), sp.end(), by_cycle))
+      std::stable_sort(sp.begin(), sp.end(), by_cycle);
+    uint64_t dst = eb[o];
+    for (size_t s : sp) {
+      const uint64_t a = (uint64_t)pstart[s], b = (uint64_t)pstop[s];
+      if (a >= b) continue;
+      min_src = std::min(min_src, a);
+      max_src = std::max(max_src, b);
+      for (uint64_t q = a; q < b; q += kSnapPeaks) {
+        SnapItem it;
+        it.src = q;
+        it.dst = dst;
+        it.n = (uint32_t)std::min<uint64_t>(kSnapPeaks, b - q);
+        it.cycle = (uint16_t)cyc[s];
+        items.push_back(it);
+        dst += it.n;
+      }
+    }
+    const uint64_t n_o = eb[o + 1] - eb[o];
+    tfirst[o + 1] = tfirst[o] + (uint32_t)((n_o + kSortTile - 1) / kSortTile);
+  }
+  const uint32_t n_tiles = tfirst[n_obs];
+  if ((uint64_t)n_tiles * 256 >= (1ull << 32) || items.size() >= (1ull << 31))
+    return set_error(ADSR_ERR_UNSUPPORTED, "run too large for the device builder");
+  const auto t1 = clk::now();
+
+  BuildBufs tmp;
+  DeviceDia &v = d->dev;
+  size_t acc = v.bytes;
+  // peaks referenced by some observation, copied as one range
+  const uint64_t lo = total ? min_src : 0, np = total ? max_src - min_src : 0;
+  float *dmz = nullptr, *dint = nullptr;
+  BTRY(tmp.alloc(&dmz, np));
+  BTRY(tmp.alloc(&dint, np));
+  if (np) {
+    HIP_TRY(hipMemcpy(dmz, peak_mz + lo, np * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dint, peak_int + lo, np * 4, hipMemcpyHostToDevice));
+  }
+  for (auto &it : items) it.src -= lo;
+  SnapItem *ditems = nullptr;
+  uint32_t *dtf = nullptr;
+  BTRY(tmp.alloc(&ditems, items.size()));
+  BTRY(tmp.alloc(&dtf, tfirst.size()));
+  HIP_TRY(hipMemcpy(ditems, items.data(), items.size() * sizeof(SnapItem),
+                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dtf, tfirst.data(), tfirst.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipDeviceSynchronize());
+  const auto t2 = clk::now();
+
+  uint32_t *keys[2];
+  uint64_t *vals[2];
+  BTRY(tmp.alloc(&keys[0], total));
+  BTRY(tmp.alloc(&keys[1], total));
+  BTRY(tmp.alloc(&vals[0], total));
+  BTRY(tmp.alloc(&vals[1], total));
+  const size_t nh = (size_t)n_tiles * 256;
+  const size_t nhb = (nh + kScanBlock - 1) / kScanBlock;
+  uint32_t *H, *hscan, *hbsum, *htot;
+  BTRY(tmp.alloc(&H, nh));
+  BTRY(tmp.alloc(&hscan, nh));
+  BTRY(tmp.alloc(&hbsum, nhb));
+  BTRY(tmp.alloc(&htot, 1));
+  // CSR arrays of the handle
+  const size_t len = (size_t)B + 1;
+  HIP_TRY(hipMalloc((void **)&v.slice_starts, std::max<size_t>(n_obs * len, 1) * 4));
+  HIP_TRY(hipMalloc((void **)&v.cycles, (total + 8) * 2));
+  HIP_TRY(hipMalloc((void **)&v.intens, std::max<uint64_t>(total, 1) * 4));
+  acc += n_obs * len * 4 + (total + 8) * 2 + total * 4;
+  HIP_TRY(hipMemset(v.cycles + total, 0xff, 8 * 2));
+  if (n_obs) HIP_TRY(hipMemset(v.slice_starts, 0, n_obs * len * 4));
+
+  const KDia kd = make_kdia(d);
+  SortGeo geo;
+  geo.ebase = v.entry_base;
+  geo.tfirst = dtf;
+  geo.n_obs = (uint32_t)n_obs;
+  if (!items.empty()) {
+    snap_kernel<<<(uint32_t)items.size(), 256>>>(kd, ditems, dmz, dint, keys[0], vals[0]);
+    HIP_TRY(hipGetLastError());
+  }
+  if (n_tiles) {
+    int cur = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+      const uint32_t shift = 8u * (uint32_t)pass;
+      radix_upsweep_kernel<<<n_tiles, kSortThreads>>>(geo, keys[cur], shift, H);
+      HIP_TRY(hipGetLastError());
+      scan_local_kernel<<<(uint32_t)nhb, 256>>>(H, (uint32_t)nh, hscan, hbsum);
+      HIP_TRY(hipGetLastError());
+      scan_blocks_kernel<<<1, 1024>>>(hbsum, (uint32_t)nhb, htot);
+      HIP_TRY(hipGetLastError());
+      if (pass < 2) {
+        radix_downsweep_kernel<false><<<n_tiles, kSortThreads>>>(
+            geo, keys[cur], vals[cur], shift, hscan, hbsum, keys[cur ^ 1],
+            vals[cur ^ 1], nullptr, nullptr);
+      } else {
+        radix_downsweep_kernel<true><<<n_tiles, kSortThreads>>>(
+            geo, keys[cur], vals[cur], shift, hscan, hbsum, keys[cur ^ 1],
+            nullptr, v.cycles, v.intens);
+      }
+      HIP_TRY(hipGetLastError());
+      cur ^= 1;
+    }
+    run_end_kernel<<<n_tiles, kSortThreads>>>(geo, keys[cur], B, v.slice_starts);
+    HIP_TRY(hipGetLastError());
+  }
+  if (n_obs) {
+    const uint32_t nc = (uint32_t)((len + kMaxScanChunk - 1) / kMaxScanChunk);
+    uint32_t *cm;
+    BTRY(tmp.alloc(&cm, n_obs * nc));
+    chunk_max_kernel<<<dim3(nc, (uint32_t)n_obs), 256>>>(v.slice_starts, (uint32_t)len, cm);
+    HIP_TRY(hipGetLastError());
+    chunk_scan_kernel<<<(uint32_t)n_obs, 256>>>(cm, nc);
+    HIP_TRY(hipGetLastError());
+    chunk_fill_kernel<<<dim3(nc, (uint32_t)n_obs), 256>>>(v.slice_starts, (uint32_t)len, cm);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  const auto t3 = clk::now();
+  v.bytes = acc;
+  auto ms = [](clk::duration x) { return std::chrono::duration<double, std::milli>(x).count(); };
+  d->build_ms[0] = ms(t1 - t0);
+  d->build_ms[1] = ms(t2 - t1);
+  d->build_ms[2] = ms(t3 - t2);
+  return ADSR_OK;
+}
+#undef BTRY
+
+extern "C" adsr_status adsr_dia_create_device(
+    const int64_t *delta, const float *iso_lo, const float *iso_hi,
+    const int64_t *pstart, const int64_t *pstop, const int64_t *cyc,
+    const float *srt, size_t n_spectra, const float *peak_mz,
+    const float *peak_int, size_t n_peaks, const float *cycle,
+    const size_t cycle_shape[4], adsr_dia **out) {
+  clear_error();
+  if (!out) return set_error(ADSR_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  adsr_status st = require_device();
+  if (st) return st;
+  std::vector<std::vector<size_t>> spectra;
+  adsr_dia *d = nullptr;
+  st = dia_prepare(delta, iso_lo, iso_hi, pstart, pstop, cyc, srt, n_spectra,
+                   peak_mz, peak_int, n_peaks, cycle, cycle_shape, false,
+                   spectra, &d);
+  if (st) return st;
+  {
+    std::lock_guard<std::mutex> lk(d->mu);
+    size_t acc = 0;
+    st = upload_dia_meta(d, acc);
+    if (!st) {
+      d->dev.bytes = acc;
+      st = build_csr_device(d, spectra, pstart, pstop, cyc, peak_mz, peak_int,
+                            n_peaks);
+    }
+    if (st) {
+      device_dia_free(d->dev);
+    } else {
+      d->dev.ready = true;
+    }
+  }
+  if (st) {
+    delete d;
+    return st;
+  }
+  *out = d;
+  return ADSR_OK;
+}
+
+extern "C" adsr_status adsr_dia_build_timings(const adsr_dia *d, double *prep_ms,
+                                              double *copy_ms, double *build_ms) {
+  clear_error();
+  if (!d) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (prep_ms) *prep_ms = d->build_ms[0];
+  if (copy_ms) *copy_ms = d->build_ms[1];
+  if (build_ms) *build_ms = d->build_ms[2];
+  return ADSR_OK;
+}
+
+namespace adsr {
+adsr_status device_dia_window(const adsr_dia *d, size_t o, uint32_t *ss,
+                              uint16_t *ci, float *in) {
+  const DeviceDia &v = d->dev;
+  if (!v.ready) return set_error(ADSR_ERR_DEVICE, "DIAData has no device CSR");
+  const size_t len = d->n_grid + 1;
+  const uint64_t a = d->entry_base[o], b = d->entry_base[o + 1];
+  if (ss) HIP_TRY(hipMemcpy(ss, v.slice_starts + o * len, len * 4, hipMemcpyDeviceToHost));
+  if (ci && b > a) HIP_TRY(hipMemcpy(ci, v.cycles + a, (b - a) * 2, hipMemcpyDeviceToHost));
+  if (in && b > a) HIP_TRY(hipMemcpy(in, v.intens + a, (b - a) * 4, hipMemcpyDeviceToHost));
+  return ADSR_OK;
+}
+}  // namespace adsr

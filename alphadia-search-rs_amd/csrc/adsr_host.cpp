@@ -217,13 +217,17 @@ extern "C" size_t adsr_gaussian_kernel(float fwhm_rt, float sigma_scale_rt,
 // ===========================================================================
 // DIAData -- src/dia_data/mod.rs:41-135, src/dia_data_builder/mod.rs:11-139
 // ===========================================================================
-extern "C" adsr_status adsr_dia_create(
-    const int64_t *delta, const float *iso_lo, const float *iso_hi,
-    const int64_t *pstart, const int64_t *pstop, const int64_t *cyc,
-    const float *srt, size_t n_spectra, const float *peak_mz,
-    const float *peak_int, size_t n_peaks, const float *cycle,
-    const size_t cycle_shape[4], adsr_dia **out) {
-  clear_error();
+namespace adsr {
+
+adsr_status dia_prepare(const int64_t *delta, const float *iso_lo,
+                        const float *iso_hi, const int64_t *pstart,
+                        const int64_t *pstop, const int64_t *cyc,
+                        const float *srt, size_t n_spectra,
+                        const float *peak_mz, const float *peak_int,
+                        size_t n_peaks, const float *cycle,
+                        const size_t cycle_shape[4], bool host_csr,
+                        std::vector<std::vector<size_t>> &spectra,
+                        adsr_dia **out) {
   if (!out) return set_error(ADSR_ERR_INVALID_ARGUMENT, "out is NULL");
   *out = nullptr;
   if (n_spectra && (!delta || !iso_lo || !iso_hi || !pstart || !pstop ||
@@ -267,7 +271,8 @@ extern "C" adsr_status adsr_dia_create(
                      "at most 65535 quadrupole observations are supported");
   }
   d->n_obs = n_obs;
-  std::vector<std::vector<size_t>> spectra(n_obs);
+  d->host_csr = host_csr;
+  spectra.assign(n_obs, {});
   std::vector<uint64_t> count(n_obs + 1, 0);
   for (size_t s = 0; s < n_spectra; ++s) {
     if (delta[s] < 0) continue;
@@ -289,9 +294,11 @@ extern "C" adsr_status adsr_dia_create(
     d->iso_hi.assign(n_obs, 0.0f);
     d->num_cycles.assign(n_obs, 0);
     d->entry_base = count;
-    d->slice_starts.assign(n_obs * (B + 1), 0u);
-    d->cycles.resize(total);
-    d->intens.resize(total);
+    if (host_csr) {
+      d->slice_starts.assign(n_obs * (B + 1), 0u);
+      d->cycles.resize(total);
+      d->intens.resize(total);
+    }
   } catch (const std::bad_alloc &) {
     delete d;
     return set_error(ADSR_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -304,20 +311,46 @@ extern "C" adsr_status adsr_dia_create(
                        "slice_starts)",
                        o);
     }
-
-  // src/dia_data_builder/mod.rs:52-139, one observation per task
+  // src/dia_data_builder/mod.rs:71-80: window bounds from the first
+  // spectrum, number of distinct cycles; an empty window keeps [0, 0]
   parallel_for(n_obs, [&](size_t o) {
     const std::vector<size_t> &sp = spectra[o];
-    uint32_t *ss = d->slice_starts.data() + o * (B + 1);
-    if (sp.empty()) return;  // :71-80 empty: window [0,0], zero slices
+    if (sp.empty()) return;
     d->iso_lo[o] = iso_lo[sp[0]];
     d->iso_hi[o] = iso_hi[sp[0]];
-    {
-      std::vector<int64_t> c(sp.size());
-      for (size_t i = 0; i < sp.size(); ++i) c[i] = cyc[sp[i]];
-      std::sort(c.begin(), c.end());
-      d->num_cycles[o] = (uint64_t)(std::unique(c.begin(), c.end()) - c.begin());
-    }
+    std::vector<int64_t> c(sp.size());
+    for (size_t i = 0; i < sp.size(); ++i) c[i] = cyc[sp[i]];
+    std::sort(c.begin(), c.end());
+    d->num_cycles[o] = (uint64_t)(std::unique(c.begin(), c.end()) - c.begin());
+  });
+  *out = d;
+  return ADSR_OK;
+}
+
+}  // namespace adsr
+
+extern "C" adsr_status adsr_dia_create(
+    const int64_t *delta, const float *iso_lo, const float *iso_hi,
+    const int64_t *pstart, const int64_t *pstop, const int64_t *cyc,
+    const float *srt, size_t n_spectra, const float *peak_mz,
+    const float *peak_int, size_t n_peaks, const float *cycle,
+    const size_t cycle_shape[4], adsr_dia **out) {
+  clear_error();
+  std::vector<std::vector<size_t>> spectra;
+  adsr_dia *d = nullptr;
+  adsr_status st = dia_prepare(delta, iso_lo, iso_hi, pstart, pstop, cyc, srt,
+                               n_spectra, peak_mz, peak_int, n_peaks, cycle,
+                               cycle_shape, true, spectra, &d);
+  if (st) return st;
+  const Grid &g = global_grid();
+  const size_t B = g.mz.size();
+  const std::vector<uint64_t> &count = d->entry_base;
+
+  // src/dia_data_builder/mod.rs:52-139, one observation per task
+  parallel_for(d->n_obs, [&](size_t o) {
+    const std::vector<size_t> &sp = spectra[o];
+    uint32_t *ss = d->slice_starts.data() + o * (B + 1);
+    if (sp.empty()) return;  // :71-80 empty: zero slices
     const size_t nw = count[o + 1] - count[o];
     if (nw == 0) return;
     // (bin, cycle as u16, intensity) in spectrum/peak order (:97-109)
@@ -451,11 +484,12 @@ extern "C" adsr_status adsr_dia_observation(
   }
   if (num_cycles) *num_cycles = d->num_cycles[o];
   if (n_ss) *n_ss = d->n_grid + 1;
+  const size_t a = d->entry_base[o], b = d->entry_base[o + 1];
+  if (n_entries) *n_entries = b - a;
+  if (!d->host_csr) return device_dia_window(d, o, ss, ci, in);
   if (ss)
     std::memcpy(ss, d->slice_starts.data() + o * (d->n_grid + 1),
                 (d->n_grid + 1) * 4);
-  const size_t a = d->entry_base[o], b = d->entry_base[o + 1];
-  if (n_entries) *n_entries = b - a;
   if (ci) std::memcpy(ci, d->cycles.data() + a, (b - a) * 2);
   if (in) std::memcpy(in, d->intens.data() + a, (b - a) * 4);
   return ADSR_OK;

@@ -73,6 +73,32 @@ void device_lib_free(DeviceLib &l);
 
 }  // namespace adsr
 
+struct adsr_dia;
+
+namespace adsr {
+
+// DIAData metadata shared by the host and the device CSR builders
+// (src/dia_data/mod.rs:41-72, src/dia_data_builder/mod.rs:27-80): argument
+// checks, RT axis, one observation per delta_scan_idx with its spectra in
+// ascending spectrum order, entry bases, window bounds and cycle counts.
+// With host_csr the CSR arrays are allocated (zeroed) for the host fill.
+adsr_status dia_prepare(const int64_t *delta, const float *iso_lo,
+                        const float *iso_hi, const int64_t *pstart,
+                        const int64_t *pstop, const int64_t *cyc,
+                        const float *srt, size_t n_spectra,
+                        const float *peak_mz, const float *peak_int,
+                        size_t n_peaks, const float *cycle,
+                        const size_t cycle_shape[4], bool host_csr,
+                        std::vector<std::vector<size_t>> &spectra,
+                        adsr_dia **out);
+
+// Copies observation o's CSR of a device-built handle to the host (any of
+// the three pointers may be NULL).
+adsr_status device_dia_window(const adsr_dia *d, size_t o, uint32_t *ss,
+                              uint16_t *ci, float *in);
+
+}  // namespace adsr
+
 // ---- opaque handles ---------------------------------------------------------
 struct adsr_dia {
   // src/dia_data/mod.rs:15-21
@@ -94,6 +120,9 @@ struct adsr_dia {
   std::vector<uint16_t> cycles;
   std::vector<float> intens;
   bool rt_sorted = true;
+  // false: the CSR was built on the device and lives only in `dev`
+  bool host_csr = true;
+  double build_ms[3] = {0, 0, 0};  // device build: prepare, copy, kernels
   adsr::DeviceDia dev;
   std::mutex mu;
 };

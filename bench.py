@@ -22,6 +22,10 @@ scalar path) is timed on the host's cores on a sample of the same library
 for cpu_baseline, and its candidates are compared with the device's on that
 sample.
 
+The run's DIAData is built on the device (adsr_dia_create_device) and,
+unless --no-csr-check, also by the host builder; the two CSRs must be equal
+byte for byte.  Both build times are reported under config.dia_build.
+
 Prints one JSON line.
 """
 import argparse
@@ -136,6 +140,9 @@ def main():
                     help="library rows timed on the host for cpu_baseline "
                          "(0 = the whole library)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-csr-check", action="store_true",
+                    help="skip the host build of the run and the byte "
+                         "comparison of its CSR with the device-built one")
     ap.add_argument("--threads-per-block", type=int, default=0)
     ap.add_argument("--precursors-per-wave", type=int, default=0)
     ap.add_argument("--margin", type=float, default=1.0,
@@ -182,9 +189,15 @@ def main():
     log(f"generated {gen['n_peaks']} peaks, {spec.n_precursors} precursors in "
         f"{time.time() - t0:.1f}s")
     t1 = time.time()
-    dia = ads.DIAData.from_arrays(*gen["dia"])
+    dia = ads.DIAData.from_arrays(*gen["dia"], device_build=True)
+    dia_wall = time.time() - t1
+    bt = dia.build_timings()
+    log(f"DIAData built on the device in {dia_wall:.2f}s (prepare "
+        f"{bt['prepare_ms']:.0f} ms, peak copy {bt['copy_ms']:.0f} ms, kernels "
+        f"{bt['kernels_ms']:.1f} ms)")
+    t1 = time.time()
     lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
-    log(f"DIAData/SpecLibFlat built and uploaded in {time.time() - t1:.1f}s")
+    log(f"SpecLibFlat built and uploaded in {time.time() - t1:.1f}s")
     params = ads.SelectionParameters()
     params.update(spec.selection_config())
     if gen["rt_tolerance_per_precursor"] is not None:
@@ -284,6 +297,29 @@ def main():
             "traffic": measured_traffic(args.workload),
         },
     }
+
+    build = {"device_wall_s": round(dia_wall, 3),
+             **{k: round(v, 3) for k, v in bt.items()}}
+    if not args.no_csr_check:
+        # the same run through the host builder (src/dia_data_builder), and
+        # every observation's CSR compared byte for byte
+        t1 = time.time()
+        hdia = ads.DIAData.from_arrays(*gen["dia"], device_build=False)
+        build["host_builder_wall_s"] = round(time.time() - t1, 3)
+        same = hdia.num_observations == dia.num_observations
+        for o in range(dia.num_observations):
+            if not same:
+                break
+            a, b = dia._observation(o), hdia._observation(o)
+            same = all(np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8))
+                       for k in ("slice_starts", "cycle_indices", "intensities",
+                                 "isolation_window")) and \
+                a["num_cycles"] == b["num_cycles"]
+        del hdia
+        build["csr_identical_to_host_builder"] = bool(same)
+        log(f"host builder {build['host_builder_wall_s']:.2f}s; device CSR "
+            f"{'identical' if same else 'DIFFERENT'}")
+    result["config"]["dia_build"] = build
 
     if fb is not None:
         fb_rate = n_prec / (fb[0] / 1e3)

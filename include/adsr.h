@@ -83,6 +83,30 @@ adsr_status adsr_dia_create(const int64_t *spectrum_delta_scan_idx,
                             const float *peak_mz, const float *peak_intensity,
                             size_t n_peaks, const float *cycle,
                             const size_t cycle_shape[4], adsr_dia **out);
+/* DIAData.from_arrays with the CSR built on the device.  Same arguments,
+ * checks and result as adsr_dia_create, bit for bit: DIADataBuilder's bin
+ * snap and stable (bin, cycle) sort (src/dia_data_builder/mod.rs:52-139)
+ * run as gfx950 kernels (a snap pass and a three-pass stable radix sort per
+ * observation), and the CSR stays in HBM only; adsr_dia_observation copies
+ * one observation back on request.  Needs a gfx950 device (DEVICE error
+ * otherwise).  Peak arrays are read once and not kept.                    */
+adsr_status adsr_dia_create_device(const int64_t *spectrum_delta_scan_idx,
+                                   const float *isolation_lower_mz,
+                                   const float *isolation_upper_mz,
+                                   const int64_t *spectrum_peak_start_idx,
+                                   const int64_t *spectrum_peak_stop_idx,
+                                   const int64_t *spectrum_cycle_idx,
+                                   const float *spectrum_rt, size_t n_spectra,
+                                   const float *peak_mz,
+                                   const float *peak_intensity, size_t n_peaks,
+                                   const float *cycle,
+                                   const size_t cycle_shape[4],
+                                   adsr_dia **out);
+/* Wall-clock milliseconds of a device build (0 for host-built handles):
+ * host preparation (spectrum order, snap items), host-to-device copy of
+ * the peaks, and the device kernels (snap, sort, slice_starts).           */
+adsr_status adsr_dia_build_timings(const adsr_dia *d, double *prep_ms,
+                                   double *copy_ms, double *build_ms);
 /* Copies the CSR and indices into device memory (HBM); idempotent. */
 adsr_status adsr_dia_upload(adsr_dia *d);
 void adsr_dia_free(adsr_dia *d);
