@@ -26,6 +26,7 @@ __all__ = [
     "SelectionParameters",
     "PeakGroupSelection",
     "CandidateCollection",
+    "DeviceCandidateCollection",
     "get_optimal_simd_backend",
     "set_simd_backend",
     "clear_simd_backend",
@@ -97,10 +98,12 @@ class DIAData:
         self._h = None
         self._n_obs = 0
 
-    def __del__(self):
+    def __del__(self, _free=N.lib.adsr_dia_free):
+        # the free function is bound at definition: module globals may be
+        # gone when the interpreter shuts down
         h = getattr(self, "_h", None)
         if h:
-            N.lib.adsr_dia_free(h)
+            _free(h)
             self._h = None
 
     @staticmethod
@@ -249,10 +252,12 @@ class SpecLibFlat:
         self._np = 0
         self._nf = 0
 
-    def __del__(self):
+    def __del__(self, _free=N.lib.adsr_lib_free):
+        # the free function is bound at definition: module globals may be
+        # gone when the interpreter shuts down
         h = getattr(self, "_h", None)
         if h:
-            N.lib.adsr_lib_free(h)
+            _free(h)
             self._h = None
 
     @staticmethod
@@ -429,10 +434,12 @@ class CandidateCollection:
     def __init__(self):
         self._h = None
 
-    def __del__(self):
+    def __del__(self, _free=N.lib.adsr_candidates_free):
+        # the free function is bound at definition: module globals may be
+        # gone when the interpreter shuts down
         h = getattr(self, "_h", None)
         if h:
-            N.lib.adsr_candidates_free(h)
+            _free(h)
             self._h = None
 
     @staticmethod
@@ -478,6 +485,56 @@ class CandidateCollection:
         return tuple(cols)
 
 
+class DeviceCandidateCollection:
+    """The candidate table of ``PeakGroupSelection.search_device``, kept in
+    device memory: the same rows, in the same order, as ``search``.  Hands
+    the table to device-side consumers (scoring) without a host round trip.
+    """
+
+    def __init__(self):
+        self._h = None
+
+    def __del__(self, _free=N.lib.adsr_device_candidates_free):
+        h = getattr(self, "_h", None)
+        if h:
+            _free(h)
+            self._h = None
+
+    def len(self) -> int:
+        return int(N.lib.adsr_device_candidates_len(self._h)) if self._h else 0
+
+    def __len__(self) -> int:
+        return self.len()
+
+    def is_empty(self) -> bool:
+        return self.len() == 0
+
+    def to_torch(self, stream=None):
+        """The nine ``to_arrays`` columns (src/candidate/entry.rs:124-173) as
+        torch tensors on the current device, written by one kernel on
+        ``stream`` (default: torch's current stream).  The u64 columns are
+        int64 tensors holding the same bits; score is float32."""
+        import torch
+        n = self.len()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        cols = [torch.empty(n, dtype=torch.float32 if i == 2 else torch.int64,
+                            device=dev) for i in range(9)]
+        s = torch.cuda.current_stream(dev) if stream is None else stream
+        if self._h and n:
+            check(N.lib.adsr_device_candidates_export(
+                self._h, *[C.c_void_p(t.data_ptr()) for t in cols],
+                C.c_void_p(s.cuda_stream)))
+        return tuple(cols)
+
+    def to_collection(self) -> "CandidateCollection":
+        """The host ``CandidateCollection`` of the same table."""
+        if not self._h:
+            return CandidateCollection()
+        out = C.c_void_p()
+        check(N.lib.adsr_device_candidates_to_host(self._h, C.byref(out)))
+        return CandidateCollection._wrap(out.value)
+
+
 # ---------------------------------------------------------------------------
 # PeakGroupSelection (src/peak_group_selection/mod.rs:66-128)
 # ---------------------------------------------------------------------------
@@ -510,3 +567,17 @@ class PeakGroupSelection:
             print(f"Found {cand.len()} candidates")
             sys.stdout.flush()
         return cand
+
+    def search_device(self, dia_data: DIAData,
+                      lib: SpecLibFlat) -> DeviceCandidateCollection:
+        """``search`` with the candidate table left in device memory."""
+        if not isinstance(dia_data, DIAData) or not isinstance(lib, SpecLibFlat):
+            raise TypeError("search_device(dia_data: DIAData, lib: SpecLibFlat)")
+        out = DeviceCandidateCollection()
+        if not dia_data._h or not lib._h:
+            return out
+        c = self._params._c()
+        h = C.c_void_p()
+        check(N.lib.adsr_select_device(dia_data._h, lib._h, C.byref(c), C.byref(h)))
+        out._h = h.value
+        return out

@@ -122,6 +122,11 @@ _sig("adsr_candidates_export", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
 _sig("adsr_candidates_from_arrays", I, c_size, vp, vp, vp, vp, vp, vp, vp, vp,
      vp, C.POINTER(vp))
 _sig("adsr_candidates_free", None, vp)
+_sig("adsr_select_device", I, vp, vp, C.POINTER(SelectionParamsC), C.POINTER(vp))
+_sig("adsr_device_candidates_len", c_size, vp)
+_sig("adsr_device_candidates_export", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
+_sig("adsr_device_candidates_to_host", I, vp, C.POINTER(vp))
+_sig("adsr_device_candidates_free", None, vp)
 _sig("adsr_gaussian_kernel", c_size, c_f32, c_f32, c_u64, c_f32, vp, c_size)
 _sig("adsr_gaussian_kernel_sigma", c_size, c_f32, c_u64, vp, c_size)
 _sig("adsr_convolution", I, vp, c_size, vp, c_size, c_size, vp)

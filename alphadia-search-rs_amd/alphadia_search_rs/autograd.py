@@ -59,10 +59,12 @@ class SelectionGradState:
         self._h = out.value
         self._fwd_done = False
 
-    def __del__(self):
+    def __del__(self, _free=N.lib.adsr_diff_free):
+        # the free function is bound at definition: module globals may be
+        # gone when the interpreter shuts down
         h = getattr(self, "_h", None)
         if h:
-            N.lib.adsr_diff_free(h)
+            _free(h)
             self._h = None
 
     def _check_inputs(self, sigma: torch.Tensor, weights: torch.Tensor):

@@ -85,6 +85,9 @@ struct KDia {
   const uint16_t *cycles;
   const float *intens;
   const uint4 *bucket2;  // (count below, grid[count], grid[count + 1], -)
+  // per bucket: count below (22 bits) | grid points in it (2) << 22 |
+  // ulp offset of the first << 24 | of the second << 28
+  const uint32_t *bucket1;
   float grid_first, grid_last;
 };
 
@@ -1438,6 +1441,55 @@ __global__ __launch_bounds__(256) void scatter_kernel(
   }
 }
 
+// A candidate row kept in HBM: the library's precursor_idx resolved, so the
+// table outlives the library handle.
+struct DevCand {
+  uint64_t pidx;
+  uint32_t rank;
+  float score;
+  uint32_t center;
+  uint32_t pad;
+};
+
+__global__ __launch_bounds__(256) void resolve_kernel(const CandRow *rows,
+                                                      uint32_t n,
+                                                      const uint64_t *p_idx,
+                                                      DevCand *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CandRow r = rows[i];
+  DevCand c;
+  c.pidx = p_idx[r.row];
+  c.rank = r.rank;
+  c.score = r.score;
+  c.center = r.center;
+  c.pad = 0;
+  out[i] = c;
+}
+
+// CandidateCollection::to_arrays (src/candidate/entry.rs:124-173) into
+// device columns; scan_* are 0 from selection (entry.rs:31-33) and the
+// cycle bounds follow src/peak_group_selection/mod.rs:174-180.
+__global__ __launch_bounds__(256) void export_kernel(
+    const DevCand *rows, uint64_t n, uint64_t pl, uint64_t nrt, uint64_t *pidx,
+    uint64_t *rank, float *score, uint64_t *scan_center, uint64_t *scan_start,
+    uint64_t *scan_stop, uint64_t *cycle_center, uint64_t *cycle_start,
+    uint64_t *cycle_stop) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevCand c = rows[i];
+  const uint64_t center = c.center;
+  if (pidx) pidx[i] = c.pidx;
+  if (rank) rank[i] = c.rank;
+  if (score) score[i] = c.score;
+  if (scan_center) scan_center[i] = 0;
+  if (scan_start) scan_start[i] = 0;
+  if (scan_stop) scan_stop[i] = 0;
+  if (cycle_center) cycle_center[i] = center;
+  if (cycle_start) cycle_start[i] = center - pl;  // usize arithmetic, mod.rs:176
+  if (cycle_stop) cycle_stop[i] = min(center + pl + 1, nrt);
+}
+
 // Algorithmic bytes per (row, selected fragment): slice_starts of the bin
 // range, every CSR entry's cycle index, the intensity of entries in the RT
 // window, per valid window.  Reduced with one atomic per block.
@@ -1920,6 +1972,7 @@ void device_dia_free(DeviceDia &d) {
   (void)hipFree(d.grid);
   (void)hipFree(d.bucket);
   (void)hipFree(d.bucket2);
+  (void)hipFree(d.bucket1);
   (void)hipFree(d.rt);
   (void)hipFree(d.iso_lo);
   (void)hipFree(d.iso_hi);
@@ -1972,6 +2025,23 @@ static adsr_status upload_dia_meta(adsr_dia *d, size_t &acc) {
       b2[4 * b + 3] = 0;
     }
     UP(v.bucket2, b2.data(), b2.size());
+    // the same buckets in 4 bytes: the grid values themselves are the
+    // bucket's bit pattern with the stored ulp offsets
+    std::vector<uint32_t> b1(g.bucket.size());
+    for (size_t b = 0; b < g.bucket.size(); ++b) {
+      const uint32_t c = g.bucket[b];
+      uint32_t nin = 0, off[2] = {0, 0};
+      for (uint32_t k = c; k < n && nin < 3; ++k) {
+        const uint32_t bits = __builtin_bit_cast(uint32_t, g.mz[k]);
+        if ((bits >> kBucketShift) - g.base_key != b) break;
+        if (nin < 2) off[nin] = bits & 15u;
+        ++nin;
+      }
+      if (nin > 2 || c >= (1u << 22))
+        return set_error(ADSR_ERR_INTERNAL, "m/z grid bucket table invariant");
+      b1[b] = c | (nin << 22) | (off[0] << 24) | (off[1] << 28);
+    }
+    UP(v.bucket1, b1.data(), b1.size());
   }
   UP(v.rt, d->rt_index.data(), d->rt_index.size());
   UP(v.iso_lo, d->iso_lo.data(), d->iso_lo.size());
@@ -2021,6 +2091,7 @@ static KDia make_kdia(const adsr_dia *d) {
   k.cycles = dv.cycles;
   k.intens = dv.intens;
   k.bucket2 = reinterpret_cast<const uint4 *>(dv.bucket2);
+  k.bucket1 = dv.bucket1;
   k.grid_first = g.mz.front();
   k.grid_last = g.mz.back();
   return k;
@@ -2458,6 +2529,36 @@ static adsr_status fetch(Workspace &w, const adsr_dia *d, const adsr_lib *l,
   return ADSR_OK;
 }
 
+// The compacted table stays in HBM: rows resolved to precursor_idx.
+static adsr_status fetch_device(Workspace &w, const adsr_dia *d,
+                                const adsr_lib *l, const adsr_selection_params *p,
+                                adsr_device_candidates **out) {
+  uint32_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, w.small + 2, 4, hipMemcpyDeviceToHost,
+                         w.stream));
+  HIP_TRY(hipStreamSynchronize(w.stream));
+  if (l->n_prec == 0) total = 0;
+  auto *c = new adsr_device_candidates();
+  c->n = total;
+  c->pl = p->peak_length;
+  c->nrt = d->rt_index.size();
+  if (hipMalloc((void **)&c->rows, std::max<size_t>(total, 1) * sizeof(DevCand)) !=
+      hipSuccess) {
+    delete c;
+    return set_error(ADSR_ERR_OUT_OF_MEMORY, "device candidate table");
+  }
+  if (total)
+    resolve_kernel<<<(total + 255) / 256, 256, 0, w.stream>>>(
+        w.out, total, l->dev.p_idx, (DevCand *)c->rows);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(w.stream) != hipSuccess) {
+    (void)hipFree(c->rows);
+    delete c;
+    return set_error(ADSR_ERR_DEVICE, "candidate table kernel failed");
+  }
+  *out = c;
+  return ADSR_OK;
+}
+
 static std::map<std::tuple<size_t, uint32_t, size_t>, LaunchShape> g_tuned;
 static std::mutex g_tuned_mu;
 
@@ -2518,11 +2619,13 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
                                  const adsr_selection_params *p, int warmup,
                                  int steps, int *tpb, int *ppw, double *step_ms,
                                  double *kernel_ms, double *graph_ms,
-                                 adsr_candidates **out) {
+                                 adsr_candidates **out,
+                                 adsr_device_candidates **dout = nullptr) {
   clear_error();
-  if (!dc || !lc || !p || !out)
+  if (!dc || !lc || !p || (!out && !dout))
     return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
-  *out = nullptr;
+  if (out) *out = nullptr;
+  if (dout) *dout = nullptr;
   adsr_dia *d = const_cast<adsr_dia *>(dc);
   adsr_lib *l = const_cast<adsr_lib *>(lc);
   adsr_status st = upload_dia(d);
@@ -2673,9 +2776,78 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
     }
   }
   st = check_status(w);
-  if (!st) st = fetch(w, d, l, p, out);
+  if (!st) st = dout ? fetch_device(w, d, l, p, dout) : fetch(w, d, l, p, out);
   ws_free(w);
   return st;
+}
+
+extern "C" adsr_status adsr_select_device(const adsr_dia *d, const adsr_lib *l,
+                                          const adsr_selection_params *p,
+                                          adsr_device_candidates **out) {
+  if (!out) {
+    clear_error();
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  }
+  return select_common(d, l, p, 0, 0, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, out);
+}
+
+extern "C" size_t adsr_device_candidates_len(const adsr_device_candidates *c) {
+  return c ? c->n : 0;
+}
+
+extern "C" adsr_status adsr_device_candidates_export(
+    const adsr_device_candidates *c, uint64_t *precursor_idx, uint64_t *rank,
+    float *score, uint64_t *scan_center, uint64_t *scan_start,
+    uint64_t *scan_stop, uint64_t *cycle_center, uint64_t *cycle_start,
+    uint64_t *cycle_stop, void *stream) {
+  clear_error();
+  if (!c) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (c->n == 0) return ADSR_OK;
+  export_kernel<<<(uint32_t)((c->n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      (const DevCand *)c->rows, c->n, c->pl, c->nrt, precursor_idx, rank, score,
+      scan_center, scan_start, scan_stop, cycle_center, cycle_start, cycle_stop);
+  HIP_TRY(hipGetLastError());
+  return ADSR_OK;
+}
+
+extern "C" adsr_status adsr_device_candidates_to_host(
+    const adsr_device_candidates *c, adsr_candidates **out) {
+  clear_error();
+  if (!c || !out) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  std::vector<DevCand> rows(c->n);
+  if (c->n)
+    HIP_TRY(hipMemcpy(rows.data(), c->rows, c->n * sizeof(DevCand),
+                      hipMemcpyDeviceToHost));
+  auto *h = new adsr_candidates();
+  const size_t n = c->n;
+  h->precursor_idx.resize(n);
+  h->rank.resize(n);
+  h->score.resize(n);
+  h->scan_center.assign(n, 0);
+  h->scan_start.assign(n, 0);
+  h->scan_stop.assign(n, 0);
+  h->cycle_center.resize(n);
+  h->cycle_start.resize(n);
+  h->cycle_stop.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const uint64_t center = rows[i].center;
+    h->precursor_idx[i] = rows[i].pidx;
+    h->rank[i] = rows[i].rank;
+    h->score[i] = rows[i].score;
+    h->cycle_center[i] = center;
+    h->cycle_start[i] = center - c->pl;
+    h->cycle_stop[i] = std::min<uint64_t>(center + c->pl + 1, c->nrt);
+  }
+  *out = h;
+  return ADSR_OK;
+}
+
+extern "C" void adsr_device_candidates_free(adsr_device_candidates *c) {
+  if (!c) return;
+  (void)hipFree(c->rows);
+  delete c;
 }
 
 extern "C" adsr_status adsr_select(const adsr_dia *d, const adsr_lib *l,
@@ -3221,30 +3393,86 @@ constexpr int kMaxScanChunk = 8192;               // prefix-max chunk
 
 // src/mz_index/mod.rs:55-90: nearest grid point, ties to the lower index,
 // exact grid values returned as themselves, NaN and values at or below the
-// first point to 0, values beyond the last point to n - 1.
-__device__ __forceinline__ uint32_t find_closest_bin(const KDia &dia, float x) {
-  const uint32_t n = dia.n_grid;
-  uint32_t lb;
-  if (!(x > dia.grid_first)) lb = 0;
-  else if (x > dia.grid_last) lb = n;
-  else lb = lb_from(dia.bucket2[bucket_key(dia, x)], x);
-  if (lb < n && dia.grid[lb] == x) return lb;
-  if (lb == 0) return 0;
-  if (lb == n) return n - 1;
-  const float left = fabsf(dia.grid[lb] - x);
-  const float right = fabsf(dia.grid[lb - 1] - x);
+// first point to 0, values beyond the last point to n - 1.  From x's
+// 4-byte bucket entry: its count gives the lower bound, and the grid points
+// next to x are rebuilt from the bucket bits and the stored ulp offsets --
+// in x's bucket, else in the next (previous) non-empty one, which is at most
+// two buckets away because 1-ppm steps span 8.4 to 16.8 ulps.
+__device__ __forceinline__ uint32_t closest_from(const KDia &dia, float x,
+                                                 uint32_t e) {
+  const uint32_t bx = __float_as_uint(x), xo = bx & 15u;
+  const uint32_t key = (bx >> kBucketShift) - dia.base_key;
+  const uint32_t c = e & 0x3fffffu, nin = (e >> 22) & 3u;
+  const uint32_t o0 = (e >> 24) & 15u, o1 = e >> 28;
+  const bool p0 = nin >= 1, p1 = nin == 2;
+  if (p0 && o0 == xo) return c;
+  if (p1 && o1 == xo) return c + 1;
+  const uint32_t lb = c + (p0 && o0 < xo ? 1u : 0u) + (p1 && o1 < xo ? 1u : 0u);
+  const uint32_t kb = bx & ~15u;
+  uint32_t above, below;
+  if (p0 && o0 > xo) {
+    above = kb | o0;
+  } else if (p1 && o1 > xo) {
+    above = kb | o1;
+  } else {
+    uint32_t k2 = key + 1, e2 = dia.bucket1[k2];
+    if (((e2 >> 22) & 3u) == 0) e2 = dia.bucket1[++k2];
+    above = ((k2 + dia.base_key) << kBucketShift) | ((e2 >> 24) & 15u);
+  }
+  if (p1 && o1 < xo) {
+    below = kb | o1;
+  } else if (p0 && o0 < xo) {
+    below = kb | o0;
+  } else {
+    uint32_t k2 = key - 1, e2 = dia.bucket1[k2];
+    if (((e2 >> 22) & 3u) == 0) e2 = dia.bucket1[--k2];
+    below = ((k2 + dia.base_key) << kBucketShift) |
+            (((e2 >> 22) & 3u) == 2 ? (e2 >> 28) : ((e2 >> 24) & 15u));
+  }
+  const float left = fabsf(__uint_as_float(above) - x);
+  const float right = fabsf(__uint_as_float(below) - x);
   return left < right ? lb : lb - 1;
 }
 
+// Each thread snaps kSnapPeaks / 256 peaks of an item, with the m/z loads,
+// then the bucket loads, of all of them in flight together.  (Loading the
+// two neighbouring buckets up front as well measured no faster: the extra
+// divergent gathers cost what the shorter dependency chain saves.)
 __global__ __launch_bounds__(256) void snap_kernel(KDia dia, const SnapItem *items,
                                                    const float *mz, const float *inten,
                                                    uint32_t *keys, uint64_t *vals) {
+  constexpr int kPer = kSnapPeaks / 256;
   const SnapItem it = items[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < it.n; i += blockDim.x) {
-    const float x = mz[it.src + i];
-    const uint32_t bits = __float_as_uint(inten[it.src + i]);
-    keys[it.dst + i] = find_closest_bin(dia, x);
-    vals[it.dst + i] = ((uint64_t)it.cycle << 32) | bits;
+  float x[kPer];
+  uint32_t ib[kPer], e[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t i = threadIdx.x + 256u * j;
+    x[j] = 0.0f;
+    ib[j] = 0;
+    if (i < it.n) {
+      x[j] = mz[it.src + i];
+      ib[j] = __float_as_uint(inten[it.src + i]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    e[j] = 0;
+    const bool inside = x[j] > dia.grid_first && !(x[j] > dia.grid_last);
+    if (threadIdx.x + 256u * j < it.n && inside)
+      e[j] = dia.bucket1[(__float_as_uint(x[j]) >> kBucketShift) - dia.base_key];
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t i = threadIdx.x + 256u * j;
+    if (i < it.n) {
+      uint32_t b;
+      if (!(x[j] > dia.grid_first)) b = 0;  // also NaN
+      else if (x[j] > dia.grid_last) b = dia.n_grid - 1;
+      else b = closest_from(dia, x[j], e[j]);
+      keys[it.dst + i] = b;
+      vals[it.dst + i] = ((uint64_t)it.cycle << 32) | ib[j];
+    }
   }
 }
 
@@ -3500,21 +3728,30 @@ __global__ __launch_bounds__(256) void chunk_scan_kernel(uint32_t *cm,
 }
 
 // In-place inclusive prefix maximum of each chunk, seeded with the maximum
-// of the chunks before it: slice_starts.
+// of the chunks before it: slice_starts.  The chunk is loaded and stored
+// coalesced through LDS; thread t scans entries 32t .. 32t + 31, stored at
+// a stride of 33 words so that the 32 threads of a bank cycle hit 32 banks.
 __global__ __launch_bounds__(256) void chunk_fill_kernel(uint32_t *ss,
                                                          uint32_t len,
                                                          const uint32_t *cmx) {
   constexpr int kPer = kMaxScanChunk / 256;
+  __shared__ uint32_t buf[kMaxScanChunk + kMaxScanChunk / 32];
   __shared__ uint32_t wm[4];
   const uint32_t o = blockIdx.y, c = blockIdx.x;
-  uint32_t *row = ss + (size_t)o * len;
-  const uint32_t i0 = c * kMaxScanChunk + threadIdx.x * kPer;
-  uint32_t v[kPer], m = 0;
+  uint32_t *row = ss + (size_t)o * len + (size_t)c * kMaxScanChunk;
+  const uint32_t n = min((uint32_t)kMaxScanChunk, len - c * kMaxScanChunk);
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
-    v[j] = i0 + j < len ? row[i0 + j] : 0u;
-    m = max(m, v[j]);
-    v[j] = m;
+    const uint32_t i = (uint32_t)j * 256 + threadIdx.x;
+    buf[i + i / 32] = i < n ? row[i] : 0u;
+  }
+  __syncthreads();
+  uint32_t *mine = buf + threadIdx.x * (kPer + 1);
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    m = max(m, mine[j]);
+    mine[j] = m;
   }
   const uint32_t incl = wave_incl_max(m);
   const int wv = threadIdx.x / kWave;
@@ -3526,8 +3763,13 @@ __global__ __launch_bounds__(256) void chunk_fill_kernel(uint32_t *ss,
   if (lane_id() == 0) excl = 0;
   before = max(before, excl);
 #pragma unroll
-  for (int j = 0; j < kPer; ++j)
-    if (i0 + j < len) row[i0 + j] = max(before, v[j]);
+  for (int j = 0; j < kPer; ++j) mine[j] = max(before, mine[j]);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t i = (uint32_t)j * 256 + threadIdx.x;
+    if (i < n) row[i] = buf[i + i / 32];
+  }
 }
 
 namespace {

@@ -48,6 +48,7 @@ struct DeviceDia {
   // per bucket: grid points below it and the first two grid values from it
   // (u32 count, f32, f32, unused)
   uint32_t *bucket2 = nullptr;
+  uint32_t *bucket1 = nullptr;  // 4-byte form of the same buckets
   float *rt = nullptr;
   float *iso_lo = nullptr, *iso_hi = nullptr;
   uint32_t *slice_starts = nullptr;  // n_obs * (n_grid + 1)
@@ -144,6 +145,14 @@ struct adsr_lib {
   bool frag_ranges_disjoint = true;
   adsr::DeviceLib dev;
   std::mutex mu;
+};
+
+// A candidate table left in HBM by adsr_select_device: `rows` holds n
+// device records (precursor_idx, rank, score, cycle centre).
+struct adsr_device_candidates {
+  void *rows = nullptr;
+  size_t n = 0;
+  uint64_t pl = 0, nrt = 0;  // peak_length and RT-index length
 };
 
 struct adsr_candidates {

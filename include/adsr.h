@@ -31,6 +31,8 @@ typedef enum adsr_status {
 typedef struct adsr_dia adsr_dia;               /* DIAData */
 typedef struct adsr_lib adsr_lib;               /* SpecLibFlat */
 typedef struct adsr_candidates adsr_candidates; /* CandidateCollection */
+/* CandidateCollection kept in device memory (adsr_select_device) */
+typedef struct adsr_device_candidates adsr_device_candidates;
 
 /* SelectionParameters (src/peak_group_selection/parameters.rs:4-43) plus two
  * optional extensions; with both NULL the result equals the reference.    */
@@ -262,6 +264,30 @@ adsr_status adsr_candidates_from_arrays(
     const uint64_t *cycle_center, const uint64_t *cycle_start,
     const uint64_t *cycle_stop, adsr_candidates **out);
 void adsr_candidates_free(adsr_candidates *c);
+
+/* ---- candidate table in HBM (handoff to scoring without a host copy) --- */
+/* PeakGroupSelection::search (src/peak_group_selection/mod.rs:87-128) with
+ * the candidate table left in device memory: the same rows, in the same
+ * order, as adsr_select.  Returns once the table is complete (its length is
+ * known on the host).  The table does not reference the library handle.  */
+adsr_status adsr_select_device(const adsr_dia *d, const adsr_lib *l,
+                               const adsr_selection_params *p,
+                               adsr_device_candidates **out);
+size_t adsr_device_candidates_len(const adsr_device_candidates *c);
+/* CandidateCollection::to_arrays (src/candidate/entry.rs:124-173) into
+ * device memory: each pointer is NULL (column skipped) or a device buffer of
+ * len elements (u64, score f32).  One kernel is enqueued on `stream` (a
+ * hipStream_t; NULL is the null stream) and the call returns without
+ * waiting for it.                                                          */
+adsr_status adsr_device_candidates_export(
+    const adsr_device_candidates *c, uint64_t *precursor_idx, uint64_t *rank,
+    float *score, uint64_t *scan_center, uint64_t *scan_start,
+    uint64_t *scan_stop, uint64_t *cycle_center, uint64_t *cycle_start,
+    uint64_t *cycle_stop, void *stream);
+/* The host CandidateCollection of the same table. */
+adsr_status adsr_device_candidates_to_host(const adsr_device_candidates *c,
+                                           adsr_candidates **out);
+void adsr_device_candidates_free(adsr_device_candidates *c);
 
 /* ---- per-stage device kernels (SimdBackend ops, src/simd/mod.rs:11-39) --
  * Host in/out buffers; the library stages them through HBM.               */
