@@ -109,6 +109,25 @@ def measured_traffic(workload):
     return None
 
 
+def measured_valu(workload):
+    """select_kernel VALU figures from the committed counter summary
+    (VALU busy fraction, VALU instructions per precursor, lane utilization,
+    VALU-issue floor in ms); None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", f"select_workload{workload}_valu.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if (len(parts) == 7 and parts[0] == "valu_summary"
+                        and parts[1] == f"workload={workload}"
+                        and parts[2] == "kernel=select_kernel"):
+                    kv = dict(p.split("=") for p in parts[3:])
+                    return {k: float(v) for k, v in kv.items()}
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def copy_probe(mib):
     """A plain copy of known size, for scaling HBM byte counters."""
     import torch
@@ -297,6 +316,11 @@ def main():
             "traffic": measured_traffic(args.workload),
         },
     }
+    valu = measured_valu(args.workload)
+    if valu is not None:
+        # the kernel's binding limit: VALU issue under the reference's exact
+        # f32 operation order (counter summary under profiles/)
+        result["config"]["select_kernel_valu"] = valu
 
     build = {"device_wall_s": round(dia_wall, 3),
              **{k: round(v, 3) for k, v in bt.items()}}
