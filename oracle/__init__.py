@@ -87,6 +87,32 @@ def lib():
         sig("orc_select_one", C.c_int, C.POINTER(OrcDia), vp, C.POINTER(OrcParams), sz,
             C.POINTER(sz), C.POINTER(sz), C.POINTER(sz), vp, vp, vp, vp, vp, sz)
         sig("orc_logf", f32, f32)
+        sig("orc_lib_build_full", vp, sz, vp, vp, vp, vp, vp, vp, sz, vp, vp, vp, vp, vp)
+        u8 = C.c_uint8
+        sig("orc_correlation_safe", f32, vp, vp, sz)
+        sig("orc_gamma_ln", f32, f32)
+        sig("orc_hyperscore_weighted", f32, vp, vp, vp, vp, sz)
+        sig("orc_longest_ion_series", None, vp, vp, vp, sz, C.POINTER(u8), C.POINTER(u8))
+        sig("orc_intensity_ion_series", f32, vp, vp, vp, sz, u8)
+        sig("orc_dot", f32, vp, vp, sz)
+        sig("orc_fwhm_rt", f32, vp, sz, C.c_uint64, vp)
+        sig("orc_normalize_profiles", None, vp, sz, sz, sz, vp)
+        sig("orc_filter_non_zero", sz, vp, sz, sz, vp)
+        sig("orc_median_axis_0", None, vp, sz, sz, vp)
+        sig("orc_median", f32, vp, sz)
+        sig("orc_std", f32, vp, sz)
+        sig("orc_ranked_mask", None, vp, sz, sz, sz, vp)
+        sig("orc_count_above", sz, vp, sz, f32, vp)
+        sig("orc_weighted_mae", f32, vp, vp, sz)
+        sig("orc_fragment_mz_errors", None, vp, vp, sz, sz, vp, vp, vp)
+        sig("orc_fill_xic_and_mz_slice", None, C.POINTER(OrcDia), sz, vp, vp,
+            C.c_uint64, C.c_uint64, f32, f32)
+        sig("orc_dense_xic_mz", None, C.POINTER(OrcDia), f32, C.c_uint64, C.c_uint64, f32,
+            vp, sz, vp, vp)
+        sig("orc_idf_counts", C.POINTER(C.c_uint32), vp, sz)
+        sig("orc_idf", f32, vp, f32, f32)
+        sig("orc_score", None, C.POINTER(OrcDia), vp, f32, sz, sz, vp, vp, vp, vp, vp,
+            vp, vp, C.c_int)
         _lib = L
     return _lib
 
@@ -157,6 +183,16 @@ class Dia:
         n = lib().orc_valid_observations(self.h, float(mz), p(out))
         return [int(x) for x in out[:n]]
 
+    def dense_xic_mz(self, precursor_mz, c0, c1, tol, frag_mz):
+        """DenseXICMZObservation::new: (xic, mz), each [n_frag][c1 - c0]."""
+        fm = c32(frag_mz)
+        L = (int(c1) - int(c0)) % (1 << 64)
+        xic = np.zeros((len(fm), L), dtype=np.float32)
+        mz = np.zeros((len(fm), L), dtype=np.float32)
+        lib().orc_dense_xic_mz(self.h, float(precursor_mz), int(c0) % (1 << 64), int(c1),
+                               float(tol), p(fm), len(fm), p(xic), p(mz))
+        return xic, mz
+
     def dense_xic(self, precursor_mz, c0, c1, tol, frag_mz):
         fm = c32(frag_mz)
         out = np.zeros((len(fm), c1 - c0), dtype=np.float32)
@@ -166,11 +202,13 @@ class Dia:
 
 class Lib:
     def __init__(self, arrays):
-        (pidx, _pmzl, pmz, _prtl, prt, _naa, fs, fe, _fmzl, fmz, fint,
+        (pidx, _pmzl, pmz, _prtl, prt, naa, fs, fe, fmzl, fmz, fint,
          _card, _chg, _loss, fnum, _pos, ftype) = arrays
-        k = [np.ascontiguousarray(a) for a in (pidx, pmz, prt, fs, fe, fmz, fint, fnum, ftype)]
-        self.h = lib().orc_lib_build(len(k[0]), p(k[0]), p(k[1]), p(k[2]), p(k[3]), p(k[4]),
-                                     len(k[5]), p(k[5]), p(k[6]), p(k[7]), p(k[8]))
+        k = [np.ascontiguousarray(a) for a in (pidx, pmz, prt, naa, fs, fe, fmz, fmzl,
+                                               fint, fnum, ftype)]
+        self.h = lib().orc_lib_build_full(len(k[0]), p(k[0]), p(k[1]), p(k[2]), p(k[3]),
+                                          p(k[4]), p(k[5]), len(k[6]), p(k[6]), p(k[7]),
+                                          p(k[8]), p(k[9]), p(k[10]))
         self.n_prec = len(k[0])
 
     def __del__(self):
@@ -280,3 +318,46 @@ def find_local_maxima(a, offset):
     val = np.zeros(max(len(a), 1), np.float32)
     n = lib().orc_find_local_maxima(p(a), len(a), offset, p(idx), p(val))
     return idx[:n].astype(np.int64), val[:n].copy()
+
+
+# CandidateFeature f32 columns in to_dict_arrays order
+# (src/candidate/features.rs:220-410), after precursor_idx and rank
+FEATURES = [
+    "score", "mean_correlation", "median_correlation", "correlation_std",
+    "intensity_correlation", "num_fragments", "num_scans", "num_over_95",
+    "num_over_90", "num_over_80", "num_over_50", "num_over_0",
+    "num_over_0_rank_0_5", "num_over_0_rank_6_11", "num_over_0_rank_12_17",
+    "num_over_0_rank_18_23", "num_over_50_rank_0_5", "num_over_50_rank_6_11",
+    "num_over_50_rank_12_17", "num_over_50_rank_18_23",
+    "hyperscore_intensity_observation", "hyperscore_intensity_library",
+    "hyperscore_inverse_mass_error", "rt_observed", "delta_rt",
+    "longest_b_series", "longest_y_series", "naa", "weighted_mass_error",
+    "log10_b_ion_intensity", "log10_y_ion_intensity", "fwhm_rt",
+    "idf_hyperscore", "idf_xic_dot_product", "idf_intensity_dot_product",
+    "median_profile_sum", "median_profile_sum_filtered", "num_profiles",
+    "num_profiles_filtered", "num_over_0_top6_idf", "num_over_50_top6_idf",
+]
+
+
+def score(dia: Dia, library: Lib, candidates, mass_tolerance=7.0, top_k_fragments=100,
+          nthreads=0):
+    """PeakGroupScoring.score of a candidate table (the 9 to_arrays columns).
+    Returns a dict of the 43 to_dict_arrays columns, rows whose precursor is
+    not in the library dropped."""
+    pidx, rank, sc, _s0, _s1, _s2, cc, cs, ce = candidates
+    n = len(pidx)
+    keep = [np.ascontiguousarray(pidx, np.uint64), c32(sc),
+            np.ascontiguousarray(cs, np.uint64), np.ascontiguousarray(ce, np.uint64),
+            np.ascontiguousarray(cc, np.uint64)]
+    feat = np.zeros((len(FEATURES), max(n, 1)), np.float32)
+    found = np.zeros(max(n, 1), np.uint8)
+    nt = nthreads or (os.cpu_count() or 1)
+    lib().orc_score(dia.h, library.h, float(mass_tolerance), int(top_k_fragments), n,
+                    p(keep[0]), p(keep[1]), p(keep[2]), p(keep[3]), p(keep[4]),
+                    p(feat), p(found), nt)
+    m = found[:n].astype(bool)
+    out = {"precursor_idx": np.asarray(pidx, np.uint64)[m],
+           "rank": np.asarray(rank, np.uint64)[m]}
+    for i, name in enumerate(FEATURES):
+        out[name] = feat[i, :n][m].copy()
+    return out

@@ -665,6 +665,16 @@ orc_lib *orc_lib_build(size_t np, const uint64_t *pidx, const float *pmz,
                        const uint64_t *fstop, size_t nfrag, const float *fmz,
                        const float *fint, const uint8_t *fnum,
                        const uint8_t *ftype) {
+  return orc_lib_build_full(np, pidx, pmz, prt, NULL, fstart, fstop, nfrag,
+                            fmz, NULL, fint, fnum, ftype);
+}
+
+orc_lib *orc_lib_build_full(size_t np, const uint64_t *pidx, const float *pmz,
+                            const float *prt, const uint8_t *naa,
+                            const uint64_t *fstart, const uint64_t *fstop,
+                            size_t nfrag, const float *fmz, const float *fmzl,
+                            const float *fint, const uint8_t *fnum,
+                            const uint8_t *ftype) {
   orc_lib *l = (orc_lib *)calloc(1, sizeof(orc_lib));
   l->n_prec = np;
   l->n_frag = nfrag;
@@ -680,12 +690,14 @@ orc_lib *orc_lib_build(size_t np, const uint64_t *pidx, const float *pmz,
   l->precursor_rt = (float *)malloc(sz * 4);
   l->frag_start = (uint64_t *)malloc(sz * 8);
   l->frag_stop = (uint64_t *)malloc(sz * 8);
+  if (naa) l->precursor_naa = (uint8_t *)malloc(sz);
   for (size_t i = 0; i < np; i++) { /* :145-163 */
     l->precursor_idx[i] = pidx[ord[i]];
     l->precursor_mz[i] = pmz[ord[i]];
     l->precursor_rt[i] = prt[ord[i]];
     l->frag_start[i] = fstart[ord[i]];
     l->frag_stop[i] = fstop[ord[i]];
+    if (naa) l->precursor_naa[i] = naa[ord[i]];
   }
   free(ord);
   size_t fz = nfrag ? nfrag : 1; /* fragment arrays keep their order */
@@ -693,6 +705,10 @@ orc_lib *orc_lib_build(size_t np, const uint64_t *pidx, const float *pmz,
   l->fragment_intensity = (float *)malloc(fz * 4);
   l->fragment_number = (uint8_t *)malloc(fz);
   l->fragment_type = (uint8_t *)malloc(fz);
+  if (fmzl) {
+    l->fragment_mz_library = (float *)malloc(fz * 4);
+    if (nfrag) memcpy(l->fragment_mz_library, fmzl, nfrag * 4);
+  }
   if (nfrag) {
     memcpy(l->fragment_mz, fmz, nfrag * 4);
     memcpy(l->fragment_intensity, fint, nfrag * 4);
@@ -704,6 +720,8 @@ orc_lib *orc_lib_build(size_t np, const uint64_t *pidx, const float *pmz,
 
 void orc_lib_free(orc_lib *l) {
   if (!l) return;
+  free(l->precursor_naa);
+  free(l->fragment_mz_library);
   free(l->precursor_idx);
   free(l->precursor_mz);
   free(l->precursor_rt);

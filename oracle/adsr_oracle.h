@@ -151,6 +151,8 @@ typedef struct orc_lib {
   uint64_t *frag_start, *frag_stop;
   float *fragment_mz, *fragment_intensity;
   uint8_t *fragment_number, *fragment_type;
+  uint8_t *precursor_naa;      /* sorted order; scoring only (may be NULL) */
+  float *fragment_mz_library;  /* fragment order; scoring only (may be NULL) */
 } orc_lib;
 
 orc_lib *orc_lib_build(size_t n_prec, const uint64_t *precursor_idx,
@@ -161,6 +163,18 @@ orc_lib *orc_lib_build(size_t n_prec, const uint64_t *precursor_idx,
                        const float *fragment_intensity,
                        const uint8_t *fragment_number,
                        const uint8_t *fragment_type);
+/* The same with the two columns scoring reads as well. */
+orc_lib *orc_lib_build_full(size_t n_prec, const uint64_t *precursor_idx,
+                            const float *precursor_mz,
+                            const float *precursor_rt,
+                            const uint8_t *precursor_naa,
+                            const uint64_t *flat_frag_start_idx,
+                            const uint64_t *flat_frag_stop_idx, size_t n_frag,
+                            const float *fragment_mz,
+                            const float *fragment_mz_library,
+                            const float *fragment_intensity,
+                            const uint8_t *fragment_number,
+                            const uint8_t *fragment_type);
 void orc_lib_free(orc_lib *l);
 
 /* ---- selection: src/peak_group_selection/mod.rs:87-195 -------------- */
@@ -195,6 +209,75 @@ int orc_select_one(const orc_dia *d, const orc_lib *l, const orc_params *p,
                    float *conv, float *score, size_t cap_cells);
 
 float orc_logf(float x); /* glibc logf, as called by the reference */
+
+/* ---- scoring: src/peak_group_scoring (adsr_oracle_score.c) ---------- */
+/* CandidateFeature f32 columns (src/candidate/features.rs:51-96, to_dict_
+ * arrays order after precursor_idx and rank) */
+enum {
+  ORC_F_SCORE, ORC_F_MEAN_CORRELATION, ORC_F_MEDIAN_CORRELATION,
+  ORC_F_CORRELATION_STD, ORC_F_INTENSITY_CORRELATION, ORC_F_NUM_FRAGMENTS,
+  ORC_F_NUM_SCANS, ORC_F_NUM_OVER_95, ORC_F_NUM_OVER_90, ORC_F_NUM_OVER_80,
+  ORC_F_NUM_OVER_50, ORC_F_NUM_OVER_0, ORC_F_NUM_OVER_0_RANK_0_5,
+  ORC_F_NUM_OVER_0_RANK_6_11, ORC_F_NUM_OVER_0_RANK_12_17,
+  ORC_F_NUM_OVER_0_RANK_18_23, ORC_F_NUM_OVER_50_RANK_0_5,
+  ORC_F_NUM_OVER_50_RANK_6_11, ORC_F_NUM_OVER_50_RANK_12_17,
+  ORC_F_NUM_OVER_50_RANK_18_23, ORC_F_HYPERSCORE_INTENSITY_OBSERVATION,
+  ORC_F_HYPERSCORE_INTENSITY_LIBRARY, ORC_F_HYPERSCORE_INVERSE_MASS_ERROR,
+  ORC_F_RT_OBSERVED, ORC_F_DELTA_RT, ORC_F_LONGEST_B_SERIES,
+  ORC_F_LONGEST_Y_SERIES, ORC_F_NAA, ORC_F_WEIGHTED_MASS_ERROR,
+  ORC_F_LOG10_B_ION_INTENSITY, ORC_F_LOG10_Y_ION_INTENSITY, ORC_F_FWHM_RT,
+  ORC_F_IDF_HYPERSCORE, ORC_F_IDF_XIC_DOT_PRODUCT,
+  ORC_F_IDF_INTENSITY_DOT_PRODUCT, ORC_F_MEDIAN_PROFILE_SUM,
+  ORC_F_MEDIAN_PROFILE_SUM_FILTERED, ORC_F_NUM_PROFILES,
+  ORC_F_NUM_PROFILES_FILTERED, ORC_F_NUM_OVER_0_TOP6_IDF,
+  ORC_F_NUM_OVER_50_TOP6_IDF, ORC_N_FEATURES
+};
+
+float orc_correlation_safe(const float *x, const float *y, size_t n);
+float orc_gamma_ln(float x);
+float orc_hyperscore_weighted(const uint8_t *types, const float *inten,
+                              const uint8_t *matched, const float *w,
+                              size_t n);
+void orc_longest_ion_series(const uint8_t *types, const uint8_t *nums,
+                            const uint8_t *matched, size_t n, uint8_t *b,
+                            uint8_t *y);
+float orc_intensity_ion_series(const uint8_t *types, const float *inten,
+                               const uint8_t *matched, size_t n,
+                               uint8_t target);
+float orc_dot(const float *a, const float *b, size_t n);
+float orc_fwhm_rt(const float *profile, size_t n, uint64_t cycle_start,
+                  const float *rt);
+void orc_normalize_profiles(const float *x, size_t rows, size_t cols,
+                            size_t dil, float *out);
+size_t orc_filter_non_zero(const float *x, size_t rows, size_t cols,
+                           float *out);
+void orc_median_axis_0(const float *x, size_t rows, size_t cols, float *out);
+float orc_median(const float *v, size_t n);
+float orc_std(const float *v, size_t n);
+void orc_ranked_mask(const float *v, size_t n, size_t r1, size_t r2,
+                     uint8_t *mask);
+size_t orc_count_above(const float *v, size_t n, float thr,
+                       const uint8_t *mask);
+float orc_weighted_mae(const float *err, const float *w, size_t n);
+void orc_fragment_mz_errors(const float *dmz, const float *dxic, size_t rows,
+                            size_t cols, const float *mz_lib, float *mz_obs,
+                            float *err);
+void orc_fill_xic_and_mz_slice(const orc_dia *d, size_t obs_idx, float *xic,
+                               float *dmz, uint64_t c0, uint64_t c1,
+                               float tol, float mz);
+void orc_dense_xic_mz(const orc_dia *d, float precursor_mz, uint64_t c0,
+                      uint64_t c1, float tol, const float *fragment_mz,
+                      size_t nf, float *xic, float *mz);
+uint32_t *orc_idf_counts(const float *fragment_mz_library, size_t n);
+float orc_idf(const uint32_t *counts, float total, float mz);
+/* PeakGroupScoring.score over n candidates: features [ORC_N_FEATURES][n]
+ * (column-major), found[i] = 0 where precursor_idx is not in the library.
+ * The library must come from orc_lib_build_full. */
+void orc_score(const orc_dia *d, const orc_lib *l, float mass_tolerance,
+               size_t top_k, size_t n, const uint64_t *precursor_idx,
+               const float *score, const uint64_t *cycle_start,
+               const uint64_t *cycle_stop, const uint64_t *cycle_center,
+               float *features, uint8_t *found, int nthreads);
 
 #ifdef __cplusplus
 }

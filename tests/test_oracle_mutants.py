@@ -14,6 +14,7 @@ import test_oracle as T
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(os.path.dirname(HERE), "oracle", "adsr_oracle.c")
+SRC_SCORE = os.path.join(os.path.dirname(HERE), "oracle", "adsr_oracle_score.c")
 
 # planted mistake -> (what, the checks that pin it)
 MUTANTS = {
@@ -40,7 +41,8 @@ def builds(tmp_path_factory):
         so = str(out / f"liboracle_m{m}.so")
         subprocess.run(
             ["gcc", "-O1", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-std=c11",
-             "-shared", f"-DORC_MUTANT={m}", "-o", so, SRC, "-lm", "-lpthread"],
+             "-shared", f"-DORC_MUTANT={m}", "-o", so, SRC, SRC_SCORE, "-lm",
+             "-lpthread"],
             check=True)
         paths[m] = so
     return paths
