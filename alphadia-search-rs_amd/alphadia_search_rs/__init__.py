@@ -27,6 +27,9 @@ __all__ = [
     "PeakGroupSelection",
     "CandidateCollection",
     "DeviceCandidateCollection",
+    "ScoringParameters",
+    "PeakGroupScoring",
+    "CandidateFeatureCollection",
     "get_optimal_simd_backend",
     "set_simd_backend",
     "clear_simd_backend",
@@ -580,4 +583,125 @@ class PeakGroupSelection:
         h = C.c_void_p()
         check(N.lib.adsr_select_device(dia_data._h, lib._h, C.byref(c), C.byref(h)))
         out._h = h.value
+        return out
+
+
+# ---------------------------------------------------------------------------
+# PeakGroupScoring (src/peak_group_scoring/mod.rs:30-102)
+# ---------------------------------------------------------------------------
+class ScoringParameters:
+    """src/peak_group_scoring/parameters.rs:4-40: read-only attributes with
+    the reference defaults, changed through ``update(dict)``."""
+
+    def __init__(self):
+        object.__setattr__(self, "_v", {"mass_tolerance": 7.0, "top_k_fragments": 100})
+
+    def __getattr__(self, name):
+        v = object.__getattribute__(self, "_v")
+        if name in v:
+            return v[name]
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        raise AttributeError(
+            f"attribute '{name}' of 'ScoringParameters' objects is not writable")
+
+    def update(self, config: dict) -> None:
+        if not isinstance(config, dict):
+            raise TypeError("update() expects a dict")
+        v = object.__getattribute__(self, "_v")
+        new = dict(v)
+        if "mass_tolerance" in config:
+            new["mass_tolerance"] = _extract_f32("mass_tolerance", config["mass_tolerance"])
+        if "top_k_fragments" in config:
+            new["top_k_fragments"] = _extract_usize("top_k_fragments",
+                                                    config["top_k_fragments"])
+        v.update(new)
+
+    def _c(self) -> N.ScoringParamsC:
+        v = object.__getattribute__(self, "_v")
+        return N.ScoringParamsC(v["mass_tolerance"], v["top_k_fragments"])
+
+
+FEATURE_NAMES = [N.lib.adsr_feature_name(i).decode() for i in range(41)]
+
+
+class CandidateFeatureCollection:
+    """src/candidate/features.rs:196-430: one row of features per scored
+    candidate."""
+
+    def __init__(self):
+        self._h = None
+
+    def __del__(self, _free=N.lib.adsr_features_free):
+        h = getattr(self, "_h", None)
+        if h:
+            _free(h)
+            self._h = None
+
+    def len(self) -> int:
+        return int(N.lib.adsr_features_len(self._h)) if self._h else 0
+
+    def __len__(self) -> int:
+        return self.len()
+
+    def is_empty(self) -> bool:
+        return self.len() == 0
+
+    def to_dict_arrays(self) -> dict:
+        """features.rs:220-410: precursor_idx and rank (uint64) and the 41
+        float32 feature columns, keyed by name."""
+        n = self.len()
+        pidx = np.zeros(n, np.uint64)
+        rank = np.zeros(n, np.uint64)
+        cols = np.zeros((len(FEATURE_NAMES), n), np.float32)
+        if self._h and n:
+            check(N.lib.adsr_features_export(self._h, ptr(pidx), ptr(rank), ptr(cols)))
+        out = {"precursor_idx": pidx, "rank": rank}
+        for i, name in enumerate(FEATURE_NAMES):
+            out[name] = cols[i].copy()
+        return out
+
+    def device_timings(self) -> dict:
+        """Device time of the fragment plan and of the scoring kernel (ms)."""
+        a, b = C.c_double(), C.c_double()
+        check(N.lib.adsr_features_timings(self._h, C.byref(a), C.byref(b)))
+        return {"plan_ms": a.value, "kernel_ms": b.value}
+
+
+class PeakGroupScoring:
+    def __init__(self, params: ScoringParameters):
+        if not isinstance(params, ScoringParameters):
+            raise TypeError("PeakGroupScoring expects ScoringParameters")
+        p = ScoringParameters()
+        object.__getattribute__(p, "_v").update(dict(object.__getattribute__(params, "_v")))
+        self._params = p
+
+    def score(self, dia_data: DIAData, lib: SpecLibFlat,
+              candidates) -> CandidateFeatureCollection:
+        """mod.rs:43-102 on the gfx950 device.  ``candidates`` is a
+        CandidateCollection or a DeviceCandidateCollection (the table left
+        in HBM by search_device)."""
+        if not isinstance(dia_data, DIAData) or not isinstance(lib, SpecLibFlat):
+            raise TypeError("score(dia_data: DIAData, lib: SpecLibFlat, candidates)")
+        out = CandidateFeatureCollection()
+        if not dia_data._h or not lib._h or not candidates._h:
+            return out
+        c = self._params._c()
+        h = C.c_void_p()
+        t0 = time.perf_counter()
+        if isinstance(candidates, DeviceCandidateCollection):
+            check(N.lib.adsr_score_device(dia_data._h, lib._h, candidates._h, C.byref(c),
+                                          C.byref(h)))
+        elif isinstance(candidates, CandidateCollection):
+            check(N.lib.adsr_score(dia_data._h, lib._h, candidates._h, C.byref(c), C.byref(h)))
+        else:
+            raise TypeError("candidates must be a CandidateCollection")
+        dt = time.perf_counter() - t0
+        out._h = h.value
+        if os.environ.get("ADSR_QUIET", "") in ("", "0"):
+            # mod.rs:94-99
+            rate = len(candidates) / dt if dt > 0 else float("inf")
+            print(f"Scored {len(candidates)} candidates at {rate:.2f} candidates/second")
+            sys.stdout.flush()
         return out

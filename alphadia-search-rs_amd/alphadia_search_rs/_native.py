@@ -46,6 +46,12 @@ ADSR_ERR_OUT_OF_MEMORY = 3
 ADSR_ERR_UNSUPPORTED = 4
 
 
+class ScoringParamsC(C.Structure):
+    """struct adsr_scoring_params (include/adsr.h)."""
+
+    _fields_ = [("mass_tolerance", c_f32), ("top_k_fragments", c_u64)]
+
+
 class SelectionParamsC(C.Structure):
     """struct adsr_selection_params (include/adsr.h)."""
 
@@ -123,6 +129,14 @@ _sig("adsr_candidates_from_arrays", I, c_size, vp, vp, vp, vp, vp, vp, vp, vp,
      vp, C.POINTER(vp))
 _sig("adsr_candidates_free", None, vp)
 _sig("adsr_select_device", I, vp, vp, C.POINTER(SelectionParamsC), C.POINTER(vp))
+_sig("adsr_feature_name", C.c_char_p, c_size)
+_sig("adsr_scoring_params_default", None, C.POINTER(ScoringParamsC))
+_sig("adsr_score", I, vp, vp, vp, C.POINTER(ScoringParamsC), C.POINTER(vp))
+_sig("adsr_score_device", I, vp, vp, vp, C.POINTER(ScoringParamsC), C.POINTER(vp))
+_sig("adsr_features_len", c_size, vp)
+_sig("adsr_features_export", I, vp, vp, vp, vp)
+_sig("adsr_features_timings", I, vp, C.POINTER(C.c_double), C.POINTER(C.c_double))
+_sig("adsr_features_free", None, vp)
 _sig("adsr_device_candidates_len", c_size, vp)
 _sig("adsr_device_candidates_export", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
 _sig("adsr_device_candidates_to_host", I, vp, C.POINTER(vp))

@@ -1994,6 +1994,9 @@ void device_lib_free(DeviceLib &l) {
   (void)hipFree(l.f_int);
   (void)hipFree(l.f_type);
   (void)hipFree(l.f_num);
+  (void)hipFree(l.p_naa);
+  (void)hipFree(l.f_mzl);
+  (void)hipFree(l.idf_counts);
   l = DeviceLib();
 }
 
@@ -4014,3 +4017,813 @@ adsr_status device_dia_window(const adsr_dia *d, size_t o, uint32_t *ss,
   return ADSR_OK;
 }
 }  // namespace adsr
+
+// ===========================================================================
+// PeakGroupScoring.score: src/peak_group_scoring/mod.rs:43-341
+// ===========================================================================
+// One wave per candidate, lane f owning selected fragment f (the m/z-sorted
+// output of get_precursor_filtered with the scoring top-k, from the same
+// plan pass as selection).  Per-fragment work (extraction, normalisation,
+// correlation, m/z) runs per lane; reductions over fragments run as loops
+// that every lane executes alike over LDS copies, in fragment order, so the
+// f32 sums keep the reference's order.  Features are written column-major.
+
+constexpr int kScoreLMax = 32;  // cycles per candidate (2 * peak_length + 1)
+constexpr int kFragB = 98, kFragY = 121;  // src/constants.rs FragmentType
+
+struct ScoreArgs {
+  KDia dia;
+  KLib lib;
+  KPlan plan;
+  const uint64_t *p_idx;  // sorted precursor_idx
+  const uint8_t *p_naa;
+  const float *f_mzl;
+  const uint32_t *idf_counts;
+  float idf_total;
+  float mass_tol;
+  uint32_t n;
+  const uint64_t *c_pidx, *c_start, *c_stop, *c_center;
+  const float *c_score;
+  float *feat;  // [ADSR_N_FEATURES][n]
+  uint8_t *found;
+  uint32_t *status;
+  uint32_t lds_per_wave;
+};
+
+constexpr uint32_t kStScoreWindowTooLong = 1u << 8;
+constexpr uint32_t kStScoreRtIndex = 1u << 9;  // the reference panics there
+
+// Rust 1.88 slice::binary_search over u16: the last equal element or the
+// insertion point (src/quadrupole_observation/mod.rs:175-178).
+__device__ __forceinline__ uint32_t rust_bsearch_u16(const uint16_t *v, uint32_t n,
+                                                     uint16_t key) {
+  if (n == 0) return 0;
+  uint32_t size = n, base = 0;
+  while (size > 1) {
+    const uint32_t half = size >> 1, mid = base + half;
+    if (!(v[mid] > key)) base = mid;
+    size -= half;
+  }
+  const uint16_t b = v[base];
+  if (b == key) return base;
+  return base + (b < key ? 1u : 0u);
+}
+
+// ndarray 0.16 sum() of a contiguous lane: eight partial sums, combined
+// pairwise, then the tail in order (sum_axis(Axis(1)), mod.rs:154)
+__device__ __forceinline__ float ndarray_sum(const float *x, uint32_t n) {
+  float p[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  uint32_t i = 0;
+  for (; i + 8 <= n; i += 8)
+    for (int k = 0; k < 8; ++k) p[k] = p[k] + x[i + k];
+  float acc = 0.0f;
+  acc = acc + (p[0] + p[4]);
+  acc = acc + (p[1] + p[5]);
+  acc = acc + (p[2] + p[6]);
+  acc = acc + (p[3] + p[7]);
+  for (; i < n; ++i) acc = acc + x[i];
+  return acc;
+}
+
+// src/peak_group_scoring/utils.rs:126-172 over x[i * sx], y[i * sy]
+__device__ float corr_safe(const float *x, const float *y, uint32_t n) {
+  if (n == 0) return 0.0f;
+  float xs = -0.0f, ys = -0.0f;  // Iterator::sum folds from -0.0
+  for (uint32_t i = 0; i < n; ++i) {
+    xs = xs + x[i];
+    ys = ys + y[i];
+  }
+  if (xs == 0.0f || ys == 0.0f) return 0.0f;
+  const float xm = xs / (float)n, ym = ys / (float)n;
+  float xv = 0.0f, yv = 0.0f, cov = 0.0f;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float dx = x[i] - xm, dy = y[i] - ym;
+    xv = xv + dx * dx;
+    yv = yv + dy * dy;
+    cov = cov + dx * dy;
+  }
+  if (xv == 0.0f || yv == 0.0f) return 0.0f;
+  const float c = cov / (sqrtf(xv) * sqrtf(yv));
+  return fminf(fmaxf(c, -1.0f), 1.0f);
+}
+
+// src/peak_group_scoring/utils.rs:293-310
+__device__ float gamma_ln_dev(float x, const LogfEntry *ltab) {
+  if (x <= 0.0f) return 0.0f;
+  if (fabsf(x - 1.0f) < 1e-6f) return 0.0f;
+  if (fabsf(x - 2.0f) < 1e-6f) return 0.0f;
+  const float ln_2pi = 1.837877f;
+  return (x - 0.5f) * logf_exact(x, ltab) - x + 0.5f * ln_2pi;
+}
+
+// src/peak_group_scoring/utils.rs:193-279 over LDS columns (w == nullptr:
+// weight 1); every lane computes the same value
+__device__ float hyperscore_dev(const uint32_t *tn, const float *inten,
+                                const uint32_t *matched, const float *w,
+                                uint32_t n, const LogfEntry *ltab) {
+  uint32_t nb = 0, ny = 0;
+  float sb = 0.0f, sy = 0.0f;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!matched[i] || inten[i] == 0.0f) continue;
+    const float v = inten[i] * (w ? w[i] : 1.0f);
+    const uint32_t t = tn[i] & 0xffu;
+    if (t == (uint32_t)kFragB) {
+      ++nb;
+      sb = sb + v;
+    } else if (t == (uint32_t)kFragY) {
+      ++ny;
+      sy = sy + v;
+    }
+  }
+  if (nb == 0 && ny == 0) return 0.0f;
+  const float fb = nb > 0 ? gamma_ln_dev((float)nb + 1.0f, ltab) : 0.0f;
+  const float fy = ny > 0 ? gamma_ln_dev((float)ny + 1.0f, ltab) : 0.0f;
+  const float lb = sb > 0.0f ? logf_exact(sb, ltab) : 0.0f;
+  const float ly = sy > 0.0f ? logf_exact(sy, ltab) : 0.0f;
+  const float h = fb + fy + lb + ly;
+  return isfinite(h) ? h : 0.0f;
+}
+
+// Bin of an m/z (src/mz_index/mod.rs:55-90)
+__device__ __forceinline__ uint32_t closest_bin(const KDia &dia, float x) {
+  if (!(x > dia.grid_first)) return 0;
+  if (x > dia.grid_last) return dia.n_grid - 1;
+  return closest_from(dia, x,
+                      dia.bucket1[(__float_as_uint(x) >> kBucketShift) - dia.base_key]);
+}
+
+// src/idf/mod.rs:13-33: fragments per bin over every fragment_mz_library
+__global__ __launch_bounds__(256) void idf_count_kernel(KDia dia, const float *mzl,
+                                                        uint64_t n, uint32_t *counts) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&counts[closest_bin(dia, mzl[i])], 1u);
+}
+
+struct ScoreLds {
+  float *xic, *mz, *norm, *sortA, *sortF;  // [ks][L] rows / [L][ks] columns
+  float *oint, *corr, *fint, *idf, *err, *wie, *buf;  // [ks]
+  uint32_t *tn, *matched, *nz;                         // [ks]
+  float *med, *medf;                                   // [kScoreLMax]
+};
+
+__host__ __device__ inline uint32_t score_lds_bytes(uint32_t ks, uint32_t L) {
+  return (5 * ks * L + 10 * ks + 2 * kScoreLMax) * 4;
+}
+
+__global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ LogfEntry ltab[16];
+  if (threadIdx.x < 16) ltab[threadIdx.x] = kDevLogfTab[threadIdx.x];
+  __syncthreads();
+  const int lane = lane_id();
+  const uint32_t wv = threadIdx.x / kWave;
+  const uint32_t ci = uni(blockIdx.x * (blockDim.x / kWave) + wv);
+  if (ci >= a.n) return;  // no block barrier follows
+  const uint32_t ks = a.plan.ks;
+  const uint64_t n_o = a.n;
+  auto F_OUT = [&](int k, float v) {
+    if (lane == 0) a.feat[(size_t)k * n_o + ci] = v;
+  };
+  // speclib_flat/mod.rs:458-475 get_precursor_by_idx_filtered: binary
+  // search of the sorted precursor_idx
+  const uint64_t key = a.c_pidx[ci];
+  uint32_t lo = 0, hi = a.lib.n_prec;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.p_idx[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  lo = uni(lo);
+  if (lo >= a.lib.n_prec || a.p_idx[lo] != key) {  // filter_map -> None
+    if (lane == 0) a.found[ci] = 0;
+    return;
+  }
+  const uint32_t row = lo;
+  const uint64_t c0 = a.c_start[ci], c1 = a.c_stop[ci];
+  const uint64_t L64 = c1 - c0;  // usize arithmetic wraps (release build)
+  if (L64 > (uint64_t)kScoreLMax) {
+    if (lane == 0) {
+      a.found[ci] = 0;
+      atomicOr(a.status, kStScoreWindowTooLong);
+    }
+    return;
+  }
+  const uint32_t L = (uint32_t)L64;
+  const uint32_t F = uni(a.plan.rec[row].z & 0xffffu);
+  // LDS of this wave
+  char *base = smem + (size_t)wv * a.lds_per_wave;
+  ScoreLds s;
+  {
+    float *p = reinterpret_cast<float *>(base);
+    s.xic = p; p += ks * L;
+    s.mz = p; p += ks * L;
+    s.norm = p; p += ks * L;
+    s.sortA = p; p += ks * L;
+    s.sortF = p; p += ks * L;
+    s.oint = p; p += ks;
+    s.corr = p; p += ks;
+    s.fint = p; p += ks;
+    s.idf = p; p += ks;
+    s.err = p; p += ks;
+    s.wie = p; p += ks;
+    s.buf = p; p += ks;
+    s.tn = reinterpret_cast<uint32_t *>(p); p += ks;
+    s.matched = reinterpret_cast<uint32_t *>(p); p += ks;
+    s.nz = reinterpret_cast<uint32_t *>(p); p += ks;
+    s.med = p; p += kScoreLMax;
+    s.medf = p;
+  }
+  const bool mine = (uint32_t)lane < F;
+  const uint32_t f = (uint32_t)lane;
+  float fmz = 0.0f, fint = 0.0f, fmzl = 0.0f;
+  uint32_t ftype = 0, fnum = 0;
+  if (mine) {
+    const size_t so = (size_t)row * ks + f;
+    const uint64_t j = a.lib.f_start[row] + a.plan.sel[so];
+    fmz = a.plan.sel_mz[so];
+    fint = a.plan.sel_w[so];
+    ftype = a.lib.f_type[j];
+    fnum = a.lib.f_num[j];
+    fmzl = a.f_mzl[j];
+    s.fint[f] = fint;
+    s.tn[f] = ftype | (fnum << 8);
+    for (uint32_t c = 0; c < L; ++c) {
+      s.xic[f * L + c] = 0.0f;
+      s.mz[f * L + c] = 0.0f;
+    }
+  }
+  // dense_xic_observation/mod.rs:98-137 with quadrupole_observation/mod.rs:
+  // 152-211, valid windows in ascending order
+  const float pmz = a.lib.p_mz[row];
+  for (uint32_t o = 0; o < a.dia.n_obs; ++o) {
+    if (!(a.dia.iso_lo[o] <= pmz && a.dia.iso_hi[o] >= pmz)) continue;
+    if (!mine) continue;
+    const float delta = fmz * a.mass_tol * 1e-6f;
+    uint32_t bs, be;
+    mz_range(a.dia, fmz - delta, fmz + delta, bs, be);
+    const uint32_t *ssw = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
+    const uint64_t eb = a.dia.entry_base[o];
+    const uint16_t key16 = (uint16_t)c0;
+    float *X = s.xic + f * L, *M = s.mz + f * L;
+    for (uint32_t b = bs; b < be; ++b) {
+      const float actual = a.dia.grid[b];
+      const uint32_t e0 = ssw[b], e1 = ssw[b + 1];
+      const uint16_t *cy = a.dia.cycles + eb + e0;
+      const float *in = a.dia.intens + eb + e0;
+      for (uint32_t t = rust_bsearch_u16(cy, e1 - e0, key16); t < e1 - e0; ++t) {
+        const uint64_t cyc = cy[t];
+        if (cyc >= c1) break;
+        const uint64_t rel64 = cyc - c0;
+        if (rel64 >= L) break;  // only for a cycle_start beyond 65535
+        const uint32_t rel = (uint32_t)rel64;
+        const float v = in[t];
+        const float x = X[rel] + v;
+        X[rel] = x;
+        if (v > 0.0f) {
+          const float prev = x - v;
+          M[rel] = prev == 0.0f ? actual : (M[rel] * prev + actual * v) / x;
+        }
+      }
+    }
+  }
+  // utils.rs:71-108 normalize_profiles(.., 1) and :7-29 filter_non_zero
+  const uint32_t cidx = L / 2;
+  if (mine) {
+    const uint32_t s0 = cidx >= 1 ? cidx - 1 : 0, s1 = min(cidx + 2, L);
+    float sum = 0.0f;
+    for (uint32_t j = s0; j < s1; ++j) sum = sum + s.xic[f * L + j];
+    const float mean = s1 > s0 ? sum / (float)(s1 - s0) : 0.0f;
+    uint32_t any = 0;
+    for (uint32_t j = 0; j < L; ++j) {
+      const float v = mean > 0.0f ? s.xic[f * L + j] / mean : 0.0f;
+      s.norm[f * L + j] = v;
+      any |= v != 0.0f ? 1u : 0u;
+    }
+    s.nz[f] = any;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t Ff = uni((uint32_t)__builtin_popcountll(ballot64(mine && s.nz[f] != 0u)));
+  // utils.rs:35-67 median_axis_0 of all rows and of the non-zero rows:
+  // each lane places its value of every column at its stable rank
+  if (mine) {
+    const bool nzf = s.nz[f] != 0u;
+    for (uint32_t c = 0; c < L; ++c) {
+      const float v = s.norm[f * L + c];
+      uint32_t ra = 0, rf = 0;
+      for (uint32_t g = 0; g < F; ++g) {
+        const float u = s.norm[g * L + c];
+        const bool before = u < v || (u == v && g < f);
+        ra += before ? 1u : 0u;
+        rf += (before && s.nz[g] != 0u) ? 1u : 0u;
+      }
+      s.sortA[c * ks + ra] = v;
+      if (nzf) s.sortF[c * ks + rf] = v;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  if ((uint32_t)lane < L) {
+    const uint32_t c = (uint32_t)lane;
+    const float *sa = s.sortA + c * ks, *sf = s.sortF + c * ks;
+    s.med[c] = F == 0 ? 0.0f
+               : (F % 2 == 0 ? (sa[F / 2 - 1] + sa[F / 2]) / 2.0f : sa[F / 2]);
+    s.medf[c] = Ff == 0 ? 0.0f
+                : (Ff % 2 == 0 ? (sf[Ff / 2 - 1] + sf[Ff / 2]) / 2.0f : sf[Ff / 2]);
+  }
+  __builtin_amdgcn_wave_barrier();
+  // mod.rs:142-149: median sums (Iterator::sum from -0.0) and FWHM
+  float med_sum = -0.0f, medf_sum = -0.0f;
+  for (uint32_t c = 0; c < L; ++c) {
+    med_sum = med_sum + s.med[c];
+    medf_sum = medf_sum + s.medf[c];
+  }
+  float fwhm = 0.0f;
+  if (L > 0) {  // utils.rs:453-476, usize index arithmetic wraps
+    const uint32_t h = L / 2;
+    const float cen = s.medf[h];
+    for (uint32_t i = 0; i < h; ++i) {
+      const float m = (s.medf[h - i] + s.medf[h + i]) / 2.0f;
+      if (m <= cen / 2.0f) {
+        const uint64_t il = c0 + h - i, ir = c0 + h + i;
+        if (il >= a.dia.n_rt || ir >= a.dia.n_rt) {
+          if (lane == 0) atomicOr(a.status, kStScoreRtIndex);
+        } else {
+          fwhm = a.dia.rt[ir] - a.dia.rt[il];
+        }
+        break;
+      }
+    }
+  }
+  // mod.rs:152-160: correlations with the filtered median, summed rows
+  float corr = 0.0f, oint = 0.0f;
+  if (mine) {
+    corr = corr_safe(s.medf, s.norm + f * L, L);
+    oint = ndarray_sum(s.xic + f * L, L);
+    s.corr[f] = corr;
+    s.oint[f] = oint;
+    s.matched[f] = oint > 0.0f ? 1u : 0u;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const float int_corr = corr_safe(s.oint, s.fint, F);
+  // mod.rs:164-197
+  float csum = -0.0f;
+  for (uint32_t g = 0; g < F; ++g) csum = csum + s.corr[g];
+  const float mean_corr = F ? csum / (float)F : 0.0f;
+  // utils/mod.rs:120-137 median and :146-155 sample std of the correlations
+  if (mine) {
+    uint32_t r = 0;
+    for (uint32_t g = 0; g < F; ++g) {
+      const float u = s.corr[g];
+      r += (u < corr || (u == corr && g < f)) ? 1u : 0u;
+    }
+    s.buf[r] = corr;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const float med_corr = F == 0 ? 0.0f
+                         : (F % 2 == 0 ? (s.buf[F / 2 - 1] + s.buf[F / 2]) / 2.0f
+                                       : s.buf[F / 2]);
+  float std_corr = 0.0f;
+  if (F >= 2) {
+    const float m = csum / (float)F;
+    float v = -0.0f;
+    for (uint32_t g = 0; g < F; ++g) {
+      const float d = s.corr[g] - m;
+      v = v + d * d;
+    }
+    std_corr = sqrtf(v / (float)(F - 1));
+  }
+  auto count = [&](float thr, bool sel) {
+    return (float)__builtin_popcountll(ballot64(mine && sel && corr > thr));
+  };
+  // utils/mod.rs:175-207 ranks by library intensity (descending, then
+  // fragment order)
+  uint32_t rint = 0;
+  if (mine)
+    for (uint32_t g = 0; g < F; ++g) {
+      const float u = s.fint[g];
+      rint += (u > fint || (u == fint && g < f)) ? 1u : 0u;
+    }
+  // mod.rs:203-247
+  const bool matched = mine && oint > 0.0f;
+  const float hs_obs = hyperscore_dev(s.tn, s.oint, s.matched, nullptr, F, ltab);
+  const float hs_lib = hyperscore_dev(s.tn, s.fint, s.matched, nullptr, F, ltab);
+  // utils.rs:314-371 longest b / y series: matched numbers of a type at
+  // their sorted rank, then one scan
+  float longest[2];
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t want = k == 0 ? (uint32_t)kFragB : (uint32_t)kFragY;
+    const bool in = matched && ftype == want;
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(ballot64(in));
+    if (in) {
+      uint32_t r = 0;
+      for (uint32_t g = 0; g < F; ++g) {
+        const uint32_t tg = s.tn[g];
+        if (!s.matched[g] || (tg & 0xffu) != want) continue;
+        const uint32_t ng = tg >> 8;
+        r += (ng < fnum || (ng == fnum && g < f)) ? 1u : 0u;
+      }
+      s.buf[r] = (float)fnum;
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t best = cnt ? 1u : 0u, cur = 1u;
+    for (uint32_t i = 1; i < cnt; ++i) {
+      if ((uint32_t)s.buf[i] == (uint32_t)s.buf[i - 1] + 1u) {
+        ++cur;
+        best = max(best, cur);
+      } else {
+        cur = 1u;
+      }
+    }
+    longest[k] = (float)(best & 0xffu);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // utils/mod.rs:11-69 observed m/z and ppm error; :83-109 weighted mean
+  // absolute error; utils.rs:374-399 inverse-mass-error weights
+  if (mine) {
+    float ws = 0.0f, wt = 0.0f;
+    for (uint32_t j = 0; j < L; ++j) {
+      const float w = s.xic[f * L + j];
+      if (w > 0.0f) {
+        ws = ws + s.mz[f * L + j] * w;
+        wt = wt + w;
+      }
+    }
+    const float m = wt > 0.0f ? ws / wt : 0.0f;
+    const float e = m > 0.0f ? ((m - fmz) / fmz) * 1e6f : 0.0f;
+    s.err[f] = e;
+    s.wie[f] = 1.0f / (fabsf(e) + 0.1f);
+    // src/idf/mod.rs:35-49
+    const uint32_t df = a.idf_counts ? a.idf_counts[closest_bin(a.dia, fmzl)] : 0u;
+    s.idf[f] = a.idf_total == 0.0f ? 1.0f
+               : logf_exact(a.idf_total / (float)(df > 1u ? df : 1u), ltab);
+  }
+  __builtin_amdgcn_wave_barrier();
+  float wmae = 0.0f;
+  {
+    float ws = 0.0f, wt = 0.0f;
+    for (uint32_t g = 0; g < F; ++g) {
+      const float w = s.fint[g], e = s.err[g];
+      if (w > 0.0f && e != 0.0f) {
+        ws = ws + fabsf(e) * w;
+        wt = wt + w;
+      }
+    }
+    wmae = (F > 0 && wt > 0.0f) ? ws / wt : 0.0f;
+  }
+  const float hs_ime = hyperscore_dev(s.tn, s.oint, s.matched, s.wie, F, ltab);
+  // mod.rs:249-268
+  const uint64_t cc = a.c_center[ci];
+  if (cc >= a.dia.n_rt && lane == 0) atomicOr(a.status, kStScoreRtIndex);
+  const float rt_obs = cc < a.dia.n_rt ? a.dia.rt[cc] : 0.0f;
+  const float delta_rt = rt_obs - a.lib.p_rt[row];
+  float ib = 0.0f, iy = 0.0f;
+  for (uint32_t g = 0; g < F; ++g) {
+    if (!s.matched[g] || !(s.oint[g] > 0.0f)) continue;
+    const uint32_t t = s.tn[g] & 0xffu;
+    if (t == (uint32_t)kFragB) ib = ib + s.oint[g];
+    if (t == (uint32_t)kFragY) iy = iy + s.oint[g];
+  }
+  // mod.rs:272-293
+  const float idf_hs = hyperscore_dev(s.tn, s.idf, s.matched, nullptr, F, ltab);
+  float dot_xic = -0.0f, dot_int = -0.0f;
+  for (uint32_t g = 0; g < F; ++g) {
+    dot_xic = dot_xic + s.idf[g] * s.corr[g];
+    dot_int = dot_int + s.idf[g] * s.oint[g];
+  }
+  if (F == 0) dot_xic = dot_int = 0.0f;
+  uint32_t ridf = 0;
+  if (mine) {
+    const float v = s.idf[f];
+    for (uint32_t g = 0; g < F; ++g) {
+      const float u = s.idf[g];
+      ridf += (u > v || (u == v && g < f)) ? 1u : 0u;
+    }
+  }
+  const float cnt_o0_i6 = count(0.0f, ridf < 6), cnt_o50_i6 = count(0.5f, ridf < 6);
+  const float c95 = count(0.95f, true), c90 = count(0.90f, true), c80 = count(0.80f, true);
+  const float c50 = count(0.5f, true), c0v = count(0.0f, true);
+  const float r0 = count(0.0f, rint < 6), r1 = count(0.0f, rint >= 6 && rint < 12);
+  const float r2 = count(0.0f, rint >= 12 && rint < 18), r3 = count(0.0f, rint >= 18 && rint < 24);
+  const float q0 = count(0.5f, rint < 6), q1 = count(0.5f, rint >= 6 && rint < 12);
+  const float q2 = count(0.5f, rint >= 12 && rint < 18), q3 = count(0.5f, rint >= 18 && rint < 24);
+  if (lane == 0) a.found[ci] = 1;
+  F_OUT(ADSR_F_SCORE, a.c_score[ci]);
+  F_OUT(ADSR_F_MEAN_CORRELATION, mean_corr);
+  F_OUT(ADSR_F_MEDIAN_CORRELATION, med_corr);
+  F_OUT(ADSR_F_CORRELATION_STD, std_corr);
+  F_OUT(ADSR_F_INTENSITY_CORRELATION, int_corr);
+  F_OUT(ADSR_F_NUM_FRAGMENTS, (float)F);
+  F_OUT(ADSR_F_NUM_SCANS, (float)L64);
+  F_OUT(ADSR_F_NUM_OVER_95, c95);
+  F_OUT(ADSR_F_NUM_OVER_90, c90);
+  F_OUT(ADSR_F_NUM_OVER_80, c80);
+  F_OUT(ADSR_F_NUM_OVER_50, c50);
+  F_OUT(ADSR_F_NUM_OVER_0, c0v);
+  F_OUT(ADSR_F_NUM_OVER_0_RANK_0_5, r0);
+  F_OUT(ADSR_F_NUM_OVER_0_RANK_6_11, r1);
+  F_OUT(ADSR_F_NUM_OVER_0_RANK_12_17, r2);
+  F_OUT(ADSR_F_NUM_OVER_0_RANK_18_23, r3);
+  F_OUT(ADSR_F_NUM_OVER_50_RANK_0_5, q0);
+  F_OUT(ADSR_F_NUM_OVER_50_RANK_6_11, q1);
+  F_OUT(ADSR_F_NUM_OVER_50_RANK_12_17, q2);
+  F_OUT(ADSR_F_NUM_OVER_50_RANK_18_23, q3);
+  F_OUT(ADSR_F_HYPERSCORE_INTENSITY_OBSERVATION, hs_obs);
+  F_OUT(ADSR_F_HYPERSCORE_INTENSITY_LIBRARY, hs_lib);
+  F_OUT(ADSR_F_HYPERSCORE_INVERSE_MASS_ERROR, hs_ime);
+  F_OUT(ADSR_F_RT_OBSERVED, rt_obs);
+  F_OUT(ADSR_F_DELTA_RT, delta_rt);
+  F_OUT(ADSR_F_LONGEST_B_SERIES, longest[0]);
+  F_OUT(ADSR_F_LONGEST_Y_SERIES, longest[1]);
+  F_OUT(ADSR_F_NAA, (float)a.p_naa[row]);
+  F_OUT(ADSR_F_WEIGHTED_MASS_ERROR, wmae);
+  F_OUT(ADSR_F_LOG10_B_ION_INTENSITY, log10f(ib + 1e-8f));
+  F_OUT(ADSR_F_LOG10_Y_ION_INTENSITY, log10f(iy + 1e-8f));
+  F_OUT(ADSR_F_FWHM_RT, fwhm);
+  F_OUT(ADSR_F_IDF_HYPERSCORE, idf_hs);
+  F_OUT(ADSR_F_IDF_XIC_DOT_PRODUCT, dot_xic);
+  F_OUT(ADSR_F_IDF_INTENSITY_DOT_PRODUCT, log10f(dot_int + 1e-8f));
+  F_OUT(ADSR_F_MEDIAN_PROFILE_SUM, med_sum);
+  F_OUT(ADSR_F_MEDIAN_PROFILE_SUM_FILTERED, medf_sum);
+  F_OUT(ADSR_F_NUM_PROFILES, (float)F);
+  F_OUT(ADSR_F_NUM_PROFILES_FILTERED, (float)Ff);
+  F_OUT(ADSR_F_NUM_OVER_0_TOP6_IDF, cnt_o0_i6);
+  F_OUT(ADSR_F_NUM_OVER_50_TOP6_IDF, cnt_o50_i6);
+}
+
+static const char *const kFeatureNames[ADSR_N_FEATURES] = {
+    "score", "mean_correlation", "median_correlation", "correlation_std",
+    "intensity_correlation", "num_fragments", "num_scans", "num_over_95",
+    "num_over_90", "num_over_80", "num_over_50", "num_over_0",
+    "num_over_0_rank_0_5", "num_over_0_rank_6_11", "num_over_0_rank_12_17",
+    "num_over_0_rank_18_23", "num_over_50_rank_0_5", "num_over_50_rank_6_11",
+    "num_over_50_rank_12_17", "num_over_50_rank_18_23",
+    "hyperscore_intensity_observation", "hyperscore_intensity_library",
+    "hyperscore_inverse_mass_error", "rt_observed", "delta_rt",
+    "longest_b_series", "longest_y_series", "naa", "weighted_mass_error",
+    "log10_b_ion_intensity", "log10_y_ion_intensity", "fwhm_rt",
+    "idf_hyperscore", "idf_xic_dot_product", "idf_intensity_dot_product",
+    "median_profile_sum", "median_profile_sum_filtered", "num_profiles",
+    "num_profiles_filtered", "num_over_0_top6_idf", "num_over_50_top6_idf"};
+
+extern "C" const char *adsr_feature_name(size_t i) {
+  return i < (size_t)ADSR_N_FEATURES ? kFeatureNames[i] : nullptr;
+}
+
+// src/peak_group_scoring/parameters.rs:24-33
+extern "C" void adsr_scoring_params_default(adsr_scoring_params *p) {
+  if (!p) return;
+  p->mass_tolerance = 7.0f;
+  p->top_k_fragments = 100;
+}
+
+// The library columns only scoring reads, and the IDF bin counts of every
+// fragment_mz_library (src/speclib_flat/mod.rs:165-166, src/idf/mod.rs).
+static adsr_status upload_lib_scoring(adsr_lib *l, const KDia &kd) {
+  std::lock_guard<std::mutex> lk(l->mu);
+  DeviceLib &v = l->dev;
+  if (v.scoring) return ADSR_OK;
+  size_t acc = 0;
+  adsr_status st = dmalloc_copy(&v.p_naa, l->precursor_naa.data(), l->n_prec, acc);
+  if (!st) st = dmalloc_copy(&v.f_mzl, l->fragment_mz_library.data(), l->n_frag, acc);
+  if (st) return st;
+  HIP_TRY(hipMalloc((void **)&v.idf_counts, (size_t)kd.n_grid * 4));
+  HIP_TRY(hipMemset(v.idf_counts, 0, (size_t)kd.n_grid * 4));
+  if (l->n_frag)
+    idf_count_kernel<<<(uint32_t)((l->n_frag + 255) / 256), 256>>>(kd, v.f_mzl,
+                                                                    l->n_frag,
+                                                                    v.idf_counts);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  v.bytes += acc + (size_t)kd.n_grid * 4;
+  v.scoring = true;
+  return ADSR_OK;
+}
+
+// Both entry points: `hc` (host table) or `dc` (table in HBM).
+static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
+                                const adsr_candidates *hc,
+                                const adsr_device_candidates *dc,
+                                const adsr_scoring_params *p,
+                                adsr_features **out) {
+  clear_error();
+  if (!dc_ || !lc || !p || !out || (!hc && !dc))
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  adsr_dia *d = const_cast<adsr_dia *>(dc_);
+  adsr_lib *l = const_cast<adsr_lib *>(lc);
+  adsr_status st = upload_dia(d);
+  if (!st) st = upload_lib(l);
+  if (st) return st;
+  const KDia kd = make_kdia(d);
+  st = upload_lib_scoring(l, kd);
+  if (st) return st;
+  const size_t n = hc ? hc->precursor_idx.size() : dc->n;
+  const uint64_t nrt = d->rt_index.size();
+  // the longest cycle range of the table bounds the LDS per candidate
+  uint64_t lmax = 0;
+  if (hc) {
+    for (size_t i = 0; i < n; ++i) {
+      lmax = std::max<uint64_t>(lmax, hc->cycle_stop[i] - hc->cycle_start[i]);
+      if (hc->cycle_center[i] >= nrt)
+        return set_error(ADSR_ERR_INVALID_ARGUMENT,
+                         "candidate %zu: cycle_center %llu is outside the RT "
+                         "axis (%llu cycles)",
+                         i, (unsigned long long)hc->cycle_center[i],
+                         (unsigned long long)nrt);
+    }
+  } else {
+    lmax = 2 * dc->pl + 1;
+  }
+  if (lmax > (uint64_t)kScoreLMax)
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "candidate cycle ranges of %llu cycles exceed %d",
+                     (unsigned long long)lmax, kScoreLMax);
+  // fragment filter and m/z order with the scoring top-k: the selection's
+  // plan pass (get_precursor_filtered(i, true, true, top_k))
+  adsr_selection_params sp_p;
+  adsr_selection_params_default(&sp_p);
+  sp_p.mass_tolerance = p->mass_tolerance;
+  sp_p.top_k_fragments = p->top_k_fragments;
+  sp_p.rt_tolerance = 0.0f;
+  sp_p.candidate_count = 1;
+  Workspace w;
+  SearchPlan sp;
+  st = build_search(d, l, &sp_p, w, sp);
+  if (st) {
+    ws_free(w);
+    return st;
+  }
+  auto fail = [&](adsr_status s2) {
+    ws_free(w);
+    return s2;
+  };
+  const uint32_t n_rows = (uint32_t)l->n_prec;
+  (void)hipEventRecord(w.ev[0], w.stream);
+  st = enqueue(w, sp, n_rows, LaunchShape(), nullptr, 0, 0, 0, false, true);
+  if (st) return fail(st);
+  (void)hipEventRecord(w.ev[1], w.stream);
+  {
+    uint32_t host[3] = {0, 0, 0};
+    if (hipMemcpyAsync(host, w.small, 12, hipMemcpyDeviceToHost, w.stream) != hipSuccess ||
+        hipStreamSynchronize(w.stream) != hipSuccess)
+      return fail(set_error(ADSR_ERR_DEVICE, "fragment plan failed"));
+    if (host[1] & kStTooManyFragments)
+      return fail(set_error(ADSR_ERR_UNSUPPORTED,
+                            "a precursor selects more than %d fragments", kMaxSel));
+  }
+  // candidate columns on the device
+  BuildBufs tmp;
+  uint64_t *cpidx, *crank, *cs, *ce, *cc;
+  float *csc, *feat;
+  uint8_t *found;
+  uint32_t *status;
+  if (tmp.alloc(&cpidx, n) || tmp.alloc(&crank, n) || tmp.alloc(&cs, n) ||
+      tmp.alloc(&ce, n) || tmp.alloc(&cc, n) || tmp.alloc(&csc, n) ||
+      tmp.alloc(&feat, (size_t)ADSR_N_FEATURES * n) || tmp.alloc(&found, n) ||
+      tmp.alloc(&status, 1))
+    return fail(set_error(ADSR_ERR_OUT_OF_MEMORY, "scoring buffers"));
+  if (hipMemsetAsync(status, 0, 4, w.stream) != hipSuccess)
+    return fail(set_error(ADSR_ERR_DEVICE, "status reset failed"));
+  if (n && hc) {
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+      return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, w.stream) == hipSuccess;
+    };
+    if (!up(cpidx, hc->precursor_idx.data(), n * 8) || !up(crank, hc->rank.data(), n * 8) ||
+        !up(cs, hc->cycle_start.data(), n * 8) || !up(ce, hc->cycle_stop.data(), n * 8) ||
+        !up(cc, hc->cycle_center.data(), n * 8) || !up(csc, hc->score.data(), n * 4))
+      return fail(set_error(ADSR_ERR_DEVICE, "candidate upload failed"));
+  } else if (n) {
+    export_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, w.stream>>>(
+        (const DevCand *)dc->rows, n, dc->pl, dc->nrt, cpidx, crank, csc, nullptr,
+        nullptr, nullptr, cc, cs, ce);
+    if (hipGetLastError() != hipSuccess)
+      return fail(set_error(ADSR_ERR_DEVICE, "candidate export failed"));
+  }
+  ScoreArgs a;
+  a.dia = sp.dia;
+  a.lib = sp.lib;
+  a.plan = sp.plan;
+  a.p_idx = l->dev.p_idx;
+  a.p_naa = l->dev.p_naa;
+  a.f_mzl = l->dev.f_mzl;
+  a.idf_counts = l->dev.idf_counts;
+  a.idf_total = (float)l->n_frag;
+  a.mass_tol = p->mass_tolerance;
+  a.n = (uint32_t)n;
+  a.c_pidx = cpidx;
+  a.c_start = cs;
+  a.c_stop = ce;
+  a.c_center = cc;
+  a.c_score = csc;
+  a.feat = feat;
+  a.found = found;
+  a.status = status;
+  a.lds_per_wave = (score_lds_bytes(sp.plan.ks, (uint32_t)lmax) + 15) / 16 * 16;
+  int wpb = 4;
+  while (wpb > 1 && (size_t)a.lds_per_wave * wpb > 64 * 1024) wpb >>= 1;
+  if ((size_t)a.lds_per_wave > 64 * 1024)
+    return fail(set_error(ADSR_ERR_UNSUPPORTED, "scoring LDS of %u bytes", a.lds_per_wave));
+  (void)hipEventRecord(w.ev[2], w.stream);
+  if (n) {
+    score_kernel<<<(uint32_t)((n + wpb - 1) / wpb), wpb * kWave,
+                   (size_t)a.lds_per_wave * wpb, w.stream>>>(a);
+    if (hipGetLastError() != hipSuccess)
+      return fail(set_error(ADSR_ERR_DEVICE, "score kernel launch failed"));
+  }
+  (void)hipEventRecord(w.ev[3], w.stream);
+  std::vector<float> hf((size_t)ADSR_N_FEATURES * n);
+  std::vector<uint8_t> hfound(n);
+  std::vector<uint64_t> hp(n), hr(n);
+  uint32_t hstatus = 0;
+  bool ok = hipMemcpyAsync(&hstatus, status, 4, hipMemcpyDeviceToHost, w.stream) == hipSuccess;
+  if (n) {
+    ok = ok && hipMemcpyAsync(hf.data(), feat, hf.size() * 4, hipMemcpyDeviceToHost,
+                              w.stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(hfound.data(), found, n, hipMemcpyDeviceToHost,
+                              w.stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(hp.data(), cpidx, n * 8, hipMemcpyDeviceToHost,
+                              w.stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(hr.data(), crank, n * 8, hipMemcpyDeviceToHost,
+                              w.stream) == hipSuccess;
+  }
+  if (!ok || hipStreamSynchronize(w.stream) != hipSuccess)
+    return fail(set_error(ADSR_ERR_DEVICE, "scoring failed: %s",
+                          hipGetErrorString(hipGetLastError())));
+  if (hstatus & kStScoreRtIndex)
+    return fail(set_error(ADSR_ERR_INVALID_ARGUMENT,
+                          "a candidate's cycle range indexes outside the RT axis"));
+  if (hstatus & kStScoreWindowTooLong)
+    return fail(set_error(ADSR_ERR_UNSUPPORTED, "a candidate cycle range exceeds %d",
+                          kScoreLMax));
+  auto *f = new adsr_features();
+  size_t m = 0;
+  for (size_t i = 0; i < n; ++i) m += hfound[i] ? 1 : 0;
+  f->precursor_idx.reserve(m);
+  f->rank.reserve(m);
+  f->columns.resize((size_t)ADSR_N_FEATURES * m);
+  size_t k = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (!hfound[i]) continue;
+    f->precursor_idx.push_back(hp[i]);
+    f->rank.push_back(hr[i]);
+    for (int c = 0; c < ADSR_N_FEATURES; ++c)
+      f->columns[(size_t)c * m + k] = hf[(size_t)c * n + i];
+    ++k;
+  }
+  float t_plan = 0, t_kern = 0;
+  (void)hipEventElapsedTime(&t_plan, w.ev[0], w.ev[1]);
+  (void)hipEventElapsedTime(&t_kern, w.ev[2], w.ev[3]);
+  f->plan_ms = t_plan;
+  f->kernel_ms = t_kern;
+  ws_free(w);
+  *out = f;
+  return ADSR_OK;
+}
+
+extern "C" adsr_status adsr_score(const adsr_dia *d, const adsr_lib *l,
+                                  const adsr_candidates *c,
+                                  const adsr_scoring_params *p,
+                                  adsr_features **out) {
+  if (!c) {
+    clear_error();
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL candidates");
+  }
+  return score_common(d, l, c, nullptr, p, out);
+}
+
+extern "C" adsr_status adsr_score_device(const adsr_dia *d, const adsr_lib *l,
+                                         const adsr_device_candidates *c,
+                                         const adsr_scoring_params *p,
+                                         adsr_features **out) {
+  if (!c) {
+    clear_error();
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL candidates");
+  }
+  return score_common(d, l, nullptr, c, p, out);
+}
+
+extern "C" size_t adsr_features_len(const adsr_features *f) {
+  return f ? f->precursor_idx.size() : 0;
+}
+
+extern "C" adsr_status adsr_features_export(const adsr_features *f,
+                                            uint64_t *precursor_idx,
+                                            uint64_t *rank, float *columns) {
+  clear_error();
+  if (!f) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  const size_t m = f->precursor_idx.size();
+  if (precursor_idx && m) std::memcpy(precursor_idx, f->precursor_idx.data(), m * 8);
+  if (rank && m) std::memcpy(rank, f->rank.data(), m * 8);
+  if (columns && m) std::memcpy(columns, f->columns.data(), f->columns.size() * 4);
+  return ADSR_OK;
+}
+
+extern "C" adsr_status adsr_features_timings(const adsr_features *f, double *plan_ms,
+                                             double *kernel_ms) {
+  clear_error();
+  if (!f) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (plan_ms) *plan_ms = f->plan_ms;
+  if (kernel_ms) *kernel_ms = f->kernel_ms;
+  return ADSR_OK;
+}
+
+extern "C" void adsr_features_free(adsr_features *f) { delete f; }

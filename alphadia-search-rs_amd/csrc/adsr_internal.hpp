@@ -66,6 +66,12 @@ struct DeviceLib {
   uint64_t *f_start = nullptr, *f_stop = nullptr;
   float *f_mz = nullptr, *f_int = nullptr;
   uint8_t *f_type = nullptr, *f_num = nullptr;
+  // scoring only, uploaded by the first score call: precursor_naa,
+  // fragment_mz_library and the IDF bin counts of all fragment_mz_library
+  bool scoring = false;
+  uint8_t *p_naa = nullptr;
+  float *f_mzl = nullptr;
+  uint32_t *idf_counts = nullptr;
   size_t bytes = 0;
 };
 
@@ -153,6 +159,14 @@ struct adsr_device_candidates {
   void *rows = nullptr;
   size_t n = 0;
   uint64_t pl = 0, nrt = 0;  // peak_length and RT-index length
+};
+
+// CandidateFeatureCollection (src/candidate/features.rs:196-430): one row
+// per scored candidate, the f32 features column-major [ADSR_N_FEATURES][n].
+struct adsr_features {
+  std::vector<uint64_t> precursor_idx, rank;
+  std::vector<float> columns;
+  double plan_ms = 0, kernel_ms = 0;
 };
 
 struct adsr_candidates {

@@ -159,6 +159,8 @@ def main():
                     help="library rows timed on the host for cpu_baseline "
                          "(0 = the whole library)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-scoring", action="store_true",
+                    help="skip PeakGroupScoring.score of the run's candidates")
     ap.add_argument("--no-csr-check", action="store_true",
                     help="skip the host build of the run and the byte "
                          "comparison of its CSR with the device-built one")
@@ -345,6 +347,34 @@ def main():
             f"{'identical' if same else 'DIFFERENT'}")
     result["config"]["dia_build"] = build
 
+    if not args.no_scoring:
+        # PeakGroupScoring.score of this search's candidate table, handed
+        # over in HBM (the next step of the reference pipeline)
+        sp = ads.ScoringParameters()
+        sp.update({"mass_tolerance": spec.mass_tolerance})
+        scorer = ads.PeakGroupScoring(sp)
+        dcands = ads.PeakGroupSelection(params).search_device(dia, lib)
+        scorer.score(dia, lib, dcands)  # warm: fragment plan, IDF table
+        t1 = time.perf_counter()
+        feats = scorer.score(dia, lib, dcands)
+        score_s = time.perf_counter() - t1
+        tm = feats.device_timings()
+        n_c = len(dcands)
+        scoring = {
+            "candidates": n_c,
+            "scored": len(feats),
+            "plan_ms": round(tm["plan_ms"], 3),
+            "kernel_ms": round(tm["kernel_ms"], 3),
+            "candidates_per_s_device": round(n_c / (tm["kernel_ms"] / 1e3), 1)
+            if tm["kernel_ms"] > 0 else None,
+            "call_s_incl_feature_copy": round(score_s, 4),
+        }
+        log(f"scoring: {n_c} candidates, kernel {tm['kernel_ms']:.2f} ms, plan "
+            f"{tm['plan_ms']:.2f} ms, call {score_s * 1e3:.1f} ms")
+        result["config"]["scoring"] = scoring
+        fd = feats.to_dict_arrays()
+        del feats
+
     if fb is not None:
         fb_rate = n_prec / (fb[0] / 1e3)
         result["config"].update({
@@ -383,6 +413,33 @@ def main():
         ob = oracle.select(odia, olib, oprm, r2, r3, nthreads=nt)
         cpu_s = time.perf_counter() - t4
         cpu_rate = rows / cpu_s
+        if not args.no_scoring:
+            # the oracle scores a sample of the same table; its features must
+            # equal the device's (log10 columns within 2e-6)
+            tab = dcands.to_collection().to_arrays()
+            k = min(len(tab[0]), 100_000)
+            samp = tuple(c[:k] for c in tab)
+            t5 = time.perf_counter()
+            ofeat = oracle.score(odia, olib, samp, mass_tolerance=spec.mass_tolerance,
+                                 nthreads=nt)
+            o_s = time.perf_counter() - t5
+            same = len(ofeat["precursor_idx"]) == k
+            for name in ofeat:
+                if not same:
+                    break
+                g, w = fd[name][:k], ofeat[name]
+                if name in ("log10_b_ion_intensity", "log10_y_ion_intensity",
+                            "idf_intensity_dot_product"):
+                    same = bool(np.all(np.abs(g - w) <= 2e-6 * np.maximum(1.0, np.abs(w))))
+                else:
+                    same = bool(np.all(g == w))
+            result["config"]["scoring"].update({
+                "cpu_oracle_candidates_per_s": round(k / o_s, 1),
+                "cpu_sample_candidates": k,
+                "cpu_sample_features_identical": same,
+            })
+            log(f"cpu oracle scoring {k / o_s:.0f} candidates/s on {nt} threads; "
+                f"sample features {'identical' if same else 'DIFFERENT'}")
         got = cand.to_arrays()
         pidx = got[0].astype(np.int64)
         m = ((pidx >= r0) & (pidx < r1)) | ((pidx >= r2) & (pidx < r3))

@@ -289,6 +289,66 @@ adsr_status adsr_device_candidates_to_host(const adsr_device_candidates *c,
                                            adsr_candidates **out);
 void adsr_device_candidates_free(adsr_device_candidates *c);
 
+/* ---- PeakGroupScoring (src/peak_group_scoring/mod.rs:43-341) ---------- */
+/* ScoringParameters (src/peak_group_scoring/parameters.rs:4-40) */
+typedef struct adsr_scoring_params {
+  float mass_tolerance;     /* ppm, default 7.0 */
+  uint64_t top_k_fragments; /* default 100 */
+} adsr_scoring_params;
+void adsr_scoring_params_default(adsr_scoring_params *p);
+
+/* CandidateFeatureCollection (src/candidate/features.rs:196-430) */
+typedef struct adsr_features adsr_features;
+/* f32 feature columns, in CandidateFeature / to_dict_arrays order after
+ * precursor_idx and rank (features.rs:51-96, 314-410) */
+enum {
+  ADSR_F_SCORE, ADSR_F_MEAN_CORRELATION, ADSR_F_MEDIAN_CORRELATION,
+  ADSR_F_CORRELATION_STD, ADSR_F_INTENSITY_CORRELATION, ADSR_F_NUM_FRAGMENTS,
+  ADSR_F_NUM_SCANS, ADSR_F_NUM_OVER_95, ADSR_F_NUM_OVER_90,
+  ADSR_F_NUM_OVER_80, ADSR_F_NUM_OVER_50, ADSR_F_NUM_OVER_0,
+  ADSR_F_NUM_OVER_0_RANK_0_5, ADSR_F_NUM_OVER_0_RANK_6_11,
+  ADSR_F_NUM_OVER_0_RANK_12_17, ADSR_F_NUM_OVER_0_RANK_18_23,
+  ADSR_F_NUM_OVER_50_RANK_0_5, ADSR_F_NUM_OVER_50_RANK_6_11,
+  ADSR_F_NUM_OVER_50_RANK_12_17, ADSR_F_NUM_OVER_50_RANK_18_23,
+  ADSR_F_HYPERSCORE_INTENSITY_OBSERVATION, ADSR_F_HYPERSCORE_INTENSITY_LIBRARY,
+  ADSR_F_HYPERSCORE_INVERSE_MASS_ERROR, ADSR_F_RT_OBSERVED, ADSR_F_DELTA_RT,
+  ADSR_F_LONGEST_B_SERIES, ADSR_F_LONGEST_Y_SERIES, ADSR_F_NAA,
+  ADSR_F_WEIGHTED_MASS_ERROR, ADSR_F_LOG10_B_ION_INTENSITY,
+  ADSR_F_LOG10_Y_ION_INTENSITY, ADSR_F_FWHM_RT, ADSR_F_IDF_HYPERSCORE,
+  ADSR_F_IDF_XIC_DOT_PRODUCT, ADSR_F_IDF_INTENSITY_DOT_PRODUCT,
+  ADSR_F_MEDIAN_PROFILE_SUM, ADSR_F_MEDIAN_PROFILE_SUM_FILTERED,
+  ADSR_F_NUM_PROFILES, ADSR_F_NUM_PROFILES_FILTERED,
+  ADSR_F_NUM_OVER_0_TOP6_IDF, ADSR_F_NUM_OVER_50_TOP6_IDF, ADSR_N_FEATURES
+};
+/* Name of feature column i (the to_dict_arrays key), NULL past the end. */
+const char *adsr_feature_name(size_t i);
+/* PeakGroupScoring::score (mod.rs:43-102): every candidate in table order;
+ * candidates whose precursor_idx is not in the library are skipped, as the
+ * reference's filter_map does.  Limits: at most 64 selected fragments per
+ * precursor and cycle_stop - cycle_start <= 32 (peak_length <= 15); a
+ * candidate whose cycle indices fall outside the RT axis, where the
+ * reference panics, returns INVALID_ARGUMENT.                             */
+adsr_status adsr_score(const adsr_dia *d, const adsr_lib *l,
+                       const adsr_candidates *c, const adsr_scoring_params *p,
+                       adsr_features **out);
+/* The same for a candidate table kept in HBM (adsr_select_device): no
+ * host copy of the candidates. */
+adsr_status adsr_score_device(const adsr_dia *d, const adsr_lib *l,
+                              const adsr_device_candidates *c,
+                              const adsr_scoring_params *p,
+                              adsr_features **out);
+size_t adsr_features_len(const adsr_features *f);
+/* to_dict_arrays (features.rs:220-410): precursor_idx and rank (u64[len])
+ * and the f32 columns, column-major [ADSR_N_FEATURES][len]; any pointer
+ * may be NULL. */
+adsr_status adsr_features_export(const adsr_features *f,
+                                 uint64_t *precursor_idx, uint64_t *rank,
+                                 float *columns);
+/* Device time (ms) of the fragment plan and of the scoring kernel. */
+adsr_status adsr_features_timings(const adsr_features *f, double *plan_ms,
+                                  double *kernel_ms);
+void adsr_features_free(adsr_features *f);
+
 /* ---- per-stage device kernels (SimdBackend ops, src/simd/mod.rs:11-39) --
  * Host in/out buffers; the library stages them through HBM.               */
 /* GaussianKernel::new (src/kernel/mod.rs:13-30); returns the length. */
