@@ -30,6 +30,9 @@ __all__ = [
     "ScoringParameters",
     "PeakGroupScoring",
     "CandidateFeatureCollection",
+    "QuantificationParameters",
+    "PeakGroupQuantification",
+    "SpecLibFlatQuantified",
     "get_optimal_simd_backend",
     "set_simd_backend",
     "clear_simd_backend",
@@ -703,5 +706,108 @@ class PeakGroupScoring:
             # mod.rs:94-99
             rate = len(candidates) / dt if dt > 0 else float("inf")
             print(f"Scored {len(candidates)} candidates at {rate:.2f} candidates/second")
+            sys.stdout.flush()
+        return out
+
+
+# ---------------------------------------------------------------------------
+# PeakGroupQuantification (src/peak_group_quantification/mod.rs:27-168)
+# ---------------------------------------------------------------------------
+class QuantificationParameters:
+    """src/peak_group_quantification/parameters.rs: ``tolerance_ppm`` and
+    ``top_k_fragments``, readable and writable, plus ``update(dict)``."""
+
+    def __init__(self):
+        self.tolerance_ppm = 7.0
+        self.top_k_fragments = 10000
+
+    def __setattr__(self, name, value):
+        if name == "tolerance_ppm":
+            value = _extract_f32(name, value)
+        elif name == "top_k_fragments":
+            value = _extract_usize(name, value)
+        else:
+            raise AttributeError(f"'QuantificationParameters' object has no attribute '{name}'")
+        object.__setattr__(self, name, value)
+
+    def update(self, config: dict) -> None:
+        if not isinstance(config, dict):
+            raise TypeError("update() expects a dict")
+        for k in ("tolerance_ppm", "top_k_fragments"):
+            if k in config:
+                setattr(self, k, config[k])
+
+
+class SpecLibFlatQuantified:
+    """src/speclib_flat_quantified/mod.rs: the quantified library."""
+
+    def __init__(self):
+        self._h = None
+
+    def __del__(self, _free=N.lib.adsr_quantified_free):
+        h = getattr(self, "_h", None)
+        if h:
+            _free(h)
+            self._h = None
+
+    @property
+    def num_precursors(self) -> int:
+        return int(N.lib.adsr_quantified_num_precursors(self._h)) if self._h else 0
+
+    @property
+    def num_fragments(self) -> int:
+        return int(N.lib.adsr_quantified_num_fragments(self._h)) if self._h else 0
+
+    def to_dict_arrays(self):
+        """mod.rs:126-211: (precursor dict, fragment dict)."""
+        n, m = self.num_precursors, self.num_fragments
+        pk = [("precursor_idx", np.uint64), ("mz_library", np.float32), ("mz", np.float32),
+              ("rt_library", np.float32), ("rt", np.float32), ("naa", np.uint8),
+              ("rank", np.uint64), ("rt_observed", np.float32),
+              ("flat_frag_start_idx", np.uint64), ("flat_frag_stop_idx", np.uint64)]
+        fk = [("mz_library", np.float32), ("mz", np.float32), ("intensity", np.float32),
+              ("cardinality", np.uint8), ("charge", np.uint8), ("loss_type", np.uint8),
+              ("number", np.uint8), ("position", np.uint8), ("type", np.uint8),
+              ("mz_observed", np.float32), ("correlation_observed", np.float32),
+              ("mass_error_observed", np.float32), ("precursor_idx", np.uint64),
+              ("rank", np.uint64)]
+        prec = {k: np.zeros(n, t) for k, t in pk}
+        frag = {k: np.zeros(m, t) for k, t in fk}
+        if self._h:
+            check(N.lib.adsr_quantified_export_precursors(self._h,
+                                                          *[ptr(prec[k]) for k, _ in pk]))
+            check(N.lib.adsr_quantified_export_fragments(self._h,
+                                                         *[ptr(frag[k]) for k, _ in fk]))
+        return prec, frag
+
+
+class PeakGroupQuantification:
+    def __init__(self, params: QuantificationParameters):
+        if not isinstance(params, QuantificationParameters):
+            raise TypeError("PeakGroupQuantification expects QuantificationParameters")
+        p = QuantificationParameters()
+        p.tolerance_ppm = params.tolerance_ppm
+        p.top_k_fragments = params.top_k_fragments
+        self._params = p
+
+    def quantify(self, dia_data: DIAData, lib: SpecLibFlat,
+                 candidates: CandidateCollection) -> SpecLibFlatQuantified:
+        """mod.rs:39-82 on the gfx950 device."""
+        if not isinstance(dia_data, DIAData) or not isinstance(lib, SpecLibFlat) or \
+                not isinstance(candidates, CandidateCollection):
+            raise TypeError("quantify(dia_data: DIAData, lib: SpecLibFlat, "
+                            "candidates: CandidateCollection)")
+        out = SpecLibFlatQuantified()
+        if not dia_data._h or not lib._h or not candidates._h:
+            return out
+        c = N.QuantParamsC(self._params.tolerance_ppm, self._params.top_k_fragments)
+        h = C.c_void_p()
+        t0 = time.perf_counter()
+        check(N.lib.adsr_quantify(dia_data._h, lib._h, candidates._h, C.byref(c), C.byref(h)))
+        out._h = h.value
+        if os.environ.get("ADSR_QUIET", "") in ("", "0"):
+            # mod.rs:72-77
+            print(f"Peak group quantification completed in {time.perf_counter() - t0:.2f}s "
+                  f"for {out.num_precursors} precursors")
             sys.stdout.flush()
         return out

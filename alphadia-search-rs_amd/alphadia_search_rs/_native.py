@@ -52,6 +52,12 @@ class ScoringParamsC(C.Structure):
     _fields_ = [("mass_tolerance", c_f32), ("top_k_fragments", c_u64)]
 
 
+class QuantParamsC(C.Structure):
+    """struct adsr_quant_params (include/adsr.h)."""
+
+    _fields_ = [("tolerance_ppm", c_f32), ("top_k_fragments", c_u64)]
+
+
 class SelectionParamsC(C.Structure):
     """struct adsr_selection_params (include/adsr.h)."""
 
@@ -137,6 +143,14 @@ _sig("adsr_features_len", c_size, vp)
 _sig("adsr_features_export", I, vp, vp, vp, vp)
 _sig("adsr_features_timings", I, vp, C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("adsr_features_free", None, vp)
+_sig("adsr_quant_params_default", None, C.POINTER(QuantParamsC))
+_sig("adsr_quantify", I, vp, vp, vp, C.POINTER(QuantParamsC), C.POINTER(vp))
+_sig("adsr_quantified_num_precursors", c_size, vp)
+_sig("adsr_quantified_num_fragments", c_size, vp)
+_sig("adsr_quantified_export_precursors", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
+_sig("adsr_quantified_export_fragments", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+     vp, vp, vp)
+_sig("adsr_quantified_free", None, vp)
 _sig("adsr_device_candidates_len", c_size, vp)
 _sig("adsr_device_candidates_export", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
 _sig("adsr_device_candidates_to_host", I, vp, C.POINTER(vp))

@@ -4048,6 +4048,11 @@ struct ScoreArgs {
   uint8_t *found;
   uint32_t *status;
   uint32_t lds_per_wave;
+  // quantification outputs: per candidate and fragment slot [n][ks]
+  float *q_int, *q_mzo, *q_corr, *q_err;
+  uint32_t *q_raw;           // library fragment index of the slot
+  uint32_t *q_nf, *q_row;    // per candidate: fragments, library row
+  float *q_rt;               // per candidate: RT at cycle_center
 };
 
 constexpr uint32_t kStScoreWindowTooLong = 1u << 8;
@@ -4170,6 +4175,10 @@ __host__ __device__ inline uint32_t score_lds_bytes(uint32_t ks, uint32_t L) {
   return (5 * ks * L + 10 * ks + 2 * kScoreLMax) * 4;
 }
 
+// QUANT: PeakGroupQuantification.quantify (src/peak_group_quantification/
+// mod.rs:86-168) -- the same extraction, filtered median profile,
+// correlations, summed intensities and m/z errors, written per fragment.
+template <bool QUANT>
 __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   extern __shared__ __align__(16) char smem[];
   __shared__ LogfEntry ltab[16];
@@ -4201,6 +4210,13 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   const uint32_t row = lo;
   const uint64_t c0 = a.c_start[ci], c1 = a.c_stop[ci];
   const uint64_t L64 = c1 - c0;  // usize arithmetic wraps (release build)
+  if constexpr (QUANT) {  // quantification/mod.rs:93-98 -> None
+    const uint64_t cc = a.c_center[ci];
+    if (L64 == 0 || cc < c0 || cc >= c1) {
+      if (lane == 0) a.found[ci] = 0;
+      return;
+    }
+  }
   if (L64 > (uint64_t)kScoreLMax) {
     if (lane == 0) {
       a.found[ci] = 0;
@@ -4450,6 +4466,15 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     const float m = wt > 0.0f ? ws / wt : 0.0f;
     const float e = m > 0.0f ? ((m - fmz) / fmz) * 1e6f : 0.0f;
     s.err[f] = e;
+    if constexpr (QUANT) {
+      const size_t q = (size_t)ci * ks + f;
+      a.q_int[q] = oint;
+      a.q_mzo[q] = m;
+      // quantification/mod.rs:121-128 correlates (profile, median)
+      a.q_corr[q] = corr_safe(s.norm + f * L, s.medf, L);
+      a.q_err[q] = e;
+      a.q_raw[q] = (uint32_t)(a.lib.f_start[row] + a.plan.sel[(size_t)row * ks + f]);
+    }
     s.wie[f] = 1.0f / (fabsf(e) + 0.1f);
     // src/idf/mod.rs:35-49
     const uint32_t df = a.idf_counts ? a.idf_counts[closest_bin(a.dia, fmzl)] : 0u;
@@ -4457,6 +4482,17 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
                : logf_exact(a.idf_total / (float)(df > 1u ? df : 1u), ltab);
   }
   __builtin_amdgcn_wave_barrier();
+  if constexpr (QUANT) {
+    if (lane == 0) {
+      a.q_nf[ci] = F;
+      a.q_row[ci] = row;
+      const uint64_t cc = a.c_center[ci];
+      if (cc >= a.dia.n_rt) atomicOr(a.status, kStScoreRtIndex);
+      a.q_rt[ci] = cc < a.dia.n_rt ? a.dia.rt[cc] : 0.0f;
+      a.found[ci] = 1;
+    }
+    return;
+  }
   float wmae = 0.0f;
   {
     float ws = 0.0f, wt = 0.0f;
@@ -4599,15 +4635,20 @@ static adsr_status upload_lib_scoring(adsr_lib *l, const KDia &kd) {
 }
 
 // Both entry points: `hc` (host table) or `dc` (table in HBM).
+// With qout set, PeakGroupQuantification.quantify with the tolerance and
+// top-k of p (src/peak_group_quantification/mod.rs:39-168).
 static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
                                 const adsr_candidates *hc,
                                 const adsr_device_candidates *dc,
                                 const adsr_scoring_params *p,
-                                adsr_features **out) {
+                                adsr_features **out,
+                                adsr_quantified **qout = nullptr) {
   clear_error();
-  if (!dc_ || !lc || !p || !out || (!hc && !dc))
+  const bool quant = qout != nullptr;
+  if (!dc_ || !lc || !p || (!out && !quant) || (!hc && !dc))
     return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
-  *out = nullptr;
+  if (out) *out = nullptr;
+  if (qout) *qout = nullptr;
   adsr_dia *d = const_cast<adsr_dia *>(dc_);
   adsr_lib *l = const_cast<adsr_lib *>(lc);
   adsr_status st = upload_dia(d);
@@ -4622,7 +4663,12 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   uint64_t lmax = 0;
   if (hc) {
     for (size_t i = 0; i < n; ++i) {
-      lmax = std::max<uint64_t>(lmax, hc->cycle_stop[i] - hc->cycle_start[i]);
+      const uint64_t li = hc->cycle_stop[i] - hc->cycle_start[i];
+      // quantification skips these rows before any extraction (mod.rs:93-98)
+      if (quant && (li == 0 || hc->cycle_center[i] < hc->cycle_start[i] ||
+                    hc->cycle_center[i] >= hc->cycle_stop[i]))
+        continue;
+      lmax = std::max<uint64_t>(lmax, li);
       if (hc->cycle_center[i] >= nrt)
         return set_error(ADSR_ERR_INVALID_ARGUMENT,
                          "candidate %zu: cycle_center %llu is outside the RT "
@@ -4676,11 +4722,20 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   float *csc, *feat;
   uint8_t *found;
   uint32_t *status;
+  const size_t ks = sp.plan.ks;
+  const size_t nq = quant ? n * ks : 0;
+  float *q_int = nullptr, *q_mzo = nullptr, *q_corr = nullptr, *q_err = nullptr,
+        *q_rt = nullptr;
+  uint32_t *q_raw = nullptr, *q_nf = nullptr, *q_row = nullptr;
   if (tmp.alloc(&cpidx, n) || tmp.alloc(&crank, n) || tmp.alloc(&cs, n) ||
       tmp.alloc(&ce, n) || tmp.alloc(&cc, n) || tmp.alloc(&csc, n) ||
-      tmp.alloc(&feat, (size_t)ADSR_N_FEATURES * n) || tmp.alloc(&found, n) ||
-      tmp.alloc(&status, 1))
+      tmp.alloc(&feat, quant ? 1 : (size_t)ADSR_N_FEATURES * n) ||
+      tmp.alloc(&found, n) || tmp.alloc(&status, 1))
     return fail(set_error(ADSR_ERR_OUT_OF_MEMORY, "scoring buffers"));
+  if (quant && (tmp.alloc(&q_int, nq) || tmp.alloc(&q_mzo, nq) || tmp.alloc(&q_corr, nq) ||
+                tmp.alloc(&q_err, nq) || tmp.alloc(&q_raw, nq) || tmp.alloc(&q_nf, n) ||
+                tmp.alloc(&q_row, n) || tmp.alloc(&q_rt, n)))
+    return fail(set_error(ADSR_ERR_OUT_OF_MEMORY, "quantification buffers"));
   if (hipMemsetAsync(status, 0, 4, w.stream) != hipSuccess)
     return fail(set_error(ADSR_ERR_DEVICE, "status reset failed"));
   if (n && hc) {
@@ -4717,6 +4772,14 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   a.feat = feat;
   a.found = found;
   a.status = status;
+  a.q_int = q_int;
+  a.q_mzo = q_mzo;
+  a.q_corr = q_corr;
+  a.q_err = q_err;
+  a.q_raw = q_raw;
+  a.q_nf = q_nf;
+  a.q_row = q_row;
+  a.q_rt = q_rt;
   a.lds_per_wave = (score_lds_bytes(sp.plan.ks, (uint32_t)lmax) + 15) / 16 * 16;
   int wpb = 4;
   while (wpb > 1 && (size_t)a.lds_per_wave * wpb > 64 * 1024) wpb >>= 1;
@@ -4724,12 +4787,112 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
     return fail(set_error(ADSR_ERR_UNSUPPORTED, "scoring LDS of %u bytes", a.lds_per_wave));
   (void)hipEventRecord(w.ev[2], w.stream);
   if (n) {
-    score_kernel<<<(uint32_t)((n + wpb - 1) / wpb), wpb * kWave,
-                   (size_t)a.lds_per_wave * wpb, w.stream>>>(a);
+    const dim3 grid((uint32_t)((n + wpb - 1) / wpb)), block(wpb * kWave);
+    const size_t lds = (size_t)a.lds_per_wave * wpb;
+    if (quant)
+      score_kernel<true><<<grid, block, lds, w.stream>>>(a);
+    else
+      score_kernel<false><<<grid, block, lds, w.stream>>>(a);
     if (hipGetLastError() != hipSuccess)
       return fail(set_error(ADSR_ERR_DEVICE, "score kernel launch failed"));
   }
   (void)hipEventRecord(w.ev[3], w.stream);
+  if (quant) {
+    std::vector<float> qi(nq), qm(nq), qc(nq), qe(nq), qrt(n);
+    std::vector<uint32_t> qraw(nq), qnf(n), qrow(n);
+    std::vector<uint8_t> hfound(n);
+    std::vector<uint64_t> hp(n), hr(n);
+    uint32_t hstatus = 0;
+    auto dl = [&](void *dst, const void *src, size_t bytes) {
+      return bytes == 0 ||
+             hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, w.stream) == hipSuccess;
+    };
+    bool ok = dl(&hstatus, status, 4) && dl(qi.data(), q_int, nq * 4) &&
+              dl(qm.data(), q_mzo, nq * 4) && dl(qc.data(), q_corr, nq * 4) &&
+              dl(qe.data(), q_err, nq * 4) && dl(qraw.data(), q_raw, nq * 4) &&
+              dl(qnf.data(), q_nf, n * 4) && dl(qrow.data(), q_row, n * 4) &&
+              dl(qrt.data(), q_rt, n * 4) && dl(hfound.data(), found, n) &&
+              dl(hp.data(), cpidx, n * 8) && dl(hr.data(), crank, n * 8);
+    if (!ok || hipStreamSynchronize(w.stream) != hipSuccess)
+      return fail(set_error(ADSR_ERR_DEVICE, "quantification failed: %s",
+                            hipGetErrorString(hipGetLastError())));
+    if (hstatus & kStScoreRtIndex)
+      return fail(set_error(ADSR_ERR_INVALID_ARGUMENT,
+                            "a candidate's cycle_center is outside the RT axis"));
+    if (hstatus & kStScoreWindowTooLong)
+      return fail(set_error(ADSR_ERR_UNSUPPORTED, "a candidate cycle range exceeds %d",
+                            kScoreLMax));
+    // precursor_quantified/mod.rs:26-94 (fragments with intensity > 0, none
+    // -> no row) and speclib_flat_quantified/mod.rs:213-330 (flat fragments
+    // in candidate order, precursor columns stably sorted by precursor_idx)
+    auto *q = new adsr_quantified();
+    std::vector<size_t> prow;  // library row of each quantified precursor
+    uint64_t pos = 0;
+    for (size_t i = 0; i < n; ++i) {
+      if (!hfound[i]) continue;
+      const size_t base = i * ks;
+      size_t kept = 0;
+      for (uint32_t f = 0; f < qnf[i]; ++f) kept += qi[base + f] > 0.0f ? 1 : 0;
+      if (!kept) continue;
+      const size_t r = qrow[i];
+      prow.push_back(r);
+      q->precursor_idx.push_back(l->precursor_idx[r]);
+      q->rank.push_back(hr[i]);
+      q->rt_observed.push_back(qrt[i]);
+      q->frag_start.push_back(pos);
+      pos += kept;
+      q->frag_stop.push_back(pos);
+      for (uint32_t f = 0; f < qnf[i]; ++f) {
+        if (!(qi[base + f] > 0.0f)) continue;
+        const size_t j = qraw[base + f];
+        q->f_mz_library.push_back(l->fragment_mz_library[j]);
+        q->f_mz.push_back(l->fragment_mz[j]);
+        q->f_intensity.push_back(qi[base + f]);
+        q->f_cardinality.push_back(l->fragment_cardinality[j]);
+        q->f_charge.push_back(l->fragment_charge[j]);
+        q->f_loss_type.push_back(l->fragment_loss_type[j]);
+        q->f_number.push_back(l->fragment_number[j]);
+        q->f_position.push_back(l->fragment_position[j]);
+        q->f_type.push_back(l->fragment_type[j]);
+        q->f_mz_observed.push_back(qm[base + f]);
+        q->f_correlation.push_back(qc[base + f]);
+        q->f_mass_error.push_back(qe[base + f]);
+        q->f_precursor_idx.push_back(l->precursor_idx[r]);
+        q->f_rank.push_back(hr[i]);
+      }
+    }
+    const size_t m = prow.size();
+    std::vector<size_t> ord(m);
+    for (size_t i = 0; i < m; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+      return q->precursor_idx[x] < q->precursor_idx[y];
+    });
+    auto perm = [&](auto &v) {
+      auto c = v;
+      for (size_t i = 0; i < m; ++i) v[i] = c[ord[i]];
+    };
+    perm(q->precursor_idx);
+    perm(q->rank);
+    perm(q->rt_observed);
+    perm(q->frag_start);
+    perm(q->frag_stop);
+    perm(prow);
+    q->mz_library.resize(m);
+    q->mz.resize(m);
+    q->rt_library.resize(m);
+    q->rt.resize(m);
+    q->naa.resize(m);
+    for (size_t i = 0; i < m; ++i) {
+      q->mz_library[i] = l->precursor_mz_library[prow[i]];
+      q->mz[i] = l->precursor_mz[prow[i]];
+      q->rt_library[i] = l->precursor_rt_library[prow[i]];
+      q->rt[i] = l->precursor_rt[prow[i]];
+      q->naa[i] = l->precursor_naa[prow[i]];
+    }
+    ws_free(w);
+    *qout = q;
+    return ADSR_OK;
+  }
   std::vector<float> hf((size_t)ADSR_N_FEATURES * n);
   std::vector<uint8_t> hfound(n);
   std::vector<uint64_t> hp(n), hr(n);
@@ -4827,3 +4990,80 @@ extern "C" adsr_status adsr_features_timings(const adsr_features *f, double *pla
 }
 
 extern "C" void adsr_features_free(adsr_features *f) { delete f; }
+
+// src/peak_group_quantification/parameters.rs:18-29
+extern "C" void adsr_quant_params_default(adsr_quant_params *p) {
+  if (!p) return;
+  p->tolerance_ppm = 7.0f;
+  p->top_k_fragments = 10000;
+}
+
+extern "C" adsr_status adsr_quantify(const adsr_dia *d, const adsr_lib *l,
+                                     const adsr_candidates *c,
+                                     const adsr_quant_params *p,
+                                     adsr_quantified **out) {
+  clear_error();
+  if (!c || !p || !out) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  adsr_scoring_params sp{p->tolerance_ppm, p->top_k_fragments};
+  return score_common(d, l, c, nullptr, &sp, nullptr, out);
+}
+
+extern "C" size_t adsr_quantified_num_precursors(const adsr_quantified *q) {
+  return q ? q->precursor_idx.size() : 0;
+}
+
+extern "C" size_t adsr_quantified_num_fragments(const adsr_quantified *q) {
+  return q ? q->f_mz.size() : 0;
+}
+
+template <class T>
+static void copy_out(T *dst, const std::vector<T> &v) {
+  if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(T));
+}
+
+extern "C" adsr_status adsr_quantified_export_precursors(
+    const adsr_quantified *q, uint64_t *precursor_idx, float *mz_library,
+    float *mz, float *rt_library, float *rt, uint8_t *naa, uint64_t *rank,
+    float *rt_observed, uint64_t *flat_frag_start_idx,
+    uint64_t *flat_frag_stop_idx) {
+  clear_error();
+  if (!q) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  copy_out(precursor_idx, q->precursor_idx);
+  copy_out(mz_library, q->mz_library);
+  copy_out(mz, q->mz);
+  copy_out(rt_library, q->rt_library);
+  copy_out(rt, q->rt);
+  copy_out(naa, q->naa);
+  copy_out(rank, q->rank);
+  copy_out(rt_observed, q->rt_observed);
+  copy_out(flat_frag_start_idx, q->frag_start);
+  copy_out(flat_frag_stop_idx, q->frag_stop);
+  return ADSR_OK;
+}
+
+extern "C" adsr_status adsr_quantified_export_fragments(
+    const adsr_quantified *q, float *mz_library, float *mz, float *intensity,
+    uint8_t *cardinality, uint8_t *charge, uint8_t *loss_type, uint8_t *number,
+    uint8_t *position, uint8_t *type, float *mz_observed,
+    float *correlation_observed, float *mass_error_observed,
+    uint64_t *precursor_idx, uint64_t *rank) {
+  clear_error();
+  if (!q) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  copy_out(mz_library, q->f_mz_library);
+  copy_out(mz, q->f_mz);
+  copy_out(intensity, q->f_intensity);
+  copy_out(cardinality, q->f_cardinality);
+  copy_out(charge, q->f_charge);
+  copy_out(loss_type, q->f_loss_type);
+  copy_out(number, q->f_number);
+  copy_out(position, q->f_position);
+  copy_out(type, q->f_type);
+  copy_out(mz_observed, q->f_mz_observed);
+  copy_out(correlation_observed, q->f_correlation);
+  copy_out(mass_error_observed, q->f_mass_error);
+  copy_out(precursor_idx, q->f_precursor_idx);
+  copy_out(rank, q->f_rank);
+  return ADSR_OK;
+}
+
+extern "C" void adsr_quantified_free(adsr_quantified *q) { delete q; }

@@ -169,6 +169,20 @@ struct adsr_features {
   double plan_ms = 0, kernel_ms = 0;
 };
 
+// SpecLibFlatQuantified (src/speclib_flat_quantified/mod.rs:18-82):
+// precursor columns sorted by precursor_idx, fragment columns in candidate
+// order with per-precursor [start, stop) ranges.
+struct adsr_quantified {
+  std::vector<uint64_t> precursor_idx, rank, frag_start, frag_stop;
+  std::vector<float> mz_library, mz, rt_library, rt, rt_observed;
+  std::vector<uint8_t> naa;
+  std::vector<float> f_mz_library, f_mz, f_intensity, f_mz_observed,
+      f_correlation, f_mass_error;
+  std::vector<uint8_t> f_cardinality, f_charge, f_loss_type, f_number,
+      f_position, f_type;
+  std::vector<uint64_t> f_precursor_idx, f_rank;
+};
+
 struct adsr_candidates {
   // src/candidate/entry.rs:5-16
   std::vector<uint64_t> precursor_idx, rank, scan_center, scan_start,

@@ -349,6 +349,41 @@ adsr_status adsr_features_timings(const adsr_features *f, double *plan_ms,
                                   double *kernel_ms);
 void adsr_features_free(adsr_features *f);
 
+/* ---- PeakGroupQuantification (src/peak_group_quantification/mod.rs) -- */
+/* QuantificationParameters (src/peak_group_quantification/parameters.rs) */
+typedef struct adsr_quant_params {
+  float tolerance_ppm;      /* default 7.0 */
+  uint64_t top_k_fragments; /* default 10000 (all) */
+} adsr_quant_params;
+void adsr_quant_params_default(adsr_quant_params *p);
+/* SpecLibFlatQuantified (src/speclib_flat_quantified/mod.rs:18-82) */
+typedef struct adsr_quantified adsr_quantified;
+/* PeakGroupQuantification::quantify (mod.rs:39-168): per candidate the
+ * dense XIC and m/z, observed intensities (sum over cycles), m/z, mass
+ * errors and correlations with the filtered median profile; fragments with
+ * zero observed intensity dropped (precursor_quantified/mod.rs:26-94);
+ * precursor columns sorted by precursor_idx.  Same limits as adsr_score. */
+adsr_status adsr_quantify(const adsr_dia *d, const adsr_lib *l,
+                          const adsr_candidates *c, const adsr_quant_params *p,
+                          adsr_quantified **out);
+size_t adsr_quantified_num_precursors(const adsr_quantified *q);
+size_t adsr_quantified_num_fragments(const adsr_quantified *q);
+/* to_dict_arrays precursor columns (speclib_flat_quantified/mod.rs:126-160),
+ * num_precursors elements each; any pointer may be NULL. */
+adsr_status adsr_quantified_export_precursors(
+    const adsr_quantified *q, uint64_t *precursor_idx, float *mz_library,
+    float *mz, float *rt_library, float *rt, uint8_t *naa, uint64_t *rank,
+    float *rt_observed, uint64_t *flat_frag_start_idx,
+    uint64_t *flat_frag_stop_idx);
+/* to_dict_arrays fragment columns (mod.rs:161-205), num_fragments each. */
+adsr_status adsr_quantified_export_fragments(
+    const adsr_quantified *q, float *mz_library, float *mz, float *intensity,
+    uint8_t *cardinality, uint8_t *charge, uint8_t *loss_type, uint8_t *number,
+    uint8_t *position, uint8_t *type, float *mz_observed,
+    float *correlation_observed, float *mass_error_observed,
+    uint64_t *precursor_idx, uint64_t *rank);
+void adsr_quantified_free(adsr_quantified *q);
+
 /* ---- per-stage device kernels (SimdBackend ops, src/simd/mod.rs:11-39) --
  * Host in/out buffers; the library stages them through HBM.               */
 /* GaussianKernel::new (src/kernel/mod.rs:13-30); returns the length. */

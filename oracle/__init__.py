@@ -113,6 +113,8 @@ def lib():
         sig("orc_idf", f32, vp, f32, f32)
         sig("orc_score", None, C.POINTER(OrcDia), vp, f32, sz, sz, vp, vp, vp, vp, vp,
             vp, vp, C.c_int)
+        sig("orc_quantify", None, C.POINTER(OrcDia), vp, f32, sz, sz, vp, vp, vp, vp, sz,
+            vp, vp, vp, vp, vp, vp, vp, vp)
         _lib = L
     return _lib
 
@@ -361,3 +363,82 @@ def score(dia: Dia, library: Lib, candidates, mass_tolerance=7.0, top_k_fragment
     for i, name in enumerate(FEATURES):
         out[name] = feat[i, :n][m].copy()
     return out
+
+
+def quantify(dia: Dia, library: Lib, lib_arrays, candidates, tolerance_ppm=7.0,
+             top_k_fragments=10000):
+    """PeakGroupQuantification.quantify (src/peak_group_quantification/mod.rs:
+    39-168): (precursor dict, fragment dict) as SpecLibFlatQuantified.
+    to_dict_arrays (src/speclib_flat_quantified/mod.rs:126-211).  The
+    per-candidate work is the C restatement; the intensity > 0 filter
+    (src/precursor_quantified/mod.rs:26-94) and the flat layout
+    (speclib_flat_quantified/mod.rs:213-330) are restated here."""
+    (pidx, pmzl, pmz, prtl, prt, naa, fs, fe, fmzl, fmz, fint, card, chg, loss,
+     fnum, fpos, ftype) = lib_arrays
+    order = np.argsort(pidx, kind="stable")  # SpecLibFlat's stable sort
+    cp, _rank, _sc, _a, _b, _c, cc, cs, ce = candidates
+    rank = np.asarray(candidates[1], np.uint64)
+    n = len(cp)
+    ks = int(max(1, (np.asarray(fe, np.int64) - np.asarray(fs, np.int64)).max() if len(fs) else 1))
+    k = [np.ascontiguousarray(cp, np.uint64), np.ascontiguousarray(cs, np.uint64),
+         np.ascontiguousarray(ce, np.uint64), np.ascontiguousarray(cc, np.uint64)]
+    row = np.zeros(max(n, 1), np.uintp)
+    nf = np.zeros(max(n, 1), np.uintp)
+    raw = np.zeros(max(n * ks, 1), np.uintp)
+    inten, mzo, corr, err = (np.zeros(max(n * ks, 1), np.float32) for _ in range(4))
+    rto = np.zeros(max(n, 1), np.float32)
+    lib().orc_quantify(dia.h, library.h, float(tolerance_ppm), int(top_k_fragments), n,
+                       p(k[0]), p(k[1]), p(k[2]), p(k[3]), ks, p(row), p(nf), p(raw),
+                       p(inten), p(mzo), p(corr), p(err), p(rto))
+    prec = {key: [] for key in ("precursor_idx", "mz_library", "mz", "rt_library", "rt",
+                                "naa", "rank", "rt_observed", "flat_frag_start_idx",
+                                "flat_frag_stop_idx")}
+    frag = {key: [] for key in ("mz_library", "mz", "intensity", "cardinality", "charge",
+                                "loss_type", "number", "position", "type", "mz_observed",
+                                "correlation_observed", "mass_error_observed",
+                                "precursor_idx", "rank")}
+    pos = 0
+    none = np.iinfo(np.uintp).max
+    for i in range(n):
+        if row[i] == none:
+            continue
+        sl = slice(i * ks, i * ks + int(nf[i]))
+        keep = np.flatnonzero(inten[sl] > 0.0)
+        if keep.size == 0:  # filter_fragments_by_intensity -> None
+            continue
+        r = int(order[int(row[i])])  # the row in the caller's array order
+        rw = raw[sl][keep].astype(np.int64)
+        for key, src in (("precursor_idx", pidx), ("mz_library", pmzl), ("mz", pmz),
+                         ("rt_library", prtl), ("rt", prt), ("naa", naa)):
+            prec[key].append(src[r])
+        prec["rank"].append(rank[i])
+        prec["rt_observed"].append(rto[i])
+        prec["flat_frag_start_idx"].append(pos)
+        pos += keep.size
+        prec["flat_frag_stop_idx"].append(pos)
+        for key, src in (("mz_library", fmzl), ("mz", fmz), ("cardinality", card),
+                         ("charge", chg), ("loss_type", loss), ("number", fnum),
+                         ("position", fpos), ("type", ftype)):
+            frag[key].append(np.asarray(src)[rw])
+        frag["intensity"].append(inten[sl][keep])
+        frag["mz_observed"].append(mzo[sl][keep])
+        frag["correlation_observed"].append(corr[sl][keep])
+        frag["mass_error_observed"].append(err[sl][keep])
+        frag["precursor_idx"].append(np.full(keep.size, pidx[r], np.uint64))
+        frag["rank"].append(np.full(keep.size, rank[i], np.uint64))
+    ptypes = {"precursor_idx": np.uint64, "mz_library": np.float32, "mz": np.float32,
+              "rt_library": np.float32, "rt": np.float32, "naa": np.uint8,
+              "rank": np.uint64, "rt_observed": np.float32,
+              "flat_frag_start_idx": np.uint64, "flat_frag_stop_idx": np.uint64}
+    pout = {key: np.asarray(v, dtype=ptypes[key]) for key, v in prec.items()}
+    srt = np.argsort(pout["precursor_idx"], kind="stable")  # sort_by_key, stable
+    pout = {key: v[srt] for key, v in pout.items()}
+    ftypes = {"mz_library": np.float32, "mz": np.float32, "intensity": np.float32,
+              "cardinality": np.uint8, "charge": np.uint8, "loss_type": np.uint8,
+              "number": np.uint8, "position": np.uint8, "type": np.uint8,
+              "mz_observed": np.float32, "correlation_observed": np.float32,
+              "mass_error_observed": np.float32, "precursor_idx": np.uint64,
+              "rank": np.uint64}
+    fout = {key: (np.concatenate(v).astype(ftypes[key]) if v else np.zeros(0, ftypes[key]))
+            for key, v in frag.items()}
+    return pout, fout

@@ -270,6 +270,14 @@ void orc_dense_xic_mz(const orc_dia *d, float precursor_mz, uint64_t c0,
                       size_t nf, float *xic, float *mz);
 uint32_t *orc_idf_counts(const float *fragment_mz_library, size_t n);
 float orc_idf(const uint32_t *counts, float total, float mz);
+/* PeakGroupQuantification.quantify per candidate, before the intensity
+ * filter and the flat layout (see adsr_oracle_score.c) */
+void orc_quantify(const orc_dia *d, const orc_lib *l, float tolerance_ppm,
+                  size_t top_k, size_t n, const uint64_t *precursor_idx,
+                  const uint64_t *cycle_start, const uint64_t *cycle_stop,
+                  const uint64_t *cycle_center, size_t ks, size_t *row,
+                  size_t *nf, size_t *raw, float *intensity, float *mz_obs,
+                  float *corr, float *mass_err, float *rt_observed);
 /* PeakGroupScoring.score over n candidates: features [ORC_N_FEATURES][n]
  * (column-major), found[i] = 0 where precursor_idx is not in the library.
  * The library must come from orc_lib_build_full. */

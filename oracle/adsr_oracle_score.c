@@ -578,3 +578,66 @@ void orc_score(const orc_dia *d, const orc_lib *l, float mass_tolerance,
   pthread_mutex_destroy(&c.mu);
   free(counts);
 }
+
+/* ---- src/peak_group_quantification/mod.rs:39-168 ---------------------- */
+/* Per candidate i: row[i] = library row (sorted order) or (size_t)-1 when
+ * the candidate yields no PrecursorQuantified (precursor not found,
+ * num_cycles == 0, centre outside [start, stop)); nf[i] selected fragments
+ * and, in slots [i * ks, i * ks + nf[i]), the raw fragment index (library
+ * fragment order) and the observed intensity (sum over cycles), m/z,
+ * correlation with the filtered median profile and mass error (ppm).  The
+ * intensity > 0 filter (precursor_quantified/mod.rs:26-94) and the flat
+ * layout are left to the caller. */
+void orc_quantify(const orc_dia *d, const orc_lib *l, float tolerance_ppm,
+                  size_t top_k, size_t n, const uint64_t *precursor_idx,
+                  const uint64_t *cycle_start, const uint64_t *cycle_stop,
+                  const uint64_t *cycle_center, size_t ks, size_t *row,
+                  size_t *nf, size_t *raw, float *intensity, float *mz_obs,
+                  float *corr, float *mass_err, float *rt_observed) {
+  for (size_t ci = 0; ci < n; ci++) {
+    row[ci] = (size_t)-1;
+    nf[ci] = 0;
+    uint64_t key = precursor_idx[ci];
+    size_t lo = 0, hi = l->n_prec;
+    while (lo < hi) {
+      size_t mid = lo + (hi - lo) / 2;
+      if (l->precursor_idx[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo >= l->n_prec || l->precursor_idx[lo] != key) continue;
+    uint64_t c0 = cycle_start[ci], c1 = cycle_stop[ci], cc = cycle_center[ci];
+    uint64_t L = c1 - c0; /* mod.rs:93-98 */
+    if (L == 0 || cc < c0 || cc >= c1) continue;
+    size_t fs = l->frag_start[lo], nraw = l->frag_stop[lo] - fs;
+    size_t *ord = malloc((nraw ? nraw : 1) * sizeof(size_t));
+    size_t F = orc_filter_sort_fragments(nraw, l->fragment_mz + fs,
+                                         l->fragment_intensity + fs,
+                                         l->fragment_number + fs,
+                                         l->fragment_type + fs, 1, 1, top_k, ord);
+    if (F > ks) F = ks; /* caller sizes ks to the longest selection */
+    size_t fz = F ? F : 1, cells = F * L;
+    if (cells == 0) cells = 1;
+    float *fmz = malloc(fz * 4), *xic = malloc(cells * 4), *dmz = malloc(cells * 4);
+    float *norm = malloc(cells * 4), *filt = malloc(cells * 4), *med = malloc(L * 4);
+    for (size_t f = 0; f < F; f++) fmz[f] = l->fragment_mz[fs + ord[f]];
+    orc_dense_xic_mz(d, l->precursor_mz[lo], c0, c1, tolerance_ppm, fmz, F, xic, dmz);
+    float *mo = malloc(fz * 4), *er = malloc(fz * 4);
+    orc_fragment_mz_errors(dmz, xic, F, L, fmz, mo, er);
+    orc_normalize_profiles(xic, F, L, 1, norm);
+    size_t Ff = orc_filter_non_zero(norm, F, L, filt);
+    orc_median_axis_0(filt, Ff, L, med);
+    for (size_t f = 0; f < F; f++) {
+      size_t s = ci * ks + f;
+      raw[s] = fs + ord[f];
+      corr[s] = orc_correlation_safe(norm + f * L, med, L); /* :121-128 */
+      intensity[s] = orc_ndarray_sum(xic + f * L, L);      /* :131 */
+      mz_obs[s] = mo[f];
+      mass_err[s] = er[f];
+    }
+    rt_observed[ci] = d->rt[cc];
+    row[ci] = lo;
+    nf[ci] = F;
+    free(ord); free(fmz); free(xic); free(dmz); free(norm); free(filt);
+    free(med); free(mo); free(er);
+  }
+}
