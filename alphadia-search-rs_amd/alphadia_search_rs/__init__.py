@@ -758,6 +758,13 @@ class SpecLibFlatQuantified:
     def num_fragments(self) -> int:
         return int(N.lib.adsr_quantified_num_fragments(self._h)) if self._h else 0
 
+    def device_timings(self) -> dict:
+        """Device time of the fragment plan and of the quantification
+        kernel (ms)."""
+        a, b = C.c_double(), C.c_double()
+        check(N.lib.adsr_quantified_timings(self._h, C.byref(a), C.byref(b)))
+        return {"plan_ms": a.value, "kernel_ms": b.value}
+
     def to_dict_arrays(self):
         """mod.rs:126-211: (precursor dict, fragment dict)."""
         n, m = self.num_precursors, self.num_fragments

@@ -150,6 +150,7 @@ _sig("adsr_quantified_num_fragments", c_size, vp)
 _sig("adsr_quantified_export_precursors", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
 _sig("adsr_quantified_export_fragments", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
      vp, vp, vp)
+_sig("adsr_quantified_timings", I, vp, C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("adsr_quantified_free", None, vp)
 _sig("adsr_device_candidates_len", c_size, vp)
 _sig("adsr_device_candidates_export", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)

@@ -4889,6 +4889,11 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
       q->rt[i] = l->precursor_rt[prow[i]];
       q->naa[i] = l->precursor_naa[prow[i]];
     }
+    float t_plan = 0, t_kern = 0;
+    (void)hipEventElapsedTime(&t_plan, w.ev[0], w.ev[1]);
+    (void)hipEventElapsedTime(&t_kern, w.ev[2], w.ev[3]);
+    q->plan_ms = t_plan;
+    q->kernel_ms = t_kern;
     ws_free(w);
     *qout = q;
     return ADSR_OK;
@@ -5067,3 +5072,12 @@ extern "C" adsr_status adsr_quantified_export_fragments(
 }
 
 extern "C" void adsr_quantified_free(adsr_quantified *q) { delete q; }
+
+extern "C" adsr_status adsr_quantified_timings(const adsr_quantified *q, double *plan_ms,
+                                               double *kernel_ms) {
+  clear_error();
+  if (!q) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  if (plan_ms) *plan_ms = q->plan_ms;
+  if (kernel_ms) *kernel_ms = q->kernel_ms;
+  return ADSR_OK;
+}

@@ -181,6 +181,7 @@ struct adsr_quantified {
   std::vector<uint8_t> f_cardinality, f_charge, f_loss_type, f_number,
       f_position, f_type;
   std::vector<uint64_t> f_precursor_idx, f_rank;
+  double plan_ms = 0, kernel_ms = 0;
 };
 
 struct adsr_candidates {

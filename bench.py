@@ -374,6 +374,32 @@ def main():
         result["config"]["scoring"] = scoring
         fd = feats.to_dict_arrays()
         del feats
+        # PeakGroupQuantification.quantify of the rank-0 candidates (the
+        # reference quantifies the post-FDR subset, about one per precursor)
+        tab = dcands.to_collection().to_arrays()
+        r0m = tab[1] == 0
+        qcols = tuple(c[r0m] for c in tab)
+        qcands = ads.CandidateCollection.from_arrays(*qcols)
+        qp = ads.QuantificationParameters()
+        qp.update({"tolerance_ppm": spec.mass_tolerance})
+        quant = ads.PeakGroupQuantification(qp)
+        quant.quantify(dia, lib, qcands)  # warm
+        t1 = time.perf_counter()
+        qlib = quant.quantify(dia, lib, qcands)
+        q_s = time.perf_counter() - t1
+        qt = qlib.device_timings()
+        result["config"]["quantification"] = {
+            "candidates": int(r0m.sum()),
+            "precursors": qlib.num_precursors,
+            "fragments": qlib.num_fragments,
+            "kernel_ms": round(qt["kernel_ms"], 3),
+            "candidates_per_s_device": round(int(r0m.sum()) / (qt["kernel_ms"] / 1e3), 1)
+            if qt["kernel_ms"] > 0 else None,
+            "call_s_incl_host_layout": round(q_s, 4),
+        }
+        log(f"quantification: {int(r0m.sum())} candidates, kernel {qt['kernel_ms']:.2f} ms, "
+            f"call {q_s * 1e3:.1f} ms")
+        del qlib
 
     if fb is not None:
         fb_rate = n_prec / (fb[0] / 1e3)
@@ -440,6 +466,22 @@ def main():
             })
             log(f"cpu oracle scoring {k / o_s:.0f} candidates/s on {nt} threads; "
                 f"sample features {'identical' if same else 'DIFFERENT'}")
+            # the oracle (one thread) quantifies a 30k sample of the rank-0
+            # table; the device quantifies the same sample for the comparison
+            qs = tuple(c[:30_000] for c in qcols)
+            qd = quant.quantify(dia, lib, ads.CandidateCollection.from_arrays(*qs)).to_dict_arrays()
+            t6 = time.perf_counter()
+            oq = oracle.quantify(odia, olib, gen["lib"], qs, tolerance_ppm=spec.mass_tolerance)
+            oq_s = time.perf_counter() - t6
+            qsame = all(np.array_equal(qd[i][k], oq[i][k]) for i in (0, 1) for k in oq[i])
+            result["config"]["quantification"].update({
+                "cpu_oracle_candidates_per_s": round(len(qs[0]) / oq_s, 1),
+                "cpu_oracle_threads": 1,
+                "cpu_sample_candidates": len(qs[0]),
+                "cpu_sample_identical": qsame,
+            })
+            log(f"cpu oracle quantification {len(qs[0]) / oq_s:.0f} candidates/s "
+                f"(1 thread); sample {'identical' if qsame else 'DIFFERENT'}")
         got = cand.to_arrays()
         pidx = got[0].astype(np.int64)
         m = ((pidx >= r0) & (pidx < r1)) | ((pidx >= r2) & (pidx < r3))

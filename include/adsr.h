@@ -382,6 +382,9 @@ adsr_status adsr_quantified_export_fragments(
     uint8_t *position, uint8_t *type, float *mz_observed,
     float *correlation_observed, float *mass_error_observed,
     uint64_t *precursor_idx, uint64_t *rank);
+/* Device time (ms) of the fragment plan and of the quantification kernel. */
+adsr_status adsr_quantified_timings(const adsr_quantified *q, double *plan_ms,
+                                    double *kernel_ms);
 void adsr_quantified_free(adsr_quantified *q);
 
 /* ---- per-stage device kernels (SimdBackend ops, src/simd/mod.rs:11-39) --
