@@ -43,6 +43,15 @@
  *     replaced by a stable sort, which picks the same set whenever the
  *     intensities at the cut differ
  *   - rayon collect order (documented as order preserving)
+ *
+ * Scoring and quantification (adsr_oracle_score.c) are pinned by the
+ * reference's unit tests of src/peak_group_scoring, src/utils, src/idf and
+ * src/dense_xic_observation, by numpy/lgamma closed forms, and by an
+ * independent float64 implementation of every feature
+ * (tests/test_oracle_score.py).  Parity unpinned there: log10f bits (glibc
+ * vs another libm), the sign of an all-zero f32 sum (Rust 1.88
+ * Iterator::sum folds from -0.0, restated so), and which row binary_search
+ * returns for a precursor_idx that two library rows share (the first).
  */
 #ifndef ADSR_ORACLE_H
 #define ADSR_ORACLE_H
