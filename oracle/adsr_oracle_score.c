@@ -42,7 +42,11 @@ float orc_correlation_safe(const float *x, const float *y, size_t n) {
     cov = cov + dx * dy;
   }
   if (xv == 0.0f || yv == 0.0f) return 0.0f;
+#if ORC_MUTANT == 7 /* planted: one variance used twice */
+  float c = cov / (sqrtf(xv) * sqrtf(xv));
+#else
   float c = cov / (sqrtf(xv) * sqrtf(yv));
+#endif
   if (c < -1.0f) c = -1.0f; /* clamp(-1, 1); NaN/inf panics there */
   if (c > 1.0f) c = 1.0f;
   return c;
@@ -81,7 +85,11 @@ float orc_hyperscore_weighted(const uint8_t *types, const float *inten,
   float fy = ny > 0 ? orc_gamma_ln((float)ny + 1.0f) : 0.0f;
   float lb = sb > 0.0f ? logf(sb) : 0.0f;
   float ly = sy > 0.0f ? logf(sy) : 0.0f;
+#if ORC_MUTANT == 9 /* planted: factorial terms dropped */
+  float h = lb + ly + 0.0f * (fb + fy);
+#else
   float h = fb + fy + lb + ly;
+#endif
   return isfinite(h) ? h : 0.0f;
 }
 
@@ -254,7 +262,11 @@ void orc_ranked_mask(const float *v, size_t n, size_t r1, size_t r2,
     if (r1 >= r2) continue;
     size_t r = 0;
     for (size_t j = 0; j < n; j++)
+#if ORC_MUTANT == 8 /* planted: ascending rank */
+      if (v[j] < v[i] || (!(v[j] < v[i]) && !(v[j] > v[i]) && j < i)) r++;
+#else
       if (v[j] > v[i] || (!(v[j] < v[i]) && !(v[j] > v[i]) && j < i)) r++;
+#endif
     mask[i] = r >= r1 && r < r2;
   }
 }
@@ -326,7 +338,11 @@ void orc_fill_xic_and_mz_slice(const orc_dia *d, size_t obs_idx, float *xic,
       if (v > 0.0f) {
         float prev = xic[rel] - v;
         if (prev == 0.0f) dmz[rel] = actual;
+#if ORC_MUTANT == 11 /* planted: unweighted running m/z */
+        else dmz[rel] = (dmz[rel] + actual) / 2.0f;
+#else
         else dmz[rel] = (dmz[rel] * prev + actual * v) / xic[rel];
+#endif
       }
     }
   }
@@ -366,7 +382,11 @@ float orc_idf(const uint32_t *counts, float total, float mz) {
   size_t B;
   const float *g = orc_global_grid(&B);
   uint32_t df = counts[orc_find_closest_index(g, B, mz)];
+#if ORC_MUTANT == 10 /* planted: document frequency not floored at 1 */
+  return logf(total / (float)df);
+#else
   return logf(total / (float)(df > 1 ? df : 1));
+#endif
 }
 
 typedef struct {
@@ -432,7 +452,11 @@ static void score_one(score_ctx *c, size_t ci) {
   /* mod.rs:130-143 */
   float *norm = malloc(cells * 4), *filt = malloc(cells * 4);
   float *med = malloc((L ? L : 1) * 4), *medf = malloc((L ? L : 1) * 4);
+#if ORC_MUTANT == 12 /* planted: centre only, no dilation */
+  orc_normalize_profiles(xic, F, L, 0, norm);
+#else
   orc_normalize_profiles(xic, F, L, 1, norm);
+#endif
   size_t Ff = orc_filter_non_zero(norm, F, L, filt);
   orc_median_axis_0(norm, F, L, med);
   orc_median_axis_0(filt, Ff, L, medf);

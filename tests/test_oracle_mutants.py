@@ -11,6 +11,7 @@ import pytest
 
 import oracle
 import test_oracle as T
+import test_oracle_score as TS
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(os.path.dirname(HERE), "oracle", "adsr_oracle.c")
@@ -30,6 +31,16 @@ MUTANTS = {
     5: ("binary search returns the first equal element",
         [T.test_rust_bsearch_last_equal_and_insertion_point]),
     6: ("Gaussian exponent without the 1/2", [T.test_kernel_closed_form_and_symmetry]),
+    # scoring (adsr_oracle_score.c); "features" is the float64 feature check
+    # (the reference's correlation KATs use y = 2x and y = -x, where this
+    # mistake is clamped back to +-1; random vectors against numpy catch it)
+    7: ("correlation with one variance twice",
+        [TS.test_correlation_median_std_against_numpy, "features"]),
+    8: ("ranked mask in ascending order", [TS.test_crate_utils_kats, "features"]),
+    9: ("hyperscore without the factorial terms", ["features"]),
+    10: ("document frequency not floored at 1", [TS.test_idf_kats]),
+    11: ("unweighted running m/z", [TS.test_dense_xic_mz_kats]),
+    12: ("profiles normalised by the centre cycle only", ["features"]),
 }
 
 
@@ -48,18 +59,27 @@ def builds(tmp_path_factory):
     return paths
 
 
+def _run_check(check, run0):
+    if check == "features":
+        # handles built by the library loaded now (a mutant or the oracle)
+        TS.test_score_features_against_float64_implementation(
+            run0, (oracle.Dia(run0["dia"]), oracle.Lib(run0["lib"])))
+    else:
+        check()
+
+
 @pytest.mark.parametrize("m", sorted(MUTANTS), ids=lambda m: MUTANTS[m][0])
-def test_planted_mistake_is_caught(builds, m):
+def test_planted_mistake_is_caught(builds, m, run0):
     prev = oracle.use_library(builds[m])
     try:
         for check in MUTANTS[m][1]:
             with pytest.raises(AssertionError):
-                check()
+                _run_check(check, run0)
     finally:
         oracle.use_library(prev)
 
 
-def test_checks_pass_on_the_oracle():
+def test_checks_pass_on_the_oracle(run0):
     for _, checks in MUTANTS.values():
         for check in checks:
-            check()
+            _run_check(check, run0)
