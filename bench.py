@@ -445,6 +445,9 @@ def main():
             tab = dcands.to_collection().to_arrays()
             k = min(len(tab[0]), 100_000)
             samp = tuple(c[:k] for c in tab)
+            # the reference builds the IDF table with the library: warm it
+            oracle.score(odia, olib, tuple(c[:1] for c in tab),
+                         mass_tolerance=spec.mass_tolerance, nthreads=nt)
             t5 = time.perf_counter()
             ofeat = oracle.score(odia, olib, samp, mass_tolerance=spec.mass_tolerance,
                                  nthreads=nt)

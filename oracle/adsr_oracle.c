@@ -722,6 +722,7 @@ void orc_lib_free(orc_lib *l) {
   if (!l) return;
   free(l->precursor_naa);
   free(l->fragment_mz_library);
+  free(l->idf_counts);
   free(l->precursor_idx);
   free(l->precursor_mz);
   free(l->precursor_rt);

@@ -162,6 +162,8 @@ typedef struct orc_lib {
   uint8_t *fragment_number, *fragment_type;
   uint8_t *precursor_naa;      /* sorted order; scoring only (may be NULL) */
   float *fragment_mz_library;  /* fragment order; scoring only (may be NULL) */
+  uint32_t *idf_counts;        /* built by the first orc_score (the
+                                  reference builds it with the library) */
 } orc_lib;
 
 orc_lib *orc_lib_build(size_t n_prec, const uint64_t *precursor_idx,

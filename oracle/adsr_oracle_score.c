@@ -580,8 +580,10 @@ void orc_score(const orc_dia *d, const orc_lib *l, float mass_tolerance,
   memset(&c, 0, sizeof(c));
   c.d = d;
   c.l = l;
-  uint32_t *counts = orc_idf_counts(l->fragment_mz_library, l->n_frag);
-  c.idf_counts = counts;
+  /* src/speclib_flat/mod.rs:165-166 builds the IDF once per library */
+  orc_lib *lm = (orc_lib *)l;
+  if (!lm->idf_counts) lm->idf_counts = orc_idf_counts(l->fragment_mz_library, l->n_frag);
+  c.idf_counts = lm->idf_counts;
   c.idf_total = (float)l->n_frag;
   c.tol = mass_tolerance;
   c.top_k = top_k;
@@ -600,7 +602,6 @@ void orc_score(const orc_dia *d, const orc_lib *l, float mass_tolerance,
   for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, score_worker, &c);
   for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
   pthread_mutex_destroy(&c.mu);
-  free(counts);
 }
 
 /* ---- src/peak_group_quantification/mod.rs:39-168 ---------------------- */
