@@ -986,12 +986,22 @@ void select_kernel(SelArgs a) {
           uint8_t *ow = l.own + fl * l.lrow + t;
           if (v) *ow = (uint8_t)lane;
           __builtin_amdgcn_wave_barrier();
-          const bool dup = v && *ow != (uint8_t)lane;
+          // lanes whose cell another lane overwrote share it with that lane
+          const bool lost = v && *ow != (uint8_t)lane;
           float *xr = l.X + fl * l.lp + kXPad;
-          if (ballot64(dup) == 0) {
+          if (ballot64(lost) == 0) {
             if (v) xr[t] = xr[t] + inten;
           } else {
-            uint64_t it2 = ballot64(v);
+            // The losers write again, so that the lane that kept a shared
+            // cell reads another id: every lane of a shared cell is then
+            // marked.  Unshared cells add at once, shared ones one lane at a
+            // time in lane (= entry) order.
+            if (lost) *ow = (uint8_t)lane;
+            __builtin_amdgcn_wave_barrier();
+            const bool shared = lost || (v && *ow != (uint8_t)lane);
+            if (v && !shared) xr[t] = xr[t] + inten;
+            __builtin_amdgcn_wave_barrier();
+            uint64_t it2 = ballot64(shared);
             while (it2) {
               const int j2 = __builtin_ctzll(it2);
               it2 &= it2 - 1;
@@ -1022,10 +1032,14 @@ void select_kernel(SelArgs a) {
         }
       };
       // Lists the hits of fragments [fa, fb) of every window.  Deferred: the
-      // list must hold the whole row (false when it would overflow).
-      // Direct: the list is accumulated into X whenever it fills and at the
-      // end of each window.
-      auto extract = [&](uint32_t fa, uint32_t fb, bool deferred) -> bool {
+      // list holds whole fragments; the first fragment not listed is
+      // returned (fb when all are).  When the list would overflow, a row
+      // with one window keeps the fragments before the one being scanned
+      // (the list is in fragment order); otherwise nothing is kept and fa is
+      // returned.  Direct: the list is accumulated into X whenever it fills
+      // and at the end of each window.
+      auto extract = [&](uint32_t fa, uint32_t fb,
+                         bool deferred) __attribute__((always_inline)) -> uint32_t {
         const uint32_t FN = fb - fa;
         for (uint32_t j = 0; j < nobs; ++j) {
           const uint32_t o = win_obs(j);
@@ -1115,18 +1129,18 @@ void select_kernel(SelArgs a) {
                                        raw[u].y & 0xffffu, raw[u].y >> 16,
                                        raw[u].z & 0xffffu, raw[u].z >> 16,
                                        raw[u].w & 0xffffu, raw[u].w >> 16};
-              uint32_t hm = 0;
+              uint32_t hm = 0, tmin = 0xffffffffu;
 #pragma unroll
-              for (int k = 0; k < 8; ++k)
+              for (int k = 0; k < 8; ++k) {
                 hm |= (cyc[k] - c0 < L ? 1u : 0u) << k;
+                tmin = min(tmin, cyc[k] - c0);
+              }
               hm &= tvm[u];
               // the reference's binary_search(c0) starts at the last entry
-              // equal to c0 in a bin: earlier ones are skipped
-              bool at_c0 = false;
-#pragma unroll
-              for (int k = 0; k < 8; ++k)
-                at_c0 = at_c0 || (((hm >> k) & 1u) && cyc[k] == c0);
-              if (ballot64(at_c0) != 0) {
+              // equal to c0 in a bin: earlier ones are skipped.  tmin == 0
+              // says that one of the eight entries (listed or not) is at c0;
+              // the exact test per entry follows.
+              if (ballot64(tmin == 0u && hm != 0u) != 0) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                   if (((hm >> k) & 1u) && cyc[k] == c0) {
@@ -1143,7 +1157,19 @@ void select_kernel(SelArgs a) {
                 uint32_t pre, tot;
                 list_slots(hm, (lane >> 5) == hh, pre, tot);
                 if (tot == 0) continue;
-                if (deferred && nh + tot > (uint32_t)kHitCap) return false;
+                if (deferred && nh + tot > (uint32_t)kHitCap) {
+                  if (nobs != 1) return fa;
+                  // lane 32 hh scans this half-tile's first fragment
+                  const uint32_t fcut = rdlane(tfr[u], 32 * hh);
+                  uint32_t keep = 0;
+                  for (uint32_t c = 0; c < nh; c += kWave) {
+                    const uint32_t idx = c + lane;
+                    const bool kp = idx < nh && ((l.hits[idx].y >> 12) & 63u) < fcut;
+                    keep += (uint32_t)__builtin_popcountll(ballot64(kp));
+                  }
+                  nh = keep;
+                  return fcut;
+                }
                 if ((lane >> 5) == hh) {
                   uint32_t pos = nh + pre;
 #pragma unroll
@@ -1170,7 +1196,7 @@ void select_kernel(SelArgs a) {
           }
           __builtin_amdgcn_wave_barrier();
         }
-        return true;
+        return fb;
       };
       auto zero_group = [&](uint32_t G) {
         float4 *xz = reinterpret_cast<float4 *>(l.X);
@@ -1180,9 +1206,19 @@ void select_kernel(SelArgs a) {
         for (uint32_t i = lane; i < G * l.wb; i += kWave) l.BD[i] = 0ull;
         __builtin_amdgcn_wave_barrier();
       };
-      // the whole row's hits in one pass, their intensities in one batch
-      bool listed = nobs > 0 && nsel > 0 && extract(0, nsel, true);
-      if (listed) {
+      // The hits of fragments [f0, fstop) in one pass, their intensities in
+      // one batch; when not even fragment f0 fits the list, fragments
+      // [f0, fstop) of one group are extracted directly instead.
+      bool listed = false;
+      uint32_t fstop = 0;
+      auto list_from = [&](uint32_t f0) __attribute__((always_inline)) {
+        nh = 0;
+        fstop = extract(f0, nsel, true);
+        listed = fstop > f0;
+        if (!listed) {  // one group directly; with more windows, all
+          fstop = nobs == 1 ? f0 + min(l.gcap, nsel - f0) : nsel;
+          return;
+        }
         uint64_t wb[kObsMax];
 #pragma unroll
         for (int j = 0; j < kObsMax; ++j) wb[j] = win_base((uint32_t)j);
@@ -1206,11 +1242,17 @@ void select_kernel(SelArgs a) {
           }
         }
         __builtin_amdgcn_wave_barrier();
-      }
-
-      if (more) stage2();
-      for (uint32_t g0 = 0; g0 < nsel && nobs > 0; g0 += l.gcap) {
-        const uint32_t G = min(l.gcap, nsel - g0);
+      };
+      // one call site of list_from (and of the deferred extraction), so
+      // that both stay inline
+      bool staged = false;
+      for (uint32_t g0 = 0, G = 0; g0 < nsel && nobs > 0; g0 += G) {
+        if (g0 == fstop) {
+          list_from(g0);
+          if (more && !staged) stage2();
+          staged = true;
+        }
+        G = min(l.gcap, fstop - g0);
         zero_group(G);
         gA = g0;
         gG = G;
@@ -1296,6 +1338,7 @@ void select_kernel(SelArgs a) {
         __builtin_amdgcn_wave_barrier();
         if (nitems) flush(nitems);
       }
+      if (more && !staged) stage2();
       __builtin_amdgcn_wave_barrier();
       if (a.dbg_scores && row >= a.dbg_row0 && row < a.dbg_row0 + a.dbg_rows &&
           L <= a.dbg_stride) {

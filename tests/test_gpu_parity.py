@@ -269,3 +269,84 @@ def test_small_variants_match_oracle(ads, variant):
         got = _search(ads, gen, top_k_fragments=9)[3]
         want = _oracle(gen, top_k_fragments=9)
     assert_tables_equal(got, want, variant)
+
+
+def _with_peak_everywhere(gen, mz_add, intensity):
+    """Adds one peak at mz_add to every spectrum (kept sorted by m/z)."""
+    dia = list(gen["dia"])
+    start, stop, mz, it = dia[3], dia[4], dia[7], dia[8]
+    new_mz, new_it, new_start, new_stop = [], [], [], []
+    pos = 0
+    for s0, s1 in zip(start, stop):
+        m = np.concatenate([mz[s0:s1], np.array([mz_add], np.float32)])
+        v = np.concatenate([it[s0:s1], np.array([intensity], np.float32)])
+        o = np.argsort(m, kind="stable")
+        new_mz.append(m[o])
+        new_it.append(v[o])
+        new_start.append(pos)
+        pos += len(m)
+        new_stop.append(pos)
+    dia[3] = np.array(new_start, np.int64)
+    dia[4] = np.array(new_stop, np.int64)
+    dia[7] = np.concatenate(new_mz).astype(np.float32)
+    dia[8] = np.concatenate(new_it).astype(np.float32)
+    return dict(gen, dia=tuple(dia))
+
+
+def test_hit_list_overflow_scores_bitwise(ads):
+    """Rows with more in-window entries than the device keeps in its per-row
+    hit list (256): dense spectra, 40 ppm, 20-24 fragments and long RT
+    windows; a fragment m/z present in every spectrum, so that one fragment
+    alone overflows the list; precursors on a window boundary (two windows).
+    Every score row is compared bit for bit, then the candidate table."""
+    import ctypes
+    import alphadia_search_rs._native as N
+    spec = synthetic.config(2, n_cycles=700, n_windows=4, peaks_per_spectrum=3000,
+                            n_targets=300, frag_min=20, frag_max=24,
+                            half_window_min=150, half_window_max=300)
+    gen = _with_peak_everywhere(synthetic.generate(spec), 777.7777, 350.0)
+    lib = [a.copy() for a in gen["lib"]]
+    fs = lib[6].astype(np.int64)
+    lib[9][fs[::4]] = np.float32(777.7777)
+    lib[9][fs[2::8] + 5] = np.float32(777.7777)
+    lib[1][1::9] = np.float32(550.0)   # window boundary: 400 + 150
+    lib[2][1::9] = np.float32(550.0)
+    gen = dict(gen, lib=tuple(lib))
+    extra = dict(mass_tolerance=40.0, top_k_fragments=24)
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    slib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    p = params_for(gen["spec"], **extra)
+    p.update({"rt_tolerance_per_precursor": gen["rt_tolerance_per_precursor"]})
+    n = slib.num_precursors
+    stride = 1024
+    scores = np.zeros((n, stride), np.float32)
+    cs = np.zeros(n, np.uint64)
+    ce = np.zeros(n, np.uint64)
+    c = p._c()
+    N.check(N.lib.adsr_select_scores(d._h, slib._h, ctypes.byref(c), 0, n, stride,
+                                     N.ptr(scores), N.ptr(cs), N.ptr(ce)))
+    od, ol = oracle.Dia(gen["dia"]), oracle.Lib(gen["lib"])
+    prm = oracle_params_for(gen["spec"], rt_tolerance_per_precursor=gen["rt_tolerance_per_precursor"],
+                            **extra)
+    big = single = two = 0
+    for r in range(n):
+        one = oracle.select_one(od, ol, prm, r)
+        if one is None:
+            assert ce[r] <= cs[r]
+            continue
+        assert cs[r] == one["cycle_start"] and ce[r] == one["cycle_stop"]
+        L = one["cycle_stop"] - one["cycle_start"]
+        assert L <= stride
+        nz = one["xic"] != 0
+        big += int(nz.sum() > 256)
+        single += int(nz.sum(axis=1).max(initial=0) > 256)
+        two += int(np.float32(lib[2][r]) == np.float32(550.0) and nz.sum() > 256)
+        got = scores[r, :L]
+        bad = np.flatnonzero(got.view(np.uint32) != one["score"].view(np.uint32))
+        assert bad.size == 0, f"row {r}: score differs at cycles {bad[:8]}"
+    # the cases are present: list overflow, one fragment alone over the
+    # list, and overflow with two windows
+    assert big > 50 and single > 20 and two > 3, (big, single, two)
+    got = ads.PeakGroupSelection(p).search(d, slib).to_arrays()
+    want = oracle.select(od, ol, prm)
+    assert_tables_equal(got, want, "hit-list overflow")
