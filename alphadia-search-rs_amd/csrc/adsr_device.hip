@@ -1038,8 +1038,7 @@ void select_kernel(SelArgs a) {
       // (the list is in fragment order); otherwise nothing is kept and fa is
       // returned.  Direct: the list is accumulated into X whenever it fills
       // and at the end of each window.
-      auto extract = [&](uint32_t fa, uint32_t fb,
-                         bool deferred) __attribute__((always_inline)) -> uint32_t {
+      auto extract = [&](uint32_t fa, uint32_t fb, bool deferred) -> uint32_t {
         const uint32_t FN = fb - fa;
         for (uint32_t j = 0; j < nobs; ++j) {
           const uint32_t o = win_obs(j);
@@ -1206,19 +1205,13 @@ void select_kernel(SelArgs a) {
         for (uint32_t i = lane; i < G * l.wb; i += kWave) l.BD[i] = 0ull;
         __builtin_amdgcn_wave_barrier();
       };
-      // The hits of fragments [f0, fstop) in one pass, their intensities in
-      // one batch; when not even fragment f0 fits the list, fragments
-      // [f0, fstop) of one group are extracted directly instead.
-      bool listed = false;
+      // The hits of fragments [0, fstop) in one pass, their intensities in
+      // one batch; fragments [fstop, nsel), which did not fit the list, are
+      // extracted directly, group by group.
       uint32_t fstop = 0;
-      auto list_from = [&](uint32_t f0) __attribute__((always_inline)) {
+      if (nobs > 0 && nsel > 0) {
         nh = 0;
-        fstop = extract(f0, nsel, true);
-        listed = fstop > f0;
-        if (!listed) {  // one group directly; with more windows, all
-          fstop = nobs == 1 ? f0 + min(l.gcap, nsel - f0) : nsel;
-          return;
-        }
+        fstop = extract(0, nsel, true);
         uint64_t wb[kObsMax];
 #pragma unroll
         for (int j = 0; j < kObsMax; ++j) wb[j] = win_base((uint32_t)j);
@@ -1242,17 +1235,12 @@ void select_kernel(SelArgs a) {
           }
         }
         __builtin_amdgcn_wave_barrier();
-      };
-      // one call site of list_from (and of the deferred extraction), so
-      // that both stay inline
-      bool staged = false;
+      }
+
+      if (more) stage2();
       for (uint32_t g0 = 0, G = 0; g0 < nsel && nobs > 0; g0 += G) {
-        if (g0 == fstop) {
-          list_from(g0);
-          if (more && !staged) stage2();
-          staged = true;
-        }
-        G = min(l.gcap, fstop - g0);
+        const bool listed = g0 < fstop;
+        G = min(l.gcap, (listed ? fstop : nsel) - g0);
         zero_group(G);
         gA = g0;
         gG = G;
@@ -1338,7 +1326,6 @@ void select_kernel(SelArgs a) {
         __builtin_amdgcn_wave_barrier();
         if (nitems) flush(nitems);
       }
-      if (more && !staged) stage2();
       __builtin_amdgcn_wave_barrier();
       if (a.dbg_scores && row >= a.dbg_row0 && row < a.dbg_row0 + a.dbg_rows &&
           L <= a.dbg_stride) {
