@@ -590,7 +590,9 @@ constexpr int kXPad = 16;
 constexpr int kItemCap = 512;   // u16 (fragment, block) items
 constexpr int kHitCap = 256;    // (u32 entry or intensity, u32 key) hits
 constexpr int kTileHits = 4 * 64;  // hits one extraction tile can add
-constexpr int kGroupTiles = 2;     // extraction tiles in flight (8 entries per lane)
+// extraction tiles in flight (8 entries per lane each); one measured
+// fastest: a second tile's registers cost more than its latency hiding
+constexpr int kGroupTiles = 1;
 
 struct LdsShape {
   uint32_t lp;     // XIC row stride (floats)
@@ -964,7 +966,7 @@ void select_kernel(SelArgs a) {
 
       // ---- extraction: src/dense_xic_observation/mod.rs:42-55 ------------
       // Windows in order; within a window the fragments' CSR ranges are cut
-      // into 4-entry groups (one 8-byte load of u16 cycles per lane), 64
+      // into 8-entry groups (one 16-byte load of u16 cycles per lane), 64
       // groups per tile across fragment boundaries, kGroupTiles tiles in
       // flight.  In-window hits go to an LDS list in entry order (window,
       // fragment, bin, entry), which is the reference's summation order.
