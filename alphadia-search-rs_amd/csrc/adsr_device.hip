@@ -4089,6 +4089,9 @@ struct ScoreArgs {
 
 constexpr uint32_t kStScoreWindowTooLong = 1u << 8;
 constexpr uint32_t kStScoreRtIndex = 1u << 9;  // the reference panics there
+constexpr uint32_t kStScoreTooManyWindows = 1u << 10;
+constexpr uint32_t kScoreObsMax = 16;  // isolation windows holding one precursor
+constexpr uint32_t kScorePool = 128;   // in-window entries staged per pass of 64 bins
 
 // Rust 1.88 slice::binary_search over u16: the last equal element or the
 // insertion point (src/quadrupole_observation/mod.rs:175-178).
@@ -4197,14 +4200,19 @@ __global__ __launch_bounds__(256) void idf_count_kernel(KDia dia, const float *m
 }
 
 struct ScoreLds {
+  uint4 *rec;    // [64] bin items of a pass: first entry (u64), pool slot | count << 16, grid m/z
+  uint2 *pool;   // [kScorePool] (cycle - c0, intensity) of the pass's in-window entries
   float *xic, *mz, *norm, *sortA, *sortF;  // [ks][L] rows / [L][ks] columns
   float *oint, *corr, *fint, *idf, *err, *wie, *buf;  // [ks]
   uint32_t *tn, *matched, *nz;                         // [ks]
   float *med, *medf;                                   // [kScoreLMax]
+  uint32_t *ipre, *ibs, *inb;                          // [ks] bin items per fragment
+  uint32_t *vo;                                        // [kScoreObsMax] valid windows
 };
 
 __host__ __device__ inline uint32_t score_lds_bytes(uint32_t ks, uint32_t L) {
-  return (5 * ks * L + 10 * ks + 2 * kScoreLMax) * 4;
+  return kWave * 16 + kScorePool * 8 +
+         (5 * ks * L + 10 * ks + 2 * kScoreLMax + 3 * ks + kScoreObsMax) * 4;
 }
 
 // QUANT: PeakGroupQuantification.quantify (src/peak_group_quantification/
@@ -4262,7 +4270,9 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   char *base = smem + (size_t)wv * a.lds_per_wave;
   ScoreLds s;
   {
-    float *p = reinterpret_cast<float *>(base);
+    s.rec = reinterpret_cast<uint4 *>(base);
+    s.pool = reinterpret_cast<uint2 *>(base + kWave * 16);
+    float *p = reinterpret_cast<float *>(base + kWave * 16 + kScorePool * 8);
     s.xic = p; p += ks * L;
     s.mz = p; p += ks * L;
     s.norm = p; p += ks * L;
@@ -4279,7 +4289,11 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     s.matched = reinterpret_cast<uint32_t *>(p); p += ks;
     s.nz = reinterpret_cast<uint32_t *>(p); p += ks;
     s.med = p; p += kScoreLMax;
-    s.medf = p;
+    s.medf = p; p += kScoreLMax;
+    s.ipre = reinterpret_cast<uint32_t *>(p); p += ks;
+    s.ibs = reinterpret_cast<uint32_t *>(p); p += ks;
+    s.inb = reinterpret_cast<uint32_t *>(p); p += ks;
+    s.vo = reinterpret_cast<uint32_t *>(p);
   }
   const bool mine = (uint32_t)lane < F;
   const uint32_t f = (uint32_t)lane;
@@ -4301,38 +4315,125 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     }
   }
   // dense_xic_observation/mod.rs:98-137 with quadrupole_observation/mod.rs:
-  // 152-211, valid windows in ascending order
+  // 152-211, valid windows in ascending order.  The (fragment, window, bin)
+  // items are searched one per lane, 64 at a time (binary search for c0,
+  // then the entries below c1, whose cycles and intensities are staged in
+  // LDS); each fragment lane then adds its items' entries in the
+  // reference's order: window, bin, entry.
   const float pmz = a.lib.p_mz[row];
+  uint32_t nvo = 0;
   for (uint32_t o = 0; o < a.dia.n_obs; ++o) {
     if (!(a.dia.iso_lo[o] <= pmz && a.dia.iso_hi[o] >= pmz)) continue;
-    if (!mine) continue;
+    if (nvo < kScoreObsMax) s.vo[nvo] = o;  // every lane stores the same value
+    ++nvo;
+  }
+  if (nvo > kScoreObsMax) {
+    if (lane == 0) {
+      a.found[ci] = 0;
+      atomicOr(a.status, kStScoreTooManyWindows);
+    }
+    return;
+  }
+  uint32_t nb_f = 0;
+  if (mine) {
     const float delta = fmz * a.mass_tol * 1e-6f;
     uint32_t bs, be;
     mz_range(a.dia, fmz - delta, fmz + delta, bs, be);
-    const uint32_t *ssw = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
-    const uint64_t eb = a.dia.entry_base[o];
-    const uint16_t key16 = (uint16_t)c0;
-    float *X = s.xic + f * L, *M = s.mz + f * L;
-    for (uint32_t b = bs; b < be; ++b) {
-      const float actual = a.dia.grid[b];
-      const uint32_t e0 = ssw[b], e1 = ssw[b + 1];
-      const uint16_t *cy = a.dia.cycles + eb + e0;
-      const float *in = a.dia.intens + eb + e0;
-      for (uint32_t t = rust_bsearch_u16(cy, e1 - e0, key16); t < e1 - e0; ++t) {
+    nb_f = be - bs;
+    s.ibs[f] = bs;
+    s.inb[f] = nb_f;
+  }
+  const uint32_t items_f = nb_f * nvo;
+  const uint32_t incl = wave_incl_scan(items_f);
+  const uint32_t ipre_f = incl - items_f;
+  const uint32_t NI = rdlane(incl, kWave - 1);
+  if (mine) s.ipre[f] = ipre_f;
+  __builtin_amdgcn_wave_barrier();
+  const uint16_t key16 = (uint16_t)c0;
+  for (uint32_t p0 = 0; p0 < NI; p0 += kWave) {
+    const uint32_t k = p0 + (uint32_t)lane;
+    uint64_t e = 0;
+    uint32_t cnt = 0;
+    float actual = 0.0f;
+    if (k < NI) {
+      // the item's fragment: the last one whose first item is <= k
+      uint32_t g = 0, gh = F;
+      while (gh - g > 1) {
+        const uint32_t mid = (g + gh) >> 1;
+        if (s.ipre[mid] <= k) g = mid;
+        else gh = mid;
+      }
+      const uint32_t r = k - s.ipre[g], nb = s.inb[g];
+      const uint32_t j = r / nb, b = s.ibs[g] + (r - j * nb);
+      const uint32_t o = s.vo[j];
+      const uint32_t *ssw = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
+      const uint32_t e0 = ssw[b], n = ssw[b + 1] - e0;
+      e = a.dia.entry_base[o] + e0;
+      actual = a.dia.grid[b];
+      const uint16_t *cy = a.dia.cycles + e;
+      uint32_t t = rust_bsearch_u16(cy, n, key16);
+      e += t;
+      for (; t < n; ++t) {
         const uint64_t cyc = cy[t];
         if (cyc >= c1) break;
-        const uint64_t rel64 = cyc - c0;
-        if (rel64 >= L) break;  // only for a cycle_start beyond 65535
-        const uint32_t rel = (uint32_t)rel64;
-        const float v = in[t];
-        const float x = X[rel] + v;
-        X[rel] = x;
-        if (v > 0.0f) {
-          const float prev = x - v;
-          M[rel] = prev == 0.0f ? actual : (M[rel] * prev + actual * v) / x;
+        if (cyc - c0 >= (uint64_t)L) break;  // only for a cycle_start beyond 65535
+        ++cnt;
+      }
+    }
+    // pool slots in item order; entries past the pool are read again below
+    const uint32_t cincl = wave_incl_scan(cnt);
+    const uint32_t off = cincl - cnt;
+    if (k < NI) {
+      s.rec[lane] = make_uint4((uint32_t)e, (uint32_t)(e >> 32), off | (cnt << 16),
+                               __float_as_uint(actual));
+      const uint32_t m = off < kScorePool ? min(cnt, kScorePool - off) : 0u;
+      for (uint32_t i0 = 0; i0 < m; i0 += 4) {
+        uint32_t cyc4[4];
+        float v4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          cyc4[u] = i0 + u < m ? a.dia.cycles[e + i0 + u] : 0u;
+          v4[u] = i0 + u < m ? a.dia.intens[e + i0 + u] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (i0 + u < m)
+            s.pool[off + i0 + u] =
+                make_uint2((uint32_t)((uint64_t)cyc4[u] - c0), __float_as_uint(v4[u]));
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (mine && items_f) {
+      const uint32_t k0 = max(ipre_f, p0), k1 = min(ipre_f + items_f, p0 + kWave);
+      float *X = s.xic + f * L, *M = s.mz + f * L;
+      for (uint32_t kk = k0; kk < k1; ++kk) {
+        const uint4 rc = s.rec[kk - p0];
+        const uint32_t c = rc.z >> 16;
+        if (c == 0) continue;
+        const uint32_t o0 = rc.z & 0xffffu;
+        const uint64_t eg = ((uint64_t)rc.y << 32) | rc.x;
+        const float act = __uint_as_float(rc.w);
+        for (uint32_t i = 0; i < c; ++i) {
+          uint32_t rel;
+          float v;
+          if (o0 + i < kScorePool) {
+            const uint2 h = s.pool[o0 + i];
+            rel = h.x;
+            v = __uint_as_float(h.y);
+          } else {
+            rel = (uint32_t)((uint64_t)a.dia.cycles[eg + i] - c0);
+            v = a.dia.intens[eg + i];
+          }
+          const float x = X[rel] + v;
+          X[rel] = x;
+          if (v > 0.0f) {
+            const float prev = x - v;
+            M[rel] = prev == 0.0f ? act : (M[rel] * prev + act * v) / x;
+          }
         }
       }
     }
+    __builtin_amdgcn_wave_barrier();
   }
   // utils.rs:71-108 normalize_profiles(.., 1) and :7-29 filter_non_zero
   const uint32_t cidx = L / 2;
@@ -4854,6 +4955,10 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
     if (hstatus & kStScoreWindowTooLong)
       return fail(set_error(ADSR_ERR_UNSUPPORTED, "a candidate cycle range exceeds %d",
                             kScoreLMax));
+    if (hstatus & kStScoreTooManyWindows)
+      return fail(set_error(ADSR_ERR_UNSUPPORTED,
+                            "a precursor m/z lies in more than %u isolation windows",
+                            kScoreObsMax));
     // precursor_quantified/mod.rs:26-94 (fragments with intensity > 0, none
     // -> no row) and speclib_flat_quantified/mod.rs:213-330 (flat fragments
     // in candidate order, precursor columns stably sorted by precursor_idx)
@@ -4954,6 +5059,10 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   if (hstatus & kStScoreWindowTooLong)
     return fail(set_error(ADSR_ERR_UNSUPPORTED, "a candidate cycle range exceeds %d",
                           kScoreLMax));
+  if (hstatus & kStScoreTooManyWindows)
+    return fail(set_error(ADSR_ERR_UNSUPPORTED,
+                          "a precursor m/z lies in more than %u isolation windows",
+                          kScoreObsMax));
   auto *f = new adsr_features();
   size_t m = 0;
   for (size_t i = 0; i < n; ++i) m += hfound[i] ? 1 : 0;
