@@ -4067,7 +4067,6 @@ struct ScoreArgs {
   KDia dia;
   KLib lib;
   KPlan plan;
-  const uint64_t *p_idx;  // sorted precursor_idx
   const uint8_t *p_naa;
   const float *f_mzl;
   const uint32_t *idf_counts;
@@ -4075,6 +4074,7 @@ struct ScoreArgs {
   float mass_tol;
   uint32_t n;
   const uint64_t *c_pidx, *c_start, *c_stop, *c_center;
+  const uint32_t *c_row;  // library row of each candidate, ~0u: not in the library
   const float *c_score;
   float *feat;  // [ADSR_N_FEATURES][n]
   uint8_t *found;
@@ -4199,6 +4199,23 @@ __global__ __launch_bounds__(256) void idf_count_kernel(KDia dia, const float *m
   if (i < n) atomicAdd(&counts[closest_bin(dia, mzl[i])], 1u);
 }
 
+// speclib_flat/mod.rs:458-475 get_precursor_by_idx_filtered: binary search
+// of the sorted precursor_idx, one thread per candidate
+__global__ __launch_bounds__(256) void cand_row_kernel(const uint64_t *p_idx, uint32_t n_prec,
+                                                       const uint64_t *c_pidx, uint64_t n,
+                                                       uint32_t *row) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t key = c_pidx[i];
+  uint32_t lo = 0, hi = n_prec;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (p_idx[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  row[i] = lo < n_prec && p_idx[lo] == key ? lo : 0xffffffffu;
+}
+
 struct ScoreLds {
   uint4 *rec;    // [64] bin items of a pass: first entry (u64), pool slot | count << 16, grid m/z
   uint2 *pool;   // [kScorePool] (cycle - c0, intensity) of the pass's in-window entries
@@ -4233,21 +4250,12 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   auto F_OUT = [&](int k, float v) {
     if (lane == 0) a.feat[(size_t)k * n_o + ci] = v;
   };
-  // speclib_flat/mod.rs:458-475 get_precursor_by_idx_filtered: binary
-  // search of the sorted precursor_idx
-  const uint64_t key = a.c_pidx[ci];
-  uint32_t lo = 0, hi = a.lib.n_prec;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (a.p_idx[mid] < key) lo = mid + 1;
-    else hi = mid;
-  }
-  lo = uni(lo);
-  if (lo >= a.lib.n_prec || a.p_idx[lo] != key) {  // filter_map -> None
+  // the library row, looked up by cand_row_kernel
+  const uint32_t row = uni(a.c_row[ci]);
+  if (row == 0xffffffffu) {  // filter_map -> None
     if (lane == 0) a.found[ci] = 0;
     return;
   }
-  const uint32_t row = lo;
   const uint64_t c0 = a.c_start[ci], c1 = a.c_stop[ci];
   const uint64_t L64 = c1 - c0;  // usize arithmetic wraps (release build)
   if constexpr (QUANT) {  // quantification/mod.rs:93-98 -> None
@@ -4298,7 +4306,11 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   const bool mine = (uint32_t)lane < F;
   const uint32_t f = (uint32_t)lane;
   float fmz = 0.0f, fint = 0.0f, fmzl = 0.0f;
-  uint32_t ftype = 0, fnum = 0;
+  uint32_t ftype = 0, fnum = 0, df = 0;
+  // mod.rs:249-268 and quantification: RT at cycle_center, loaded early
+  const uint64_t ccen = a.c_center[ci];
+  const float rt_cen = ccen < a.dia.n_rt ? a.dia.rt[ccen] : 0.0f;
+  const float prt = a.lib.p_rt[row];
   if (mine) {
     const size_t so = (size_t)row * ks + f;
     const uint64_t j = a.lib.f_start[row] + a.plan.sel[so];
@@ -4307,6 +4319,9 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     ftype = a.lib.f_type[j];
     fnum = a.lib.f_num[j];
     fmzl = a.f_mzl[j];
+    // src/idf/mod.rs:35-49, loaded here so that its latency overlaps the
+    // extraction
+    df = a.idf_counts ? a.idf_counts[closest_bin(a.dia, fmzl)] : 0u;
     s.fint[f] = fint;
     s.tn[f] = ftype | (fnum << 8);
     for (uint32_t c = 0; c < L; ++c) {
@@ -4322,10 +4337,14 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   // reference's order: window, bin, entry.
   const float pmz = a.lib.p_mz[row];
   uint32_t nvo = 0;
-  for (uint32_t o = 0; o < a.dia.n_obs; ++o) {
-    if (!(a.dia.iso_lo[o] <= pmz && a.dia.iso_hi[o] >= pmz)) continue;
-    if (nvo < kScoreObsMax) s.vo[nvo] = o;  // every lane stores the same value
-    ++nvo;
+  for (uint32_t o0 = 0; o0 < a.dia.n_obs; o0 += kWave) {
+    const uint32_t o = o0 + (uint32_t)lane;
+    const bool in = o < a.dia.n_obs && a.dia.iso_lo[o] <= pmz && a.dia.iso_hi[o] >= pmz;
+    const uint64_t m = ballot64(in);
+    const uint32_t pos = nvo + __builtin_amdgcn_mbcnt_hi(
+                                   (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (in && pos < kScoreObsMax) s.vo[pos] = o;  // windows in ascending order
+    nvo += (uint32_t)__builtin_popcountll(m);
   }
   if (nvo > kScoreObsMax) {
     if (lane == 0) {
@@ -4610,7 +4629,6 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     }
     s.wie[f] = 1.0f / (fabsf(e) + 0.1f);
     // src/idf/mod.rs:35-49
-    const uint32_t df = a.idf_counts ? a.idf_counts[closest_bin(a.dia, fmzl)] : 0u;
     s.idf[f] = a.idf_total == 0.0f ? 1.0f
                : logf_exact(a.idf_total / (float)(df > 1u ? df : 1u), ltab);
   }
@@ -4619,9 +4637,8 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     if (lane == 0) {
       a.q_nf[ci] = F;
       a.q_row[ci] = row;
-      const uint64_t cc = a.c_center[ci];
-      if (cc >= a.dia.n_rt) atomicOr(a.status, kStScoreRtIndex);
-      a.q_rt[ci] = cc < a.dia.n_rt ? a.dia.rt[cc] : 0.0f;
+      if (ccen >= a.dia.n_rt) atomicOr(a.status, kStScoreRtIndex);
+      a.q_rt[ci] = rt_cen;
       a.found[ci] = 1;
     }
     return;
@@ -4640,10 +4657,9 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   }
   const float hs_ime = hyperscore_dev(s.tn, s.oint, s.matched, s.wie, F, ltab);
   // mod.rs:249-268
-  const uint64_t cc = a.c_center[ci];
-  if (cc >= a.dia.n_rt && lane == 0) atomicOr(a.status, kStScoreRtIndex);
-  const float rt_obs = cc < a.dia.n_rt ? a.dia.rt[cc] : 0.0f;
-  const float delta_rt = rt_obs - a.lib.p_rt[row];
+  if (ccen >= a.dia.n_rt && lane == 0) atomicOr(a.status, kStScoreRtIndex);
+  const float rt_obs = rt_cen;
+  const float delta_rt = rt_obs - prt;
   float ib = 0.0f, iy = 0.0f;
   for (uint32_t g = 0; g < F; ++g) {
     if (!s.matched[g] || !(s.oint[g] > 0.0f)) continue;
@@ -4854,7 +4870,7 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   uint64_t *cpidx, *crank, *cs, *ce, *cc;
   float *csc, *feat;
   uint8_t *found;
-  uint32_t *status;
+  uint32_t *status, *crow;
   const size_t ks = sp.plan.ks;
   const size_t nq = quant ? n * ks : 0;
   float *q_int = nullptr, *q_mzo = nullptr, *q_corr = nullptr, *q_err = nullptr,
@@ -4863,7 +4879,7 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   if (tmp.alloc(&cpidx, n) || tmp.alloc(&crank, n) || tmp.alloc(&cs, n) ||
       tmp.alloc(&ce, n) || tmp.alloc(&cc, n) || tmp.alloc(&csc, n) ||
       tmp.alloc(&feat, quant ? 1 : (size_t)ADSR_N_FEATURES * n) ||
-      tmp.alloc(&found, n) || tmp.alloc(&status, 1))
+      tmp.alloc(&found, n) || tmp.alloc(&status, 1) || tmp.alloc(&crow, n))
     return fail(set_error(ADSR_ERR_OUT_OF_MEMORY, "scoring buffers"));
   if (quant && (tmp.alloc(&q_int, nq) || tmp.alloc(&q_mzo, nq) || tmp.alloc(&q_corr, nq) ||
                 tmp.alloc(&q_err, nq) || tmp.alloc(&q_raw, nq) || tmp.alloc(&q_nf, n) ||
@@ -4890,7 +4906,6 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   a.dia = sp.dia;
   a.lib = sp.lib;
   a.plan = sp.plan;
-  a.p_idx = l->dev.p_idx;
   a.p_naa = l->dev.p_naa;
   a.f_mzl = l->dev.f_mzl;
   a.idf_counts = l->dev.idf_counts;
@@ -4901,6 +4916,7 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   a.c_start = cs;
   a.c_stop = ce;
   a.c_center = cc;
+  a.c_row = crow;
   a.c_score = csc;
   a.feat = feat;
   a.found = found;
@@ -4920,6 +4936,8 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
     return fail(set_error(ADSR_ERR_UNSUPPORTED, "scoring LDS of %u bytes", a.lds_per_wave));
   (void)hipEventRecord(w.ev[2], w.stream);
   if (n) {
+    cand_row_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, w.stream>>>(
+        l->dev.p_idx, (uint32_t)l->n_prec, cpidx, n, crow);
     const dim3 grid((uint32_t)((n + wpb - 1) / wpb)), block(wpb * kWave);
     const size_t lds = (size_t)a.lds_per_wave * wpb;
     if (quant)
