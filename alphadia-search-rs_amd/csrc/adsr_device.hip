@@ -4156,32 +4156,59 @@ __device__ float gamma_ln_dev(float x, const LogfEntry *ltab) {
   return (x - 0.5f) * logf_exact(x, ltab) - x + 0.5f * ln_2pi;
 }
 
-// src/peak_group_scoring/utils.rs:193-279 over LDS columns (w == nullptr:
-// weight 1); every lane computes the same value
-__device__ float hyperscore_dev(const uint32_t *tn, const float *inten,
-                                const uint32_t *matched, const float *w,
-                                uint32_t n, const LogfEntry *ltab) {
-  uint32_t nb = 0, ny = 0;
-  float sb = 0.0f, sy = 0.0f;
+// src/peak_group_scoring/utils.rs:193-279: the four hyperscores of a
+// candidate (observed intensity, library intensity, observed intensity x
+// inverse mass error weight, IDF) in one pass over the fragments, each with
+// its own counts and sums in fragment order; then the 16 logarithms
+// (ln_gamma of the counts, ln of the sums), one per lane, added in the
+// reference's order.  Every lane returns the same four values.
+__device__ void hyperscores_dev(const uint32_t *tn, const uint32_t *matched,
+                                const float *oint, const float *fint,
+                                const float *wie, const float *idf, uint32_t n,
+                                const LogfEntry *ltab, float (&hs)[4]) {
+  uint32_t nb[4] = {0u, 0u, 0u, 0u}, ny[4] = {0u, 0u, 0u, 0u};
+  float sb[4] = {0.f, 0.f, 0.f, 0.f}, sy[4] = {0.f, 0.f, 0.f, 0.f};
   for (uint32_t i = 0; i < n; ++i) {
-    if (!matched[i] || inten[i] == 0.0f) continue;
-    const float v = inten[i] * (w ? w[i] : 1.0f);
+    if (!matched[i]) continue;
     const uint32_t t = tn[i] & 0xffu;
-    if (t == (uint32_t)kFragB) {
-      ++nb;
-      sb = sb + v;
-    } else if (t == (uint32_t)kFragY) {
-      ++ny;
-      sy = sy + v;
+    const bool isb = t == (uint32_t)kFragB;
+    if (!isb && t != (uint32_t)kFragY) continue;
+    const float o = oint[i], fl = fint[i], d = idf[i];
+    const float v[4] = {o, fl, o * wie[i], d};
+    const bool on[4] = {o != 0.0f, fl != 0.0f, o != 0.0f, d != 0.0f};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!on[k]) continue;
+      if (isb) {
+        ++nb[k];
+        sb[k] = sb[k] + v[k];
+      } else {
+        ++ny[k];
+        sy[k] = sy[k] + v[k];
+      }
     }
   }
-  if (nb == 0 && ny == 0) return 0.0f;
-  const float fb = nb > 0 ? gamma_ln_dev((float)nb + 1.0f, ltab) : 0.0f;
-  const float fy = ny > 0 ? gamma_ln_dev((float)ny + 1.0f, ltab) : 0.0f;
-  const float lb = sb > 0.0f ? logf_exact(sb, ltab) : 0.0f;
-  const float ly = sy > 0.0f ? logf_exact(sy, ltab) : 0.0f;
-  const float h = fb + fy + lb + ly;
-  return isfinite(h) ? h : 0.0f;
+  const int lane = lane_id();
+  const int k = (lane >> 2) & 3, m = lane & 3;  // term m of hyperscore k
+  float term = 0.0f;
+  if (lane < 16) {
+    const uint32_t c = m == 0 ? (k == 0 ? nb[0] : k == 1 ? nb[1] : k == 2 ? nb[2] : nb[3])
+                               : (k == 0 ? ny[0] : k == 1 ? ny[1] : k == 2 ? ny[2] : ny[3]);
+    const float sv = m == 2 ? (k == 0 ? sb[0] : k == 1 ? sb[1] : k == 2 ? sb[2] : sb[3])
+                            : (k == 0 ? sy[0] : k == 1 ? sy[1] : k == 2 ? sy[2] : sy[3]);
+    if (m < 2) term = c > 0u ? gamma_ln_dev((float)c + 1.0f, ltab) : 0.0f;
+    else term = sv > 0.0f ? logf_exact(sv, ltab) : 0.0f;
+  }
+  const int tb = __builtin_bit_cast(int, term);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float fb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(tb, 4 * q));
+    const float fy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(tb, 4 * q + 1));
+    const float lb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(tb, 4 * q + 2));
+    const float ly = __builtin_bit_cast(float, __builtin_amdgcn_readlane(tb, 4 * q + 3));
+    const float h = fb + fy + lb + ly;
+    hs[q] = (nb[q] == 0u && ny[q] == 0u) ? 0.0f : (isfinite(h) ? h : 0.0f);
+  }
 }
 
 // Bin of an m/z (src/mz_index/mod.rs:55-90)
@@ -4470,22 +4497,26 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     s.nz[f] = any;
   }
   __builtin_amdgcn_wave_barrier();
-  const uint32_t Ff = uni((uint32_t)__builtin_popcountll(ballot64(mine && s.nz[f] != 0u)));
+  // bit g: fragment g has a non-zero profile (lane g holds fragment g)
+  const uint64_t nzm = ballot64(mine && s.nz[f] != 0u);
+  const uint32_t Ff = (uint32_t)__builtin_popcountll(nzm);
   // utils.rs:35-67 median_axis_0 of all rows and of the non-zero rows:
-  // each lane places its value of every column at its stable rank
-  if (mine) {
-    const bool nzf = s.nz[f] != 0u;
-    for (uint32_t c = 0; c < L; ++c) {
-      const float v = s.norm[f * L + c];
+  // every (column, fragment) pair, one per lane, places its value at its
+  // stable rank in the column
+  for (uint32_t q0 = 0; q0 < L * F; q0 += kWave) {
+    const uint32_t q = q0 + (uint32_t)lane;
+    if (q < L * F) {
+      const uint32_t c = q / F, fq = q - c * F;
+      const float v = s.norm[fq * L + c];
       uint32_t ra = 0, rf = 0;
       for (uint32_t g = 0; g < F; ++g) {
         const float u = s.norm[g * L + c];
-        const bool before = u < v || (u == v && g < f);
+        const bool before = u < v || (u == v && g < fq);
         ra += before ? 1u : 0u;
-        rf += (before && s.nz[g] != 0u) ? 1u : 0u;
+        rf += (before && ((nzm >> g) & 1ull)) ? 1u : 0u;
       }
       s.sortA[c * ks + ra] = v;
-      if (nzf) s.sortF[c * ks + rf] = v;
+      if ((nzm >> fq) & 1ull) s.sortF[c * ks + rf] = v;
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -4572,8 +4603,6 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     }
   // mod.rs:203-247
   const bool matched = mine && oint > 0.0f;
-  const float hs_obs = hyperscore_dev(s.tn, s.oint, s.matched, nullptr, F, ltab);
-  const float hs_lib = hyperscore_dev(s.tn, s.fint, s.matched, nullptr, F, ltab);
   // utils.rs:314-371 longest b / y series: matched numbers of a type at
   // their sorted rank, then one scan
   float longest[2];
@@ -4655,7 +4684,9 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     }
     wmae = (F > 0 && wt > 0.0f) ? ws / wt : 0.0f;
   }
-  const float hs_ime = hyperscore_dev(s.tn, s.oint, s.matched, s.wie, F, ltab);
+  float hs4[4];
+  hyperscores_dev(s.tn, s.matched, s.oint, s.fint, s.wie, s.idf, F, ltab, hs4);
+  const float hs_obs = hs4[0], hs_lib = hs4[1], hs_ime = hs4[2], idf_hs = hs4[3];
   // mod.rs:249-268
   if (ccen >= a.dia.n_rt && lane == 0) atomicOr(a.status, kStScoreRtIndex);
   const float rt_obs = rt_cen;
@@ -4668,7 +4699,6 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     if (t == (uint32_t)kFragY) iy = iy + s.oint[g];
   }
   // mod.rs:272-293
-  const float idf_hs = hyperscore_dev(s.tn, s.idf, s.matched, nullptr, F, ltab);
   float dot_xic = -0.0f, dot_int = -0.0f;
   for (uint32_t g = 0; g < F; ++g) {
     dot_xic = dot_xic + s.idf[g] * s.corr[g];
