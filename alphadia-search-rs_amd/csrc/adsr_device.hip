@@ -4091,7 +4091,9 @@ constexpr uint32_t kStScoreWindowTooLong = 1u << 8;
 constexpr uint32_t kStScoreRtIndex = 1u << 9;  // the reference panics there
 constexpr uint32_t kStScoreTooManyWindows = 1u << 10;
 constexpr uint32_t kScoreObsMax = 16;  // isolation windows holding one precursor
-constexpr uint32_t kScorePool = 128;   // in-window entries staged per pass of 64 bins
+// in-window entries staged per pass of 64 bins; 136 so that the staging
+// area also holds the column sorts of 24 fragments x 11 cycles
+constexpr uint32_t kScorePool = 136;
 
 // Rust 1.88 slice::binary_search over u16: the last equal element or the
 // insertion point (src/quadrupole_observation/mod.rs:175-178).
@@ -4254,9 +4256,17 @@ struct ScoreLds {
   uint32_t *vo;                                        // [kScoreObsMax] valid windows
 };
 
+// The column sorts (2 ks L floats) reuse the bin-item records and the pool
+// when they fit there: those are dead after the extraction.
+constexpr uint32_t kScoreStageBytes = kWave * 16 + kScorePool * 8;
+
+__host__ __device__ inline bool score_sorts_in_stage(uint32_t ks, uint32_t L) {
+  return 2 * ks * L * 4 <= kScoreStageBytes;
+}
+
 __host__ __device__ inline uint32_t score_lds_bytes(uint32_t ks, uint32_t L) {
-  return kWave * 16 + kScorePool * 8 +
-         (5 * ks * L + 10 * ks + 2 * kScoreLMax + 3 * ks + kScoreObsMax) * 4;
+  return kScoreStageBytes + (score_sorts_in_stage(ks, L) ? 0u : 2 * ks * L * 4) +
+         (3 * ks * L + 10 * ks + 2 * kScoreLMax + 3 * ks + kScoreObsMax) * 4;
 }
 
 // QUANT: PeakGroupQuantification.quantify (src/peak_group_quantification/
@@ -4274,8 +4284,10 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   if (ci >= a.n) return;  // no block barrier follows
   const uint32_t ks = a.plan.ks;
   const uint64_t n_o = a.n;
+  // feature k is gathered into lane k and the 41 are stored at once
+  float fv = 0.0f;
   auto F_OUT = [&](int k, float v) {
-    if (lane == 0) a.feat[(size_t)k * n_o + ci] = v;
+    if (lane == k) fv = v;
   };
   // the library row, looked up by cand_row_kernel
   const uint32_t row = uni(a.c_row[ci]);
@@ -4307,12 +4319,17 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   {
     s.rec = reinterpret_cast<uint4 *>(base);
     s.pool = reinterpret_cast<uint2 *>(base + kWave * 16);
-    float *p = reinterpret_cast<float *>(base + kWave * 16 + kScorePool * 8);
+    float *p = reinterpret_cast<float *>(base + kScoreStageBytes);
     s.xic = p; p += ks * L;
     s.mz = p; p += ks * L;
     s.norm = p; p += ks * L;
-    s.sortA = p; p += ks * L;
-    s.sortF = p; p += ks * L;
+    if (score_sorts_in_stage(ks, L)) {
+      s.sortA = reinterpret_cast<float *>(base);
+    } else {
+      s.sortA = p;
+      p += 2 * ks * L;
+    }
+    s.sortF = s.sortA + ks * L;
     s.oint = p; p += ks;
     s.corr = p; p += ks;
     s.fint = p; p += ks;
@@ -4750,18 +4767,23 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   F_OUT(ADSR_F_LONGEST_Y_SERIES, longest[1]);
   F_OUT(ADSR_F_NAA, (float)a.p_naa[row]);
   F_OUT(ADSR_F_WEIGHTED_MASS_ERROR, wmae);
-  F_OUT(ADSR_F_LOG10_B_ION_INTENSITY, log10f(ib + 1e-8f));
-  F_OUT(ADSR_F_LOG10_Y_ION_INTENSITY, log10f(iy + 1e-8f));
+  // the three log10 features, one evaluation in their own lanes
+  const float lg = log10f(lane == ADSR_F_LOG10_B_ION_INTENSITY   ? ib + 1e-8f
+                          : lane == ADSR_F_LOG10_Y_ION_INTENSITY ? iy + 1e-8f
+                                                                 : dot_int + 1e-8f);
+  F_OUT(ADSR_F_LOG10_B_ION_INTENSITY, lg);
+  F_OUT(ADSR_F_LOG10_Y_ION_INTENSITY, lg);
   F_OUT(ADSR_F_FWHM_RT, fwhm);
   F_OUT(ADSR_F_IDF_HYPERSCORE, idf_hs);
   F_OUT(ADSR_F_IDF_XIC_DOT_PRODUCT, dot_xic);
-  F_OUT(ADSR_F_IDF_INTENSITY_DOT_PRODUCT, log10f(dot_int + 1e-8f));
+  F_OUT(ADSR_F_IDF_INTENSITY_DOT_PRODUCT, lg);
   F_OUT(ADSR_F_MEDIAN_PROFILE_SUM, med_sum);
   F_OUT(ADSR_F_MEDIAN_PROFILE_SUM_FILTERED, medf_sum);
   F_OUT(ADSR_F_NUM_PROFILES, (float)F);
   F_OUT(ADSR_F_NUM_PROFILES_FILTERED, (float)Ff);
   F_OUT(ADSR_F_NUM_OVER_0_TOP6_IDF, cnt_o0_i6);
   F_OUT(ADSR_F_NUM_OVER_50_TOP6_IDF, cnt_o50_i6);
+  if (lane < ADSR_N_FEATURES) a.feat[(size_t)lane * n_o + ci] = fv;
 }
 
 static const char *const kFeatureNames[ADSR_N_FEATURES] = {
