@@ -4436,6 +4436,8 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
       const uint16_t *cy = a.dia.cycles + e;
       uint32_t t = rust_bsearch_u16(cy, n, key16);
       e += t;
+      // a linear scan: most bins have no entry in the window, so it stops
+      // at its first load (a bound search was measured slower)
       for (; t < n; ++t) {
         const uint64_t cyc = cy[t];
         if (cyc >= c1) break;
@@ -4638,14 +4640,15 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
       s.buf[r] = (float)fnum;
     }
     __builtin_amdgcn_wave_barrier();
-    uint32_t best = cnt ? 1u : 0u, cur = 1u;
-    for (uint32_t i = 1; i < cnt; ++i) {
-      if ((uint32_t)s.buf[i] == (uint32_t)s.buf[i - 1] + 1u) {
-        ++cur;
-        best = max(best, cur);
-      } else {
-        cur = 1u;
-      }
+    // bit i: sorted number i continues number i - 1; the longest run of
+    // such bits plus one is the longest series
+    const uint32_t il = (uint32_t)lane;
+    uint64_t run = ballot64(il >= 1u && il < cnt &&
+                            (uint32_t)s.buf[il] == (uint32_t)s.buf[il - 1] + 1u);
+    uint32_t best = cnt ? 1u : 0u;
+    while (run) {
+      run &= run >> 1;
+      ++best;
     }
     longest[k] = (float)(best & 0xffu);
     __builtin_amdgcn_wave_barrier();
