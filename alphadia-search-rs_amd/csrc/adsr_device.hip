@@ -78,6 +78,8 @@ struct KDia {
   const float *rt;
   uint32_t n_rt;
   float delta_t;
+  uint32_t rt_strict;  // RT axis strictly increasing
+  float rt_inv_dt;     // 1 / mean spacing when rt_strict (else 0)
   uint32_t n_obs;
   const float *iso_lo, *iso_hi;
   const uint32_t *ss;  // n_obs * (n_grid + 1)
@@ -141,6 +143,20 @@ __device__ __forceinline__ uint32_t rust_bsearch_f32(const float *v,
   const float b = v[base];
   if (b == key) return base;
   return base + (b < key ? 1u : 0u);
+}
+
+// rust_bsearch_f32 over the RT axis.  On a strictly increasing axis every
+// value is unique, so the binary search's result is the lower bound of the
+// key, found here by a walk from the index the mean spacing predicts (one
+// or two probes on an evenly spaced axis instead of about 13).
+__device__ __forceinline__ uint32_t rt_search(const float *rt, uint32_t n, uint32_t strict,
+                                              float inv_dt, float key) {
+  if (!strict || n == 0 || key != key) return rust_bsearch_f32(rt, n, key);
+  const float gf = fminf(fmaxf((key - rt[0]) * inv_dt, 0.0f), (float)(n - 1));
+  uint32_t g = (uint32_t)gf;
+  while (g > 0 && rt[g - 1] >= key) --g;
+  while (g < n && rt[g] < key) ++g;
+  return g;
 }
 
 __device__ __forceinline__ uint32_t grid_lb(const KDia &d, float x) {
@@ -345,8 +361,8 @@ __global__ __launch_bounds__(kWave) void plan_kernel(KDia dia, KLib lib,
     } else if (lo > dia.rt[n - 1]) {
       c0 = c1 = n;
     } else {
-      c0 = (lo <= dia.rt[0]) ? 0u : rust_bsearch_f32(dia.rt, n, lo);
-      c1 = (hi >= dia.rt[n - 1]) ? n : rust_bsearch_f32(dia.rt, n, hi);
+      c0 = (lo <= dia.rt[0]) ? 0u : rt_search(dia.rt, n, dia.rt_strict, dia.rt_inv_dt, lo);
+      c1 = (hi >= dia.rt[n - 1]) ? n : rt_search(dia.rt, n, dia.rt_strict, dia.rt_inv_dt, hi);
     }
   }
   // src/dia_data/mod.rs:79-87 valid observations (inclusive, ascending)
@@ -399,17 +415,24 @@ __global__ __launch_bounds__(kWave) void plan_kernel(KDia dia, KLib lib,
 // window, valid windows and fragment filter/order as plan_kernel, one
 // fragment per lane, so the O(n^2) orderings cost O(n) per lane.
 // ---------------------------------------------------------------------------
+constexpr int kPlanThreads = 256;  // 4 waves per block: more waves per CU than 1-wave blocks
+
 template <int LP>
-__global__ __launch_bounds__(kWave) void plan_group_kernel(KDia dia, KLib lib,
-                                                           KParams prm, KPlan plan,
-                                                           uint32_t *status) {
-  __shared__ float g_mz[kWave], g_I[kWave];
-  __shared__ uint32_t g_key[kWave];
+__global__ __launch_bounds__(kPlanThreads) void plan_group_kernel(KDia dia, KLib lib,
+                                                                  KParams prm, KPlan plan,
+                                                                  uint32_t *status) {
+  __shared__ float g_mz_b[kPlanThreads], g_I_b[kPlanThreads];
+  __shared__ uint32_t g_key_b[kPlanThreads];
   const int lane = lane_id();
+  const int wv = (int)threadIdx.x / kWave;
+  // this wave's slice of the block's LDS, indexed by lane
+  float *const g_mz = g_mz_b + wv * kWave, *const g_I = g_I_b + wv * kWave;
+  uint32_t *const g_key = g_key_b + wv * kWave;
   const int gl = lane % LP;           // lane within the row's group
   const int gb = lane - gl;           // first lane of the group
   const uint64_t gmask = (LP == 64) ? ~0ull : (((1ull << LP) - 1) << gb);
-  const uint32_t row = blockIdx.x * (kWave / LP) + (uint32_t)(lane / LP);
+  const uint32_t row = (blockIdx.x * (kPlanThreads / kWave) + (uint32_t)wv) * (kWave / LP) +
+                       (uint32_t)(lane / LP);
   const bool live = row < lib.n_prec;
   uint32_t st = 0;
   // src/peak_group_selection/mod.rs:139-148 -> rt_index/mod.rs:89-138
@@ -427,8 +450,8 @@ __global__ __launch_bounds__(kWave) void plan_group_kernel(KDia dia, KLib lib,
       } else if (lo > dia.rt[n - 1]) {
         c0 = c1 = n;
       } else {
-        c0 = (lo <= dia.rt[0]) ? 0u : rust_bsearch_f32(dia.rt, n, lo);
-        c1 = (hi >= dia.rt[n - 1]) ? n : rust_bsearch_f32(dia.rt, n, hi);
+        c0 = (lo <= dia.rt[0]) ? 0u : rt_search(dia.rt, n, dia.rt_strict, dia.rt_inv_dt, lo);
+        c1 = (hi >= dia.rt[n - 1]) ? n : rt_search(dia.rt, n, dia.rt_strict, dia.rt_inv_dt, hi);
       }
     }
   }
@@ -2118,6 +2141,9 @@ static KDia make_kdia(const adsr_dia *d) {
   k.rt = dv.rt;
   k.n_rt = (uint32_t)d->rt_index.size();
   k.delta_t = d->delta_t;
+  k.rt_strict = d->rt_strict ? 1u : 0u;
+  k.rt_inv_dt = (d->rt_strict && std::isfinite(d->delta_t) && d->delta_t > 0.0f)
+                    ? 1.0f / d->delta_t : 0.0f;
   k.n_obs = (uint32_t)d->n_obs;
   k.iso_lo = dv.iso_lo;
   k.iso_hi = dv.iso_hi;
@@ -2380,15 +2406,16 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
   // fragment lists of at most 64 entries: a lane group per row; longer
   // ones: one thread per row with an LDS stash / global scratch
   auto launch_plan = [&]() {
+    constexpr uint32_t wpb = kPlanThreads / kWave;  // rows per block: wpb * 64 / LP
     if (sp.max_raw <= 16)
-      plan_group_kernel<16><<<(n_rows + 3) / 4, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm,
-                                                              sp.plan, status);
+      plan_group_kernel<16><<<(n_rows + 4 * wpb - 1) / (4 * wpb), kPlanThreads, 0, s>>>(
+          sp.dia, sp.lib, sp.prm, sp.plan, status);
     else if (sp.max_raw <= 32)
-      plan_group_kernel<32><<<(n_rows + 1) / 2, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm,
-                                                              sp.plan, status);
+      plan_group_kernel<32><<<(n_rows + 2 * wpb - 1) / (2 * wpb), kPlanThreads, 0, s>>>(
+          sp.dia, sp.lib, sp.prm, sp.plan, status);
     else if (sp.max_raw <= 64)
-      plan_group_kernel<64><<<n_rows, kWave, 0, s>>>(sp.dia, sp.lib, sp.prm, sp.plan,
-                                                      status);
+      plan_group_kernel<64><<<(n_rows + wpb - 1) / wpb, kPlanThreads, 0, s>>>(
+          sp.dia, sp.lib, sp.prm, sp.plan, status);
     else
       plan_kernel<kStashFrag><<<(n_rows + kWave - 1) / kWave, kWave, 0, s>>>(
           sp.dia, sp.lib, sp.prm, sp.plan, w.key_scratch, status);

@@ -260,6 +260,8 @@ adsr_status dia_prepare(const int64_t *delta, const float *iso_lo,
                        : 1.0f;
   for (size_t i = 1; i < nrt; ++i)
     if (d->rt_index[i] < d->rt_index[i - 1]) d->rt_sorted = false;
+  for (size_t i = 1; i < nrt; ++i)
+    if (!(d->rt_index[i] > d->rt_index[i - 1])) d->rt_strict = false;
 
   // src/dia_data_builder/mod.rs:33-46: observations 0..=max(delta)
   int64_t maxd = delta[0];

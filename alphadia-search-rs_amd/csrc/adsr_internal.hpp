@@ -127,6 +127,7 @@ struct adsr_dia {
   std::vector<uint16_t> cycles;
   std::vector<float> intens;
   bool rt_sorted = true;
+  bool rt_strict = true;  // strictly increasing
   // false: the CSR was built on the device and lives only in `dev`
   bool host_csr = true;
   double build_ms[3] = {0, 0, 0};  // device build: prepare, copy, kernels

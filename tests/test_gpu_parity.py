@@ -172,9 +172,27 @@ def test_workload0_candidates_match_oracle(ads, run0):
 
 
 @pytest.mark.parametrize("variant", ["decoys_uneven", "per_window_sigma", "ties", "skips",
-                                     "cross_bin_sums", "top_k_drop_one", "long_fragment_lists"])
+                                     "cross_bin_sums", "top_k_drop_one", "long_fragment_lists",
+                                     "uneven_rt", "repeated_rt"])
 def test_small_variants_match_oracle(ads, variant):
-    if variant == "decoys_uneven":
+    if variant in ("uneven_rt", "repeated_rt"):
+        # RT axes other than one value per cycle: unevenly spaced but strictly
+        # increasing (the device's guessed search walks further), and with
+        # every value twice (the reference's binary search on equal values)
+        spec = synthetic.config(1, n_cycles=500, n_windows=6, peaks_per_spectrum=600,
+                                n_targets=800, half_window_min=40, half_window_max=40)
+        gen = synthetic.generate(spec)
+        dia = list(gen["dia"])
+        c = dia[5].astype(np.float64)
+        if variant == "uneven_rt":
+            rt = c + 0.4 * np.sin(1.7 * c) + 3.0 * (c > 250)
+        else:
+            rt = np.floor(c / 2.0) * 2.0
+        dia[6] = rt.astype(np.float32)
+        gen = dict(gen, dia=tuple(dia))
+        got = _search(ads, gen)[3]
+        want = _oracle(gen)
+    elif variant == "decoys_uneven":
         spec = synthetic.config(2, n_cycles=600, n_windows=12, peaks_per_spectrum=800,
                                 n_targets=1500)
         gen = synthetic.generate(spec)
