@@ -490,6 +490,40 @@ class CandidateCollection:
             check(N.lib.adsr_candidates_export(self._h, *[ptr(a) for a in cols]))
         return tuple(cols)
 
+    # The file hand-off of scripts/candidate_selection.py:121-209 writes the
+    # nine to_arrays columns to parquet (there renamed frame_* and rescaled
+    # by the run's cycle length, with the library's elution group and decoy
+    # columns merged in, which is alphaDIA's side).  Here the columns keep
+    # their to_arrays names and dtypes, so that a table read back is the
+    # same CandidateCollection.
+    COLUMNS = ("precursor_idx", "rank", "score", "scan_center", "scan_start",
+               "scan_stop", "cycle_center", "cycle_start", "cycle_stop")
+
+    def to_arrow(self):
+        """The nine columns as a pyarrow Table (u64 columns, score f32)."""
+        import pyarrow as pa
+        return pa.table(dict(zip(self.COLUMNS, self.to_arrays())))
+
+    def to_parquet(self, path) -> None:
+        """Writes ``to_arrow()`` to a parquet file."""
+        import pyarrow.parquet as pq
+        pq.write_table(self.to_arrow(), path)
+
+    @staticmethod
+    def from_arrow(table) -> "CandidateCollection":
+        """The collection of a table with the nine columns (extra columns
+        are ignored); ValueError when one is missing."""
+        missing = [c for c in CandidateCollection.COLUMNS if c not in table.column_names]
+        if missing:
+            raise ValueError(f"candidate table lacks columns {missing}")
+        return CandidateCollection.from_arrays(
+            *[table.column(c).to_numpy() for c in CandidateCollection.COLUMNS])
+
+    @staticmethod
+    def read_parquet(path) -> "CandidateCollection":
+        import pyarrow.parquet as pq
+        return CandidateCollection.from_arrow(pq.read_table(path))
+
 
 class DeviceCandidateCollection:
     """The candidate table of ``PeakGroupSelection.search_device``, kept in
@@ -531,6 +565,12 @@ class DeviceCandidateCollection:
                 self._h, *[C.c_void_p(t.data_ptr()) for t in cols],
                 C.c_void_p(s.cuda_stream)))
         return tuple(cols)
+
+    def to_dlpack(self, stream=None):
+        """The nine columns of ``to_torch`` as DLPack capsules, for consumers
+        outside torch; the tensors stay alive through the capsules."""
+        from torch.utils.dlpack import to_dlpack
+        return tuple(to_dlpack(t) for t in self.to_torch(stream))
 
     def to_collection(self) -> "CandidateCollection":
         """The host ``CandidateCollection`` of the same table."""

@@ -43,6 +43,13 @@ def test_device_table_equals_host_table(ads, run0):
     torch.cuda.synchronize()
     assert_tables_equal(_host_view(cols), want, "device table")
     assert_tables_equal(dc.to_collection().to_arrays(), want, "device table to host")
+    # DLPack capsules of the same columns, read back through torch
+    from torch.utils.dlpack import from_dlpack
+    caps = dc.to_dlpack()
+    back = [from_dlpack(cp) for cp in caps]
+    torch.cuda.synchronize()
+    assert all(b.is_cuda for b in back)
+    assert_tables_equal(_host_view(back), want, "device table through DLPack")
 
 
 def test_device_table_on_side_stream(ads, run0):

@@ -277,3 +277,34 @@ def test_diff_rejects_shared_fragment_ranges_and_needs_gpu(run0):
     from alphadia_search_rs.autograd import SelectionGradState
     with pytest.raises(RuntimeError, match="gfx950"):
         SelectionGradState(d, ok, ads.SelectionParameters())
+
+
+def test_candidate_parquet_round_trip(tmp_path):
+    """The file hand-off (scripts/candidate_selection.py:121-209 writes the
+    to_arrays columns to parquet): columns, dtypes and values survive a
+    write and read, extra columns are ignored, a missing one is an error."""
+    import pyarrow as pa
+    import alphadia_search_rs as ads
+    rng = np.random.default_rng(3)
+    n = 1000
+    cols = [rng.integers(0, 2 ** 63, n, dtype=np.uint64) if i != 2 else
+            rng.standard_normal(n).astype(np.float32) for i in range(9)]
+    cols[0][0] = np.uint64(2 ** 64 - 1)  # the u64 range survives
+    c = ads.CandidateCollection.from_arrays(*cols)
+    t = c.to_arrow()
+    assert t.column_names == list(ads.CandidateCollection.COLUMNS)
+    assert t.schema.field("score").type == pa.float32()
+    assert all(t.schema.field(k).type == pa.uint64()
+               for k in ads.CandidateCollection.COLUMNS if k != "score")
+    path = tmp_path / "candidates.parquet"
+    c.to_parquet(path)
+    back = ads.CandidateCollection.read_parquet(path)
+    for a, b in zip(back.to_arrays(), cols):
+        assert a.dtype == b.dtype and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    wider = t.append_column("decoy", pa.array(np.zeros(n, np.uint8)))
+    assert ads.CandidateCollection.from_arrow(wider).len() == n
+    with pytest.raises(ValueError):
+        ads.CandidateCollection.from_arrow(t.drop(["rank"]))
+    empty = ads.CandidateCollection()
+    empty.to_parquet(tmp_path / "empty.parquet")
+    assert ads.CandidateCollection.read_parquet(tmp_path / "empty.parquet").len() == 0
