@@ -616,6 +616,9 @@ constexpr int kTileHits = 4 * 64;  // hits one extraction tile can add
 // extraction tiles in flight (8 entries per lane each); one measured
 // fastest: a second tile's registers cost more than its latency hiding
 constexpr int kGroupTiles = 1;
+// a batch's last items up to this count run one output per lane (one
+// pass of 64 outputs; 16 measured best, 32 and 48 slower)
+constexpr int kTailItems = 16;
 
 struct LdsShape {
   uint32_t lp;     // XIC row stride (floats)
@@ -1283,6 +1286,41 @@ void select_kernel(SelArgs a) {
           // conv + ln + weighted score, 1 block per lane per pass; score
           // updates are applied fragment by fragment (reference row order)
           for (uint32_t c = 0; c < n; c += kWave) {
+            if (HALF == 10 && n - c <= (uint32_t)kTailItems) {
+              // The last items, one output per lane (4 lanes per item): a
+              // pass of 64 blocks for them would leave most lanes idle.
+              // Each output is the block pass's chain for that output.
+              const uint32_t no = 4 * (n - c);
+              for (uint32_t c2 = 0; c2 < no; c2 += kWave) {
+                const uint32_t oi = c2 + lane;
+                const bool v = oi < no;
+                const uint32_t item = l.items[c + (v ? oi >> 2 : 0u)];
+                const uint32_t f = item >> 10;
+                const uint32_t pos = (item & 0x3ffu) * 4 + (oi & 3u);
+                const float *xc = l.X + f * l.lp + kXPad + pos;
+                float y = xc[0] * k10[10];
+#pragma unroll
+                for (int j = 0; j < 10; ++j) {
+                  const int d = 10 - j;
+                  y = y + (xc[-d] + xc[d]) * k10[j];
+                }
+                const bool act = v && pos >= half && pos + half < L && y != 0.0f;
+                const float wf = l.w[g0 + f];
+                float t = 0.0f;
+                if (ballot64(act && !logf_is_normal(y + 1.0f)) == 0) {
+                  if (act) t = logf_normal(y + 1.0f, ltab) * wf;
+                } else {
+                  if (act) t = logf_exact(y + 1.0f, ltab) * wf;
+                }
+                const uint32_t flo = rdlane(item >> 10, 0);
+                const uint32_t fhi = rdlane(item >> 10, min(no - c2, (uint32_t)kWave) - 1);
+                for (uint32_t ff = flo; ff <= fhi; ++ff) {
+                  if (act && f == ff) l.R[pos] = l.R[pos] + t;
+                  __builtin_amdgcn_wave_barrier();
+                }
+              }
+              break;
+            }
             const uint32_t idx = c + lane;
             const bool v = idx < n;
             const uint32_t item = v ? l.items[idx] : l.items[c];
