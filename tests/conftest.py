@@ -68,3 +68,25 @@ def assert_tables_equal(got, want, what=""):
         assert bad.size == 0, (
             f"{what}: column {n} differs at {bad.size} rows, first {bad[:5]}: "
             f"got {a[bad[:5]]} want {b[bad[:5]]}")
+
+
+def with_peak_everywhere(gen, mz_add, intensity):
+    """Adds one peak at mz_add to every spectrum (kept sorted by m/z)."""
+    dia = list(gen["dia"])
+    start, stop, mz, it = dia[3], dia[4], dia[7], dia[8]
+    new_mz, new_it, new_start, new_stop = [], [], [], []
+    pos = 0
+    for s0, s1 in zip(start, stop):
+        m = np.concatenate([mz[s0:s1], np.array([mz_add], np.float32)])
+        v = np.concatenate([it[s0:s1], np.array([intensity], np.float32)])
+        o = np.argsort(m, kind="stable")
+        new_mz.append(m[o])
+        new_it.append(v[o])
+        new_start.append(pos)
+        pos += len(m)
+        new_stop.append(pos)
+    dia[3] = np.array(new_start, np.int64)
+    dia[4] = np.array(new_stop, np.int64)
+    dia[7] = np.concatenate(new_mz).astype(np.float32)
+    dia[8] = np.concatenate(new_it).astype(np.float32)
+    return dict(gen, dia=tuple(dia))

@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 import synthetic
-from conftest import assert_tables_equal, oracle_params_for, params_for
+from conftest import assert_tables_equal, oracle_params_for, params_for, with_peak_everywhere
 
 pytestmark = pytest.mark.gpu
 
@@ -289,28 +289,6 @@ def test_small_variants_match_oracle(ads, variant):
     assert_tables_equal(got, want, variant)
 
 
-def _with_peak_everywhere(gen, mz_add, intensity):
-    """Adds one peak at mz_add to every spectrum (kept sorted by m/z)."""
-    dia = list(gen["dia"])
-    start, stop, mz, it = dia[3], dia[4], dia[7], dia[8]
-    new_mz, new_it, new_start, new_stop = [], [], [], []
-    pos = 0
-    for s0, s1 in zip(start, stop):
-        m = np.concatenate([mz[s0:s1], np.array([mz_add], np.float32)])
-        v = np.concatenate([it[s0:s1], np.array([intensity], np.float32)])
-        o = np.argsort(m, kind="stable")
-        new_mz.append(m[o])
-        new_it.append(v[o])
-        new_start.append(pos)
-        pos += len(m)
-        new_stop.append(pos)
-    dia[3] = np.array(new_start, np.int64)
-    dia[4] = np.array(new_stop, np.int64)
-    dia[7] = np.concatenate(new_mz).astype(np.float32)
-    dia[8] = np.concatenate(new_it).astype(np.float32)
-    return dict(gen, dia=tuple(dia))
-
-
 def test_hit_list_overflow_scores_bitwise(ads):
     """Rows with more in-window entries than the device keeps in its per-row
     hit list (256): dense spectra, 40 ppm, 20-24 fragments and long RT
@@ -322,7 +300,7 @@ def test_hit_list_overflow_scores_bitwise(ads):
     spec = synthetic.config(2, n_cycles=700, n_windows=4, peaks_per_spectrum=3000,
                             n_targets=300, frag_min=20, frag_max=24,
                             half_window_min=150, half_window_max=300)
-    gen = _with_peak_everywhere(synthetic.generate(spec), 777.7777, 350.0)
+    gen = with_peak_everywhere(synthetic.generate(spec), 777.7777, 350.0)
     lib = [a.copy() for a in gen["lib"]]
     fs = lib[6].astype(np.int64)
     lib[9][fs[::4]] = np.float32(777.7777)

@@ -14,7 +14,7 @@ import pytest
 
 import oracle
 import synthetic
-from conftest import params_for
+from conftest import params_for, with_peak_everywhere
 
 pytestmark = pytest.mark.gpu
 
@@ -104,3 +104,47 @@ def test_scoring_edge_rows(ads, run0):
     _, _, _, got, want = _run(ads, gen, extra_rows=extra)
     assert 10 ** 9 not in set(got["precursor_idx"].tolist())
     _assert_features_equal(got, want, "edge rows")
+
+
+def test_scoring_entry_pool_overflow(ads):
+    """Candidates whose bins hold more in-window entries than the kernel
+    stages in LDS per pass of 64 bins (136): 21 peaks 2 ppm apart around
+    each of 4 m/z values in every spectrum, fragments placed on them, 25 ppm
+    (every one of a fragment's dense bins has an entry in each of the 11
+    cycles).  The entries past the pool are read again from HBM; features
+    identical to the oracle's."""
+    spec = synthetic.config(0, n_cycles=300, n_windows=4, peaks_per_spectrum=800,
+                            n_targets=300, half_window_min=40, half_window_max=40)
+    gen = synthetic.generate(spec)
+    base = np.linspace(300.0, 1500.0, 4)
+    for m in base:
+        for d in range(-20, 21, 2):
+            gen = with_peak_everywhere(gen, np.float32(m * (1 + d * 1e-6)), 500.0 + d)
+    lib = [a.copy() for a in gen["lib"]]
+    fs, fe = lib[6].astype(np.int64), lib[7].astype(np.int64)
+    for r in range(0, len(fs), 3):
+        k = min(4, int(fe[r] - fs[r]))
+        lib[9][fs[r]:fs[r] + k] = base[:k].astype(np.float32)
+    gen = dict(gen, lib=tuple(lib))
+    _, _, _, got, want = _run(ads, gen, tol=25.0)
+    assert len(want["precursor_idx"]) > 500
+    _assert_features_equal(got, want, "entry pool overflow")
+
+
+def test_scoring_too_many_windows_is_unsupported(ads):
+    """A precursor m/z inside more than 16 isolation windows: the device
+    reports the limit (the reference would extract from all of them)."""
+    import alphadia_search_rs._native as N
+    spec = synthetic.config(0, n_cycles=60, n_windows=20, peaks_per_spectrum=200,
+                            n_targets=40, half_window_min=10, half_window_max=10)
+    gen = synthetic.generate(spec)
+    dia = list(gen["dia"])
+    dia[1] = np.full_like(dia[1], 400.0)   # every window covers 400-1000
+    dia[2] = np.full_like(dia[2], 1000.0)
+    d = ads.DIAData.from_arrays(*dia)
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    pidx = int(gen["lib"][0][0])
+    cands = ads.CandidateCollection.from_arrays([pidx], [0], [1.0], [0], [0], [0],
+                                                [30], [25], [36])
+    with pytest.raises(N.AdsrError, match="isolation windows"):
+        ads.PeakGroupScoring(ads.ScoringParameters()).score(d, lib, cands)
