@@ -143,5 +143,11 @@ def rank_zero_scores(state: SelectionGradState, sigma, weights):
 
 def hinge_loss(s_star: torch.Tensor, targets: torch.Tensor,
                decoys: torch.Tensor, margin: float = 1.0) -> torch.Tensor:
-    """sum over pairs of max(0, margin - (s*_target - s*_decoy))."""
-    return torch.clamp(margin - (s_star[targets] - s_star[decoys]), min=0.0).sum()
+    """sum over pairs of max(0, margin - (s*_target - s*_decoy)).
+
+    The rows are gathered with index_select, whose backward adds the pairs'
+    gradients with atomics instead of sorting the indices; each precursor's
+    gradient is a sum of -1, 0 and +1 terms, exact in any order."""
+    st = s_star.index_select(0, targets)
+    sd = s_star.index_select(0, decoys)
+    return torch.clamp(margin - (st - sd), min=0.0).sum()

@@ -3125,6 +3125,11 @@ struct DiffState {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float fwd_ms = 0.0f, bwd_ms = 0.0f;
   bool have_forward = false;
+  // host copies of the taps and their sigma derivatives, uploaded on the
+  // workspace stream; alive until the next forward, which starts after the
+  // previous one has synchronized
+  std::vector<float> taps_h;
+  std::vector<double> dk_h;
 };
 
 // d k[j] / d sigma for the taps j = 0..half of gaussian_kernel_1d
@@ -3225,16 +3230,20 @@ adsr_status adsr_diff_forward(adsr_diff *h, const float *sigma,
     if (!(s->sig[o] > 0.0f) || !std::isfinite(s->sig[o]))
       return set_error(ADSR_ERR_INVALID_ARGUMENT, "sigma[%zu] must be > 0", o);
   // taps: index 0 from fwhm_rt, 1 + o from sigma[o]
-  std::vector<float> taps(nk * klen);
+  std::vector<float> &taps = s->taps_h;
+  taps.assign(nk * klen, 0.0f);
   adsr_gaussian_kernel(s->p.fwhm_rt, 1.0f, s->p.kernel_size, 1.0f, taps.data(), klen);
   for (size_t o = 0; o < d->n_obs; ++o)
     adsr_gaussian_kernel_sigma(s->sig[o], klen, taps.data() + (1 + o) * klen, klen);
-  std::vector<double> dk(nk * (half + 1));
+  std::vector<double> &dk = s->dk_h;
+  dk.assign(nk * (half + 1), 0.0);
   kernel_dsigma((double)(s->p.fwhm_rt / 2.3548f), klen, dk.data());
   for (size_t o = 0; o < d->n_obs; ++o)
     kernel_dsigma((double)s->sig[o], klen, dk.data() + (1 + o) * (half + 1));
-  HIP_TRY(hipMemcpy(s->w.kernels, taps.data(), taps.size() * 4, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(s->dk, dk.data(), dk.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(s->w.kernels, taps.data(), taps.size() * 4, hipMemcpyHostToDevice,
+                         s->w.stream));
+  HIP_TRY(hipMemcpyAsync(s->dk, dk.data(), dk.size() * 8, hipMemcpyHostToDevice,
+                         s->w.stream));
   s->sp.lib.f_int = weights;
   LaunchShape shape{4, 2};
   const uint32_t n = (uint32_t)l->n_prec;
