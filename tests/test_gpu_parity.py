@@ -346,3 +346,35 @@ def test_hit_list_overflow_scores_bitwise(ads):
     got = ads.PeakGroupSelection(p).search(d, slib).to_arrays()
     want = oracle.select(od, ol, prm)
     assert_tables_equal(got, want, "hit-list overflow")
+
+
+def test_concurrent_searches_match_sequential(ads, run0):
+    """Searches from four Python threads on the same DIAData and SpecLibFlat
+    (ctypes releases the GIL; each search has its own stream and
+    workspace) give the tables of the same searches run one after another."""
+    import threading
+    d = ads.DIAData.from_arrays(*run0["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*run0["lib"])
+    variants = [dict(), dict(candidate_count=5), dict(top_k_fragments=9),
+                dict(mass_tolerance=12.0)]
+    params = [params_for(run0["spec"], **v) for v in variants]
+    want = [ads.PeakGroupSelection(p).search(d, lib).to_arrays() for p in params]
+    got = [None] * len(params)
+    errors = []
+
+    def run(i):
+        try:
+            for _ in range(3):
+                got[i] = ads.PeakGroupSelection(params[i]).search(d, lib).to_arrays()
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(len(params))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for i in range(len(params)):
+        assert got[i] is not None, f"thread {i} did not finish"
+        assert_tables_equal(got[i], want[i], f"concurrent variant {variants[i]}")
