@@ -601,6 +601,14 @@ struct SelArgs {
   uint32_t *status;
   float *dbg_scores;
   uint32_t dbg_stride, dbg_row0, dbg_rows;
+  // Rows longer than split_l cycles (<= lcap) are not searched but appended
+  // to defer_rows; a second launch searches the rows of such a list:
+  // rows[q] for q < *n_rows_at (rows == nullptr: row q, q < n_rows).
+  uint32_t split_l;
+  uint32_t *defer_rows;
+  uint32_t *defer_count;
+  const uint32_t *rows;
+  const uint32_t *n_rows_at;
 };
 
 // Per-wave LDS.  XIC rows have kXPad zero floats on both sides so that a
@@ -696,11 +704,10 @@ __device__ __forceinline__ WaveLds carve(char *base, uint32_t L, uint32_t nsel,
   return l;
 }
 
-__global__ void reset_queue_kernel(uint32_t *q, uint32_t *status) {
-  if (threadIdx.x == 0) {
-    q[0] = 0;
-    status[0] = 0;
-  }
+// small[0]: queue, [1]: status, [3]: queue of the long-row launch,
+// [4]: deferred row count ([2], the candidate total, is written by the scan)
+__global__ void reset_queue_kernel(uint32_t *small) {
+  if (threadIdx.x < 5 && threadIdx.x != 2) small[threadIdx.x] = 0;
 }
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t lane) {
@@ -847,6 +854,7 @@ void select_kernel(SelArgs a) {
   // that their latency overlaps the current row's work.
   // The queue is grabbed one chunk ahead: `ahead` is the start of the
   // chunk after the current one, requested when the current one began.
+  const uint32_t n_rows = a.n_rows_at ? *a.n_rows_at : a.n_rows;
   uint32_t nxt = 0, cend = 0, ahead = 0;
   auto request = [&]() {
     uint32_t r0 = 0;
@@ -855,8 +863,8 @@ void select_kernel(SelArgs a) {
   };
   auto grab = [&]() {
     nxt = uni(__shfl(ahead, 0, kWave));
-    cend = min(nxt + a.chunk, a.n_rows);
-    ahead = nxt < a.n_rows ? request() : 0xffffffffu;
+    cend = min(nxt + a.chunk, n_rows);
+    ahead = nxt < n_rows ? request() : 0xffffffffu;
   };
   ahead = request();
   // Outputs of the last finished row are stored after the next row's first
@@ -879,8 +887,11 @@ void select_kernel(SelArgs a) {
   uint4 p_elo = make_uint4(0u, 0u, 0u, 0u), p_ehi = make_uint4(0u, 0u, 0u, 0u);
   uint2 p_rng = make_uint2(0u, 0u);
   uint64_t p_eb = 0;
+  uint32_t p_row = 0;
   const float mtol = a.prm.mass_tol;
-  auto stage1 = [&](uint32_t r) {
+  auto stage1 = [&](uint32_t q) {
+    const uint32_t r = a.rows ? a.rows[q] : q;
+    p_row = r;
     p_rec = a.plan.rec[r];
     p_sw = lane < (int)ks ? a.plan.sel_w[(size_t)r * ks + lane] : 0.0f;
     p_smz = lane < (int)ks ? a.plan.sel_mz[(size_t)r * ks + lane] : 0.0f;
@@ -927,21 +938,21 @@ void select_kernel(SelArgs a) {
     }
   };
   grab();
-  if (nxt < a.n_rows) {
+  if (nxt < n_rows) {
     stage1(nxt);
     stage2();
     stage3();
   }
-  while (nxt < a.n_rows) {
+  while (nxt < n_rows) {
     {
-      const uint32_t row = nxt;
+      const uint32_t row = p_row;
       const uint4 rec = p_rec;
       const float sw = p_sw, tap = p_tap;
       const uint32_t obs_l = p_obs, bs_l = p_bs, be_l = p_be;
       const uint2 rng0 = p_rng;
       const uint64_t eb_l = p_eb;
       if (++nxt >= cend) grab();
-      const bool more = nxt < a.n_rows;
+      const bool more = nxt < n_rows;
       if (more) stage1(nxt);
       const uint32_t c0 = uni(rec.x), c1 = uni(rec.y);
       const uint32_t nsel = uni(rec.z & 0xffffu), nobs = uni(rec.z >> 16);
@@ -958,9 +969,16 @@ void select_kernel(SelArgs a) {
         continue;
       }
       const uint32_t L = c1 - c0;
-      if (L > a.lcap) {  // excluded by the host bound; flagged if it happens
-        a.cand_count[row] = 0;
-        atomicOr(a.status, (uint32_t)kStWindowTooLong);
+      if (L > a.split_l) {
+        if (L > a.lcap) {  // excluded by the host bound; flagged if it happens
+          a.cand_count[row] = 0;
+          atomicOr(a.status, (uint32_t)kStWindowTooLong);
+        } else {
+          // deferred to the launch over long rows.  Every lane adds (lane 0
+          // one, the others zero), so lane 0's old value is the row's slot.
+          const uint32_t slot = uni(atomicAdd(a.defer_count, lane == 0 ? 1u : 0u));
+          a.defer_rows[slot] = row;
+        }
         if (more) {
           stage2();
           stage3();
@@ -2440,7 +2458,7 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
                            bool plan_only = false, bool with_bins = false) {
   hipStream_t s = w.stream;
   uint32_t *queue = w.small, *status = w.small + 1, *total = w.small + 2;
-  reset_queue_kernel<<<1, 64, 0, s>>>(queue, status);
+  reset_queue_kernel<<<1, 64, 0, s>>>(w.small);
   // fragment lists of at most 64 entries: a lane group per row; longer
   // ones: one thread per row with an LDS stash / global scratch
   auto launch_plan = [&]() {
@@ -2493,13 +2511,26 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
       shape.waves >>= 1;
     // the wave's share of the budget, less its part of the block's static
     // LDS (logf table), so that the budget's occupancy is kept
+    const uint32_t wv = (uint32_t)shape.waves;
+    const uint32_t fit = (sp.lds_budget * wv - kSelectStaticLds) / wv / 16 * 16;
     a.lds_per_wave = sp.lds_per_wave;
-    if (sp.lds_grow) {
-      const uint32_t wv = (uint32_t)shape.waves;
-      const uint32_t fit = (sp.lds_budget * wv - kSelectStaticLds) / wv / 16 * 16;
-      if ((size_t)fit * wv <= kMaxDynLds) a.lds_per_wave = std::max(a.lds_per_wave, fit);
+    if (sp.lds_grow && (size_t)fit * wv <= kMaxDynLds)
+      a.lds_per_wave = std::max(a.lds_per_wave, fit);
+    // Rows whose one-fragment tile does not fit the budget's share would set
+    // the share, and so the occupancy, of the whole launch: when there are
+    // such rows, a first launch at the budget's share defers them (rows of
+    // up to split_l cycles are searched) and a second one searches them.
+    a.split_l = sp.lcap;
+    a.defer_rows = w.local;  // free until the scan after the select launches
+    a.defer_count = w.small + 4;
+    a.rows = nullptr;
+    a.n_rows_at = nullptr;
+    if (lds_bytes(sp.lcap, 1) > fit) {
+      uint32_t sl = 0;
+      for (uint32_t L = 64; L < sp.lcap && lds_bytes(L, 1) <= fit; L += 64) sl = L;
+      if (sl > 0) a.split_l = sl;
     }
-    const size_t lds = (size_t)a.lds_per_wave * shape.waves;
+    const bool split = a.split_l < sp.lcap;
     const bool h10 = sp.prm.half == 10;
     const void *fn =
         h10 ? (shape.waves == 1   ? (const void *)select_kernel<1, 10>
@@ -2508,20 +2539,35 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
             : (shape.waves == 1   ? (const void *)select_kernel<1, 0>
                : shape.waves == 2 ? (const void *)select_kernel<2, 0>
                                   : (const void *)select_kernel<4, 0>);
-    const int per_cu = occupancy_blocks(fn, shape.waves * kWave, lds);
-    const uint64_t want = ((uint64_t)n_rows + shape.chunk - 1) / shape.chunk;
-    const uint32_t blocks = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>((uint64_t)per_cu * sp.cus,
-                              (want + shape.waves - 1) / shape.waves));
+    auto launch_select = [&](const SelArgs &x) {
+      const size_t lds = (size_t)x.lds_per_wave * shape.waves;
+      const int per_cu = occupancy_blocks(fn, shape.waves * kWave, lds);
+      const uint64_t want = ((uint64_t)n_rows + shape.chunk - 1) / shape.chunk;
+      const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+          1, std::min<uint64_t>((uint64_t)per_cu * sp.cus,
+                                (want + shape.waves - 1) / shape.waves));
+      if (h10) {
+        if (shape.waves == 1) select_kernel<1, 10><<<blocks, 64, lds, s>>>(x);
+        else if (shape.waves == 2) select_kernel<2, 10><<<blocks, 128, lds, s>>>(x);
+        else select_kernel<4, 10><<<blocks, 256, lds, s>>>(x);
+      } else {
+        if (shape.waves == 1) select_kernel<1, 0><<<blocks, 64, lds, s>>>(x);
+        else if (shape.waves == 2) select_kernel<2, 0><<<blocks, 128, lds, s>>>(x);
+        else select_kernel<4, 0><<<blocks, 256, lds, s>>>(x);
+      }
+    };
     if (timed_select) HIP_TRY(hipEventRecord(w.ev[2], s));
-    if (h10) {
-      if (shape.waves == 1) select_kernel<1, 10><<<blocks, 64, lds, s>>>(a);
-      else if (shape.waves == 2) select_kernel<2, 10><<<blocks, 128, lds, s>>>(a);
-      else select_kernel<4, 10><<<blocks, 256, lds, s>>>(a);
+    if (split) {
+      SelArgs b = a;
+      a.lds_per_wave = fit;
+      launch_select(a);
+      b.split_l = sp.lcap;
+      b.queue = w.small + 3;
+      b.rows = w.local;
+      b.n_rows_at = w.small + 4;
+      launch_select(b);
     } else {
-      if (shape.waves == 1) select_kernel<1, 0><<<blocks, 64, lds, s>>>(a);
-      else if (shape.waves == 2) select_kernel<2, 0><<<blocks, 128, lds, s>>>(a);
-      else select_kernel<4, 0><<<blocks, 256, lds, s>>>(a);
+      launch_select(a);
     }
     if (timed_select) HIP_TRY(hipEventRecord(w.ev[3], s));
     const uint32_t nb = (n_rows + kScanBlock - 1) / kScanBlock;

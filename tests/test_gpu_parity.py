@@ -348,6 +348,53 @@ def test_hit_list_overflow_scores_bitwise(ads):
     assert_tables_equal(got, want, "hit-list overflow")
 
 
+def test_long_rows_second_launch_bitwise(ads):
+    """Workload 4's RT windows (half-windows of 50-400 cycles) on a small
+    run: rows longer than 704 cycles need more than a wave's 10 KB LDS share
+    even for one fragment, so the first select launch defers them to a
+    second launch over a device-side row list.  Both kinds of row are
+    present; every score row is compared bit for bit, then the candidate
+    table."""
+    import ctypes
+    import alphadia_search_rs._native as N
+    spec = synthetic.config(4, n_cycles=1200, n_windows=4, peaks_per_spectrum=600,
+                            n_targets=500)
+    gen = synthetic.generate(spec)
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    slib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    p = params_for(gen["spec"])
+    p.update({"rt_tolerance_per_precursor": gen["rt_tolerance_per_precursor"]})
+    n = slib.num_precursors
+    stride = 1024
+    scores = np.zeros((n, stride), np.float32)
+    cs = np.zeros(n, np.uint64)
+    ce = np.zeros(n, np.uint64)
+    c = p._c()
+    N.check(N.lib.adsr_select_scores(d._h, slib._h, ctypes.byref(c), 0, n, stride,
+                                     N.ptr(scores), N.ptr(cs), N.ptr(ce)))
+    od, ol = oracle.Dia(gen["dia"]), oracle.Lib(gen["lib"])
+    prm = oracle_params_for(gen["spec"],
+                            rt_tolerance_per_precursor=gen["rt_tolerance_per_precursor"])
+    long_rows = short_rows = 0
+    for r in range(n):
+        one = oracle.select_one(od, ol, prm, r)
+        if one is None:
+            assert ce[r] <= cs[r]
+            continue
+        assert cs[r] == one["cycle_start"] and ce[r] == one["cycle_stop"]
+        L = one["cycle_stop"] - one["cycle_start"]
+        assert L <= stride
+        long_rows += int(L > 704)
+        short_rows += int(L <= 704)
+        got = scores[r, :L]
+        bad = np.flatnonzero(got.view(np.uint32) != one["score"].view(np.uint32))
+        assert bad.size == 0, f"row {r} (L={L}): score differs at cycles {bad[:8]}"
+    assert long_rows > 50 and short_rows > 400, (long_rows, short_rows)
+    got = ads.PeakGroupSelection(p).search(d, slib).to_arrays()
+    want = oracle.select(od, ol, prm)
+    assert_tables_equal(got, want, "long rows")
+
+
 def test_concurrent_searches_match_sequential(ads, run0):
     """Searches from four Python threads on the same DIAData and SpecLibFlat
     (ctypes releases the GIL; each search has its own stream and
