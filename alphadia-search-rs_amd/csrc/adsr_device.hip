@@ -147,14 +147,27 @@ __device__ __forceinline__ uint32_t rust_bsearch_f32(const float *v,
 
 // rust_bsearch_f32 over the RT axis.  On a strictly increasing axis every
 // value is unique, so the binary search's result is the lower bound of the
-// key, found here by a walk from the index the mean spacing predicts (one
-// or two probes on an evenly spaced axis instead of about 13).
+// key, found here by a walk from the index g the mean spacing predicts.
+// rt[g - 1], rt[g] and rt[g + 1] are read at once, which settles an evenly
+// spaced axis in one round trip; the walk goes on only past them.
 __device__ __forceinline__ uint32_t rt_search(const float *rt, uint32_t n, uint32_t strict,
                                               float inv_dt, float key) {
   if (!strict || n == 0 || key != key) return rust_bsearch_f32(rt, n, key);
   const float gf = fminf(fmaxf((key - rt[0]) * inv_dt, 0.0f), (float)(n - 1));
-  uint32_t g = (uint32_t)gf;
-  while (g > 0 && rt[g - 1] >= key) --g;
+  uint32_t g = (uint32_t)gf;  // g < n
+  const float vm = g > 0 ? rt[g - 1] : 0.0f;
+  const float v0 = rt[g];
+  const float vp = g + 1 < n ? rt[g + 1] : 0.0f;
+  if (g > 0 && vm >= key) {
+    // rt[g - 1] >= key: walk down; rt[g] >= key holds at every stop
+    --g;
+    while (g > 0 && rt[g - 1] >= key) --g;
+    return g;
+  }
+  if (!(v0 < key)) return g;
+  ++g;
+  if (!(g < n && vp < key)) return g;
+  ++g;
   while (g < n && rt[g] < key) ++g;
   return g;
 }
@@ -435,11 +448,39 @@ __global__ __launch_bounds__(kPlanThreads) void plan_group_kernel(KDia dia, KLib
                        (uint32_t)(lane / LP);
   const bool live = row < lib.n_prec;
   uint32_t st = 0;
+  // The row's independent loads are issued first, so that their latencies
+  // overlap: RT, tolerance, m/z, fragment range, the first isolation windows
+  // of the group's lanes, then the fragments.
+  const float rt_tol = live && prm.rt_tol_pp ? prm.rt_tol_pp[row] : prm.rt_tol;
+  const float prt = live ? lib.p_rt[row] : 0.0f;
+  const float pmz = live ? lib.p_mz[row] : 0.0f;
+  const uint64_t fs = live ? lib.f_start[row] : 0;
+  const uint64_t fe = live ? lib.f_stop[row] : 0;
+  constexpr int kIsoBatch = 4;  // LP-window batches per round trip
+  float iso_lo[kIsoBatch], iso_hi[kIsoBatch];
+  auto load_iso = [&](uint32_t w0) {
+#pragma unroll
+    for (int u = 0; u < kIsoBatch; ++u) {
+      const uint32_t w = w0 + (uint32_t)(u * LP + gl);
+      const bool in = w < dia.n_obs;
+      iso_lo[u] = in ? dia.iso_lo[w] : 0.0f;
+      iso_hi[u] = in ? dia.iso_hi[w] : 0.0f;
+    }
+  };
+  load_iso(0);
+  const uint32_t nraw = (uint32_t)(fe - fs);
+  const bool mine = gl < (int)nraw;
+  float v = 0.0f, mz = 0.0f;
+  uint32_t ftype = 0, fnum = 0;
+  if (mine) {
+    v = lib.f_int[fs + gl];
+    mz = lib.f_mz[fs + gl];
+    ftype = lib.f_type[fs + gl];
+    fnum = lib.f_num[fs + gl];
+  }
   // src/peak_group_selection/mod.rs:139-148 -> rt_index/mod.rs:89-138
   uint32_t c0 = 0, c1 = 0;
   if (live) {
-    const float rt_tol = prm.rt_tol_pp ? prm.rt_tol_pp[row] : prm.rt_tol;
-    const float prt = lib.p_rt[row];
     const uint32_t n = dia.n_rt;
     if (n > 0) {
       const float min_tol = (10.0f + prm.pad_cycles) * dia.delta_t;
@@ -456,30 +497,30 @@ __global__ __launch_bounds__(kPlanThreads) void plan_group_kernel(KDia dia, KLib
     }
   }
   // src/dia_data/mod.rs:79-87 valid observations (inclusive, ascending)
-  const float pmz = live ? lib.p_mz[row] : 0.0f;
   uint32_t nobs = 0, first = 0xffffffffu;
-  for (uint32_t w0 = 0; w0 < dia.n_obs; w0 += LP) {
-    const uint32_t w = w0 + (uint32_t)gl;
-    const bool ok = live && w < dia.n_obs && dia.iso_lo[w] <= pmz && dia.iso_hi[w] >= pmz;
-    const uint64_t m = ballot64(ok) & gmask;
-    if (ok) {
-      const uint32_t rank = nobs + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1));
-      if (rank < (uint32_t)kObsMax) plan.obs[(size_t)row * kObsMax + rank] = (uint16_t)w;
+  for (uint32_t wb = 0; wb < dia.n_obs; wb += kIsoBatch * LP) {
+    if (wb > 0) load_iso(wb);
+#pragma unroll
+    for (int u = 0; u < kIsoBatch; ++u) {
+      const uint32_t w0 = wb + (uint32_t)(u * LP);
+      if (w0 >= dia.n_obs) break;
+      const uint32_t w = w0 + (uint32_t)gl;
+      const bool ok = live && w < dia.n_obs && iso_lo[u] <= pmz && iso_hi[u] >= pmz;
+      const uint64_t m = ballot64(ok) & gmask;
+      if (ok) {
+        const uint32_t rank = nobs + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1));
+        if (rank < (uint32_t)kObsMax) plan.obs[(size_t)row * kObsMax + rank] = (uint16_t)w;
+      }
+      if (m && first == 0xffffffffu) first = w0 + (uint32_t)(__builtin_ctzll(m) - gb);
+      nobs += (uint32_t)__builtin_popcountll(m);
     }
-    if (m && first == 0xffffffffu) first = w0 + (uint32_t)(__builtin_ctzll(m) - gb);
-    nobs += (uint32_t)__builtin_popcountll(m);
   }
   if (nobs > (uint32_t)kObsMax) st |= kStTooManyWindows;
   // src/speclib_flat/mod.rs:388-456 get_precursor_filtered(i, true, true, k)
-  const uint64_t fs = live ? lib.f_start[row] : 0;
-  const uint32_t nraw = live ? (uint32_t)(lib.f_stop[row] - fs) : 0u;
-  const bool mine = gl < (int)nraw;
-  float I = -__builtin_inff(), mz = 0.0f;
+  float I = -__builtin_inff();
   if (mine) {
-    const float v = lib.f_int[fs + gl];
-    const bool keep = v > 0.0f && !(lib.f_type[fs + gl] == 121 && lib.f_num[fs + gl] == 1);
+    const bool keep = v > 0.0f && !(ftype == 121u && fnum == 1u);
     I = keep ? v : -__builtin_inff();
-    mz = lib.f_mz[fs + gl];
   }
   g_I[lane] = I;
   g_mz[lane] = mz;
@@ -543,7 +584,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_group_kernel(KDia dia, KLib
     }
     plan.sel[(size_t)row * plan.ks + pos] = (uint16_t)gl;
     plan.sel_mz[(size_t)row * plan.ks + pos] = mz;
-    plan.sel_w[(size_t)row * plan.ks + pos] = lib.f_int[fs + gl];
+    plan.sel_w[(size_t)row * plan.ks + pos] = v;
   }
   if (live && gl == 0) {
     const uint32_t kidx = (prm.per_window && first != 0xffffffffu) ? 1u + first : 0u;

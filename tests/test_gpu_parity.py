@@ -348,18 +348,23 @@ def test_hit_list_overflow_scores_bitwise(ads):
     assert_tables_equal(got, want, "hit-list overflow")
 
 
+def _long_row_run():
+    """Workload 4's RT windows (half-windows of 50-400 cycles) on a
+    1,200-cycle run of 1,000 precursors."""
+    spec = synthetic.config(4, n_cycles=1200, n_windows=4, peaks_per_spectrum=600,
+                            n_targets=500)
+    return synthetic.generate(spec)
+
+
 def test_long_rows_second_launch_bitwise(ads):
-    """Workload 4's RT windows (half-windows of 50-400 cycles) on a small
-    run: rows longer than 704 cycles need more than a wave's 10 KB LDS share
+    """Rows longer than 704 cycles need more than a wave's 10 KB LDS share
     even for one fragment, so the first select launch defers them to a
     second launch over a device-side row list.  Both kinds of row are
     present; every score row is compared bit for bit, then the candidate
     table."""
     import ctypes
     import alphadia_search_rs._native as N
-    spec = synthetic.config(4, n_cycles=1200, n_windows=4, peaks_per_spectrum=600,
-                            n_targets=500)
-    gen = synthetic.generate(spec)
+    gen = _long_row_run()
     d = ads.DIAData.from_arrays(*gen["dia"])
     slib = ads.SpecLibFlat.from_arrays(*gen["lib"])
     p = params_for(gen["spec"])
@@ -393,6 +398,31 @@ def test_long_rows_second_launch_bitwise(ads):
     got = ads.PeakGroupSelection(p).search(d, slib).to_arrays()
     want = oracle.select(od, ol, prm)
     assert_tables_equal(got, want, "long rows")
+
+
+def test_graph_replays_with_long_rows_match_oracle(ads):
+    """The timed search: the autotuner's sweep, eager steps, then the search
+    (plan, both select launches, compaction) captured in a hipGraph and
+    replayed three times; the table of the last replay equals the oracle's."""
+    import ctypes
+    import alphadia_search_rs._native as N
+    gen = _long_row_run()
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    slib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    p = params_for(gen["spec"])
+    p.update({"rt_tolerance_per_precursor": gen["rt_tolerance_per_precursor"]})
+    c = p._c()
+    tpb, ppw = ctypes.c_int(0), ctypes.c_int(0)
+    step_ms, kern_ms, graph_ms = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    out = ctypes.c_void_p()
+    N.check(N.lib.adsr_select_timed(d._h, slib._h, ctypes.byref(c), 1, 3,
+                                    ctypes.byref(tpb), ctypes.byref(ppw),
+                                    ctypes.byref(step_ms), ctypes.byref(kern_ms),
+                                    ctypes.byref(graph_ms), ctypes.byref(out)))
+    got = ads.CandidateCollection._wrap(out.value).to_arrays()
+    assert graph_ms.value > 0.0 and tpb.value in (64, 128, 256)
+    want = _oracle(gen)
+    assert_tables_equal(got, want, "graph replay, long rows")
 
 
 def test_concurrent_searches_match_sequential(ads, run0):
