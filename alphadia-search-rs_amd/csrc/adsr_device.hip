@@ -1762,36 +1762,62 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
                                              kBwdHitCap * 8);
   if (row >= a.n_rows) return;  // whole wave
   const uint32_t half = a.prm.half;
+  // The row's loads are issued in three rounds of independent loads: what
+  // the row index gives, then what those values give (bucket entries,
+  // window bases, kernel taps), then the first window's CSR ranges.
   const uint4 rec = a.plan.rec[row];
+  const uint32_t g_raw = __builtin_bit_cast(uint32_t, a.grad_s[row]);
+  const uint32_t ncand_raw = a.cand_count[row];
+  const uint32_t cen_raw = a.cand_center[(size_t)row * a.cc];
+  const bool fl = lane < (int)ks;
+  const float smz_l = fl ? a.plan.sel_mz[(size_t)row * ks + lane] : 0.0f;
+  const uint32_t sel_l = fl ? (uint32_t)a.plan.sel[(size_t)row * ks + lane] : 0u;
+  const float selw_l = fl ? a.plan.sel_w[(size_t)row * ks + lane] : 0.0f;
+  const uint32_t obs_l = lane < kObsMax ? (uint32_t)a.plan.obs[(size_t)row * kObsMax + lane] : 0u;
+  const uint64_t fs = a.lib.f_start[row];
   const uint32_t c0 = uni(rec.x), c1 = uni(rec.y);
   const uint32_t nsel = uni(rec.z & 0xffffu), nobs = uni(rec.z >> 16);
-  const float g = __builtin_bit_cast(float, uni(__builtin_bit_cast(uint32_t, a.grad_s[row])));
-  const uint32_t ncand = uni(a.cand_count[row]);
+  const float g = __builtin_bit_cast(float, uni(g_raw));
+  const uint32_t ncand = uni(ncand_raw);
   if (ncand == 0 || g == 0.0f || c1 <= c0 || nobs == 0 || nsel == 0) {
     if (lane == 0) a.contrib[row] = 0.0;
     return;
   }
-  const uint32_t istar = uni(a.cand_center[(size_t)row * a.cc]) - c0;
+  const uint32_t istar = uni(cen_raw) - c0;
   // cycles [hc0, hc1) of the apex window; x[t] = X[hc0 + t]
   const uint32_t hc0 = c0 + istar - half;
   const uint32_t wl = max(hc0, c0), wh = min(hc0 + 2 * half + 1, c1);
-  // bin range of fragment `lane`
+  // bin range of fragment `lane`; window bases; kernel taps and their
+  // sigma derivatives (lane j holds tap j)
   uint32_t bst = 0, ben = 0;
-  if (lane < (int)nsel)
-    bin_range(a.dia, a.plan.sel_mz[(size_t)row * ks + lane], a.prm.mass_tol, bst, ben);
-  for (uint32_t i = lane; i < nsel * kWin; i += kWave) xw[i] = 0.0f;
-  __builtin_amdgcn_wave_barrier();
-  // the XIC window, summed in the forward pass's order: windows, then the
-  // fragments' CSR entries in order (8-entry groups, 64 per tile)
-  for (uint32_t j = 0; j < nobs; ++j) {
-    const uint32_t o = uni(a.plan.obs[(size_t)row * kObsMax + j]);
-    const uint64_t base = a.dia.entry_base[o];
-    uint32_t rx = 0, ry = 0;
+  if (lane < (int)nsel) bin_range(a.dia, smz_l, a.prm.mass_tol, bst, ben);
+  const uint32_t nob = min(nobs, (uint32_t)kObsMax);
+  const uint64_t eb_l = lane < (int)nob ? a.dia.entry_base[obs_l] : 0ull;
+  const float kk_l = lane <= (int)half ? a.prm.kernels[(size_t)rec.w * a.prm.klen + lane] : 0.0f;
+  const double dk_l = lane <= (int)half ? a.dk[(size_t)rec.w * (half + 1) + lane] : 0.0;
+  // CSR entry range of fragment `lane` in window j
+  auto csr_range = [&](uint32_t j, uint32_t &rx, uint32_t &ry) {
+    const uint32_t o = rdlane(obs_l, j);
+    rx = 0;
+    ry = 0;
     if (lane < (int)nsel) {
       const uint32_t *ssw = a.dia.ss + (size_t)o * (a.dia.n_grid + 1);
       rx = ssw[bst];
       ry = ssw[ben];
     }
+  };
+  uint32_t rx0, ry0;
+  csr_range(0, rx0, ry0);
+  for (uint32_t i = lane; i < nsel * kWin; i += kWave) xw[i] = 0.0f;
+  __builtin_amdgcn_wave_barrier();
+  // the XIC window, summed in the forward pass's order: windows, then the
+  // fragments' CSR entries in order (8-entry groups, 64 per tile)
+  for (uint32_t j = 0; j < nob; ++j) {
+    const uint32_t o = rdlane(obs_l, j);
+    const uint64_t base = ((uint64_t)rdlane((uint32_t)(eb_l >> 32), j) << 32) |
+                          rdlane((uint32_t)eb_l, j);
+    uint32_t rx = rx0, ry = ry0;
+    if (j > 0) csr_range(j, rx, ry);
     const uint32_t bo = (uint32_t)base & 7u;
     const uint16_t *cw = a.dia.cycles + (base - bo);
     const uint32_t gsr = (bo + rx) >> 3;
@@ -1901,26 +1927,29 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
   }
   __builtin_amdgcn_wave_barrier();
   // per fragment: Y at the apex, dY/dsigma, ln(Y+1)
+  auto tap = [&](uint32_t j) {
+    return __builtin_bit_cast(float, rdlane(__builtin_bit_cast(uint32_t, kk_l), j));
+  };
+  auto dtap = [&](uint32_t j) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, dk_l);
+    return __builtin_bit_cast(double, ((uint64_t)rdlane((uint32_t)(b >> 32), j) << 32) |
+                                          rdlane((uint32_t)b, j));
+  };
+  const float *x = xw + min((uint32_t)lane, nsel - 1) * kWin + half;  // x[0] = apex cycle
+  float y = 0.0f;
+  y = y + x[0] * tap(half);
+  double dy = (double)x[0] * dtap(half);
+  for (uint32_t jj = 0; jj < half; ++jj) {
+    const uint32_t d = half - jj;
+    y = y + (x[-(int)d] + x[d]) * tap(jj);
+    dy += ((double)x[-(int)d] + (double)x[d]) * dtap(jj);
+  }
   if (lane < (int)nsel) {
-    const float *x = xw + lane * kWin + half;  // x[0] = apex cycle
-    const float *kk = a.prm.kernels + (size_t)rec.w * a.prm.klen;
-    const double *dk = a.dk + (size_t)rec.w * (half + 1);
-    float y = 0.0f;
-    y = y + x[0] * kk[half];
-    double dy = (double)x[0] * dk[half];
-    for (uint32_t jj = 0; jj < half; ++jj) {
-      const uint32_t d = half - jj;
-      y = y + (x[-(int)d] + x[d]) * kk[jj];
-      dy += ((double)x[-(int)d] + (double)x[d]) * dk[jj];
-    }
     // d s*/d w_f = ln(1 + Y_f), evaluated as log1p in f64: the f32
     // logf(Y + 1) of the forward loses the low bits of Y when Y << 1
     const double lnv = log1p((double)y);
-    const uint32_t sel = a.plan.sel[(size_t)row * ks + lane];
-    const uint64_t fs = a.lib.f_start[row];
-    a.grad_w[fs + sel] = (float)((double)g * lnv);
-    const double wf = (double)a.plan.sel_w[(size_t)row * ks + lane];
-    part[lane] = wf * dy / ((double)y + 1.0);
+    a.grad_w[fs + sel_l] = (float)((double)g * lnv);
+    part[lane] = (double)selw_l * dy / ((double)y + 1.0);
   }
   __builtin_amdgcn_wave_barrier();
   if (lane == 0) {
