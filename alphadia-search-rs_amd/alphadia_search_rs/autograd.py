@@ -141,13 +141,69 @@ def rank_zero_scores(state: SelectionGradState, sigma, weights):
     return RankZeroScore.apply(state, sigma, weights)
 
 
+_checked_pairs: dict = {}
+
+
+def _pair_indices(t, device) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(t)
+    return t.to(device=device, dtype=torch.int64).contiguous()
+
+
+def _check_pairs(t: torch.Tensor, d: torch.Tensor, n_rows: int) -> None:
+    """Raise on an index outside [0, n_rows); an index tensor pair is
+    checked once while its storage and version stay the same (the kernel
+    itself skips such pairs, so a stale entry cannot make it read out of
+    bounds)."""
+    if t.shape != d.shape or t.dim() != 1:
+        raise ValueError("targets and decoys must be 1-D and of equal length")
+    key = (t.data_ptr(), t._version, d.data_ptr(), d._version, t.numel(), n_rows)
+    if key in _checked_pairs:
+        return
+    if t.numel():
+        lo = torch.minimum(t.min(), d.min())
+        hi = torch.maximum(t.max(), d.max())
+        if bool(lo < 0) or bool(hi >= n_rows):
+            raise ValueError(f"pair indices must lie in [0, {n_rows})")
+    if len(_checked_pairs) > 64:
+        _checked_pairs.clear()
+    _checked_pairs[key] = True
+
+
+class HingeLoss(torch.autograd.Function):
+    """sum over pairs of max(0, margin - (s*_target - s*_decoy)) in one
+    device kernel (adsr_hinge_loss) on the current stream: the per-pair
+    terms and the gradient, -1 to the target and +1 to the decoy of every
+    pair with margin - (s*_t - s*_d) >= 0 (the subgradient torch.clamp
+    uses).  The loss is torch's sum of the terms, bit-identical to
+    ``torch.clamp(margin - (s[t] - s[d]), min=0).sum()``."""
+
+    @staticmethod
+    def forward(ctx, s_star, targets, decoys, margin):
+        if s_star.dtype != torch.float32 or not s_star.is_cuda or s_star.dim() != 1:
+            raise ValueError("s_star must be a 1-D float32 device tensor")
+        s = s_star.detach().contiguous()
+        n = s.numel()
+        t = _pair_indices(targets, s.device)
+        d = _pair_indices(decoys, s.device)
+        _check_pairs(t, d, n)
+        terms = torch.empty(t.numel(), dtype=torch.float32, device=s.device)
+        g = torch.zeros(n, dtype=torch.float32, device=s.device)
+        stream = C.c_void_p(torch.cuda.current_stream(s.device).cuda_stream)
+        check(N.lib.adsr_hinge_loss(_dev_ptr(s), n, _dev_ptr(t), _dev_ptr(d),
+                                    t.numel(), float(margin), _dev_ptr(terms),
+                                    _dev_ptr(g), stream))
+        ctx.save_for_backward(g)
+        return terms.sum()
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return g * grad_out, None, None, None
+
+
 def hinge_loss(s_star: torch.Tensor, targets: torch.Tensor,
                decoys: torch.Tensor, margin: float = 1.0) -> torch.Tensor:
-    """sum over pairs of max(0, margin - (s*_target - s*_decoy)).
-
-    The rows are gathered with index_select, whose backward adds the pairs'
-    gradients with atomics instead of sorting the indices; each precursor's
-    gradient is a sum of -1, 0 and +1 terms, exact in any order."""
-    st = s_star.index_select(0, targets)
-    sd = s_star.index_select(0, decoys)
-    return torch.clamp(margin - (st - sd), min=0.0).sum()
+    """sum over pairs of max(0, margin - (s*_target - s*_decoy)),
+    differentiable with respect to s* (see HingeLoss)."""
+    return HingeLoss.apply(s_star, targets, decoys, margin)

@@ -1714,6 +1714,33 @@ __global__ void fill_f32_kernel(float *p, uint64_t n, float v) {
   if (t < n) p[t] = v;
 }
 
+// Hinge loss over (target, decoy) pairs of rank-0 scores: terms[p] =
+// max(0, margin - (s[t] - s[d])), NaN kept (torch.clamp), and the loss's
+// gradient added to g: -1 to the target and +1 to the decoy of every pair
+// with margin - (s[t] - s[d]) >= 0 (torch.clamp passes the gradient where
+// its input >= min).  The +-1 float atomics are exact in any order while a
+// row's count stays below 2^24.  Pairs with an index outside [0, n) get a
+// zero term and no gradient.
+__global__ __launch_bounds__(256) void hinge_kernel(const float *s, uint64_t n,
+                                                    const int64_t *t, const int64_t *d,
+                                                    uint64_t np, float margin,
+                                                    float *terms, float *g) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += stride) {
+    const int64_t ti = t[p], di = d[p];
+    if (ti < 0 || di < 0 || (uint64_t)ti >= n || (uint64_t)di >= n) {
+      terms[p] = 0.0f;
+      continue;
+    }
+    const float m = margin - (s[ti] - s[di]);
+    terms[p] = m != m ? m : (m > 0.0f ? m : 0.0f);
+    if (m >= 0.0f) {
+      atomicAdd(g + ti, -1.0f);
+      atomicAdd(g + di, 1.0f);
+    }
+  }
+}
+
 __global__ void sstar_kernel(const float *cs, const uint32_t *cnt, uint32_t cc,
                              uint32_t n, float *s_star, uint8_t *has) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3427,6 +3454,24 @@ adsr_status adsr_diff_backward(adsr_diff *h, const float *grad_s,
     HIP_TRY(hipMemcpy(g.data(), s->gsig, d->n_obs * 8, hipMemcpyDeviceToHost));
   std::memcpy(grad_sigma, g.data(), d->n_obs * 8);
   (void)hipEventElapsedTime(&s->bwd_ms, s->ev[2], s->ev[3]);
+  return ADSR_OK;
+}
+
+adsr_status adsr_hinge_loss(const float *s_star, size_t n_rows,
+                            const int64_t *targets, const int64_t *decoys,
+                            size_t n_pairs, float margin, float *terms,
+                            float *grad_s, void *stream) {
+  clear_error();
+  adsr_status st = require_device();
+  if (st) return st;
+  if (n_pairs == 0) return ADSR_OK;
+  if (!s_star || !targets || !decoys || !terms || !grad_s)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  const uint32_t blocks =
+      (uint32_t)std::min<uint64_t>((n_pairs + 255) / 256, 1u << 16);
+  hinge_kernel<<<blocks, 256, 0, (hipStream_t)stream>>>(
+      s_star, n_rows, targets, decoys, n_pairs, margin, terms, grad_s);
+  HIP_TRY(hipGetLastError());
   return ADSR_OK;
 }
 

@@ -241,6 +241,19 @@ adsr_status adsr_diff_forward(adsr_diff *s, const float *sigma,
  * f64[num_observations].                                                  */
 adsr_status adsr_diff_backward(adsr_diff *s, const float *grad_s,
                                float *grad_weights, double *grad_sigma);
+/* Hinge loss over (target, decoy) pairs of s_star (device f32[n_rows];
+ * targets, decoys: device i64[n_pairs]), for training the differentiable
+ * selection; no counterpart in the reference.  terms (device
+ * f32[n_pairs]) receives max(0, margin - (s[t] - s[d])) per pair (the loss
+ * is their sum); grad_s (device f32[n_rows], zeroed by the caller) gets
+ * dLoss/ds: -1 to the target and +1 to the decoy of every pair with
+ * margin - (s[t] - s[d]) >= 0.  Pairs with an index outside [0, n_rows)
+ * are skipped.  Enqueued on `stream` (a hipStream_t; NULL: the null
+ * stream) without synchronizing.                                          */
+adsr_status adsr_hinge_loss(const float *s_star, size_t n_rows,
+                            const int64_t *targets, const int64_t *decoys,
+                            size_t n_pairs, float margin, float *terms,
+                            float *grad_s, void *stream);
 /* Device time of the last forward / backward call (ms). */
 adsr_status adsr_diff_timings(const adsr_diff *s, double *forward_ms,
                               double *backward_ms);

@@ -257,3 +257,47 @@ def test_diff_backward_before_forward_raises(env):
         st.backward(torch.zeros(lib.num_precursors, device="cuda"))
     with pytest.raises(ValueError):
         st.forward(torch.full((3,), 2.0), torch.zeros(lib.num_fragments, device="cuda"))
+
+
+def test_hinge_loss_kernel_matches_torch_ops(env):
+    """adsr_hinge_loss (HingeLoss) against the same loss written as torch
+    operations: index_select, clamp(min=0), sum.  The loss is bit-identical
+    (the same per-pair terms, summed by the same torch reduction) and so is
+    the gradient, including pairs exactly at the kink (margin - diff == 0,
+    where torch.clamp passes the gradient) and precursors in several pairs."""
+    torch = env["torch"]
+    from alphadia_search_rs.autograd import hinge_loss
+    rng = np.random.default_rng(11)
+    n, npairs, margin = 5000, 3000, 1.0
+    s = rng.normal(0.0, 1.0, n).astype(np.float32)
+    t = rng.integers(0, n, npairs)
+    d = rng.integers(0, n, npairs)
+    s[t[:5]] = np.float32(1.5)  # margin - (1.5 - 0.5) == 0 exactly
+    s[d[:5]] = np.float32(0.5)
+    d[5:9] = d[0]               # repeated decoys
+    s_dev = torch.tensor(s, device="cuda")
+    t_dev = torch.tensor(t, device="cuda")
+    d_dev = torch.tensor(d, device="cuda")
+
+    a = s_dev.clone().requires_grad_(True)
+    got = hinge_loss(a, t_dev, d_dev, margin)
+    got.backward()
+    b = s_dev.clone().requires_grad_(True)
+    want = torch.clamp(margin - (b.index_select(0, t_dev) - b.index_select(0, d_dev)),
+                       min=0.0).sum()
+    want.backward()
+    m = np.float32(margin) - (s[t] - s[d])
+    assert (m == 0).sum() >= 5 and (m > 0).sum() > 100 and (m < 0).sum() > 100
+    assert got.detach().cpu().numpy().view(np.uint32) == want.detach().cpu().numpy().view(np.uint32)
+    ga, gb = a.grad.cpu().numpy(), b.grad.cpu().numpy()
+    np.testing.assert_array_equal(ga.view(np.uint32), gb.view(np.uint32))
+
+    # a second call with a scaled output gradient, and an empty pair list
+    a.grad = None
+    (2.5 * hinge_loss(a, t_dev, d_dev, margin)).backward()
+    np.testing.assert_array_equal(a.grad.cpu().numpy(), 2.5 * gb)
+    e = torch.empty(0, dtype=torch.int64, device="cuda")
+    assert float(hinge_loss(s_dev, e, e, margin)) == 0.0
+
+    with pytest.raises(ValueError):
+        hinge_loss(s_dev, t_dev, torch.full_like(d_dev, n), margin)
