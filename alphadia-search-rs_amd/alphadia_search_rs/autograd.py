@@ -77,20 +77,24 @@ class SelectionGradState:
                 f"weights must be a float32 device tensor with "
                 f"{self.num_fragments} values")
 
-    def forward(self, sigma: torch.Tensor, weights: torch.Tensor):
-        """Return (s_star f32[P], has_candidate bool[P]) on the device."""
+    def forward(self, sigma: torch.Tensor, weights: torch.Tensor,
+                with_has: bool = True):
+        """Return (s_star f32[P], has_candidate bool[P] or None when
+        ``with_has`` is false) on the device."""
         self._check_inputs(sigma, weights)
         w = weights.detach().contiguous()
         sig = np.ascontiguousarray(
             sigma.detach().to("cpu", torch.float32).numpy().reshape(-1))
         dev = w.device
         s_star = torch.empty(self.num_precursors, dtype=torch.float32, device=dev)
-        has = torch.empty(self.num_precursors, dtype=torch.uint8, device=dev)
+        has = (torch.empty(self.num_precursors, dtype=torch.uint8, device=dev)
+               if with_has else None)
         torch.cuda.synchronize(dev)  # the library runs on its own stream
         check(N.lib.adsr_diff_forward(self._h, N.ptr(sig), _dev_ptr(w),
-                                      _dev_ptr(s_star), _dev_ptr(has)))
+                                      _dev_ptr(s_star),
+                                      _dev_ptr(has) if has is not None else None))
         self._fwd_done = True
-        return s_star, has.bool()
+        return s_star, (has.bool() if has is not None else None)
 
     def backward(self, grad_s: torch.Tensor):
         """Return (grad_weights f32[n_fragments] on the device,
@@ -124,7 +128,7 @@ class RankZeroScore(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, state: SelectionGradState, sigma, weights):
-        s_star, _ = state.forward(sigma, weights)
+        s_star, _ = state.forward(sigma, weights, with_has=False)
         ctx.state = state
         ctx.sigma_meta = (sigma.dtype, sigma.device)
         return s_star
