@@ -129,6 +129,7 @@ _sig("adsr_diff_forward", I, vp, vp, vp, vp, vp)
 _sig("adsr_diff_backward", I, vp, vp, vp, vp)
 _sig("adsr_diff_timings", I, vp, C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("adsr_hinge_loss", I, vp, c_size, vp, vp, c_size, c_f32, vp, vp, vp)
+_sig("adsr_diff_forward_async", I, vp, vp, vp, vp, vp, vp)
 _sig("adsr_diff_free", None, vp)
 _sig("adsr_candidates_len", c_size, vp)
 _sig("adsr_candidates_export", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)

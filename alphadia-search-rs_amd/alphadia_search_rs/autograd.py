@@ -78,9 +78,13 @@ class SelectionGradState:
                 f"{self.num_fragments} values")
 
     def forward(self, sigma: torch.Tensor, weights: torch.Tensor,
-                with_has: bool = True):
+                with_has: bool = True, asynchronous: bool = False):
         """Return (s_star f32[P], has_candidate bool[P] or None when
-        ``with_has`` is false) on the device."""
+        ``with_has`` is false) on the device.
+
+        ``asynchronous``: the forward is queued on torch's current stream
+        and the call returns without waiting for it; device status errors
+        are then raised by the next backward (or forward, or timings)."""
         self._check_inputs(sigma, weights)
         w = weights.detach().contiguous()
         sig = np.ascontiguousarray(
@@ -89,10 +93,15 @@ class SelectionGradState:
         s_star = torch.empty(self.num_precursors, dtype=torch.float32, device=dev)
         has = (torch.empty(self.num_precursors, dtype=torch.uint8, device=dev)
                if with_has else None)
-        torch.cuda.synchronize(dev)  # the library runs on its own stream
-        check(N.lib.adsr_diff_forward(self._h, N.ptr(sig), _dev_ptr(w),
-                                      _dev_ptr(s_star),
-                                      _dev_ptr(has) if has is not None else None))
+        has_p = _dev_ptr(has) if has is not None else None
+        if asynchronous:
+            stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+            check(N.lib.adsr_diff_forward_async(self._h, N.ptr(sig), _dev_ptr(w),
+                                                _dev_ptr(s_star), has_p, stream))
+        else:
+            torch.cuda.synchronize(dev)  # the library runs on its own stream
+            check(N.lib.adsr_diff_forward(self._h, N.ptr(sig), _dev_ptr(w),
+                                          _dev_ptr(s_star), has_p))
         self._fwd_done = True
         return s_star, (has.bool() if has is not None else None)
 
@@ -128,7 +137,7 @@ class RankZeroScore(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, state: SelectionGradState, sigma, weights):
-        s_star, _ = state.forward(sigma, weights, with_has=False)
+        s_star, _ = state.forward(sigma, weights, with_has=False, asynchronous=True)
         ctx.state = state
         ctx.sigma_meta = (sigma.dtype, sigma.device)
         return s_star

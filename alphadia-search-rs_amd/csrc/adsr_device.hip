@@ -3269,11 +3269,29 @@ struct DiffState {
   float fwd_ms = 0.0f, bwd_ms = 0.0f;
   bool have_forward = false;
   // host copies of the taps and their sigma derivatives, uploaded on the
-  // workspace stream; alive until the next forward, which starts after the
-  // previous one has synchronized
+  // forward's stream; alive until the next forward, which starts after the
+  // previous one has been waited for
   std::vector<float> taps_h;
   std::vector<double> dk_h;
+  // a forward enqueued by adsr_diff_forward_async whose device status has
+  // not been read yet, and the caller's stream it runs on
+  bool pending = false;
+  hipStream_t pending_stream = nullptr;
 };
+
+// Waits for a pending asynchronous forward and reports its device status.
+static adsr_status diff_settle(DiffState *s) {
+  if (!s->pending) return ADSR_OK;
+  s->pending = false;
+  HIP_TRY(hipStreamSynchronize(s->pending_stream));
+  const adsr_status st = check_status(s->w);
+  if (st) {
+    s->have_forward = false;
+    return st;
+  }
+  (void)hipEventElapsedTime(&s->fwd_ms, s->ev[0], s->ev[1]);
+  return ADSR_OK;
+}
 
 // d k[j] / d sigma for the taps j = 0..half of gaussian_kernel_1d
 // (src/kernel/mod.rs:32-56): k = e / sum(e), e_j = exp(-x_j^2 / (2 sigma^2)),
@@ -3357,13 +3375,16 @@ adsr_status adsr_diff_create(const adsr_dia *dc, const adsr_lib *lc,
   return ADSR_OK;
 }
 
-adsr_status adsr_diff_forward(adsr_diff *h, const float *sigma,
-                              const float *weights, float *s_star,
-                              uint8_t *has) {
+// The forward on the state's own stream (async false: waits for the device
+// and reads the status) or enqueued on the caller's stream (async true).
+static adsr_status diff_forward(DiffState *s, const float *sigma,
+                                const float *weights, float *s_star,
+                                uint8_t *has, bool async, hipStream_t stream) {
   clear_error();
-  auto *s = reinterpret_cast<DiffState *>(h);
   if (!s || !sigma || !weights || !s_star)
     return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  adsr_status pst = diff_settle(s);
+  if (pst) return pst;
   const adsr_dia *d = s->d;
   const adsr_lib *l = s->l;
   const size_t klen = s->sp.prm.klen, half = s->sp.prm.half;
@@ -3383,26 +3404,62 @@ adsr_status adsr_diff_forward(adsr_diff *h, const float *sigma,
   kernel_dsigma((double)(s->p.fwhm_rt / 2.3548f), klen, dk.data());
   for (size_t o = 0; o < d->n_obs; ++o)
     kernel_dsigma((double)s->sig[o], klen, dk.data() + (1 + o) * (half + 1));
-  HIP_TRY(hipMemcpyAsync(s->w.kernels, taps.data(), taps.size() * 4, hipMemcpyHostToDevice,
-                         s->w.stream));
-  HIP_TRY(hipMemcpyAsync(s->dk, dk.data(), dk.size() * 8, hipMemcpyHostToDevice,
-                         s->w.stream));
-  s->sp.lib.f_int = weights;
-  LaunchShape shape{4, 2};
-  const uint32_t n = (uint32_t)l->n_prec;
-  HIP_TRY(hipEventRecord(s->ev[0], s->w.stream));
-  adsr_status st = enqueue(s->w, s->sp, n, shape, nullptr, 0, 0, 0, false);
-  if (st) return st;
-  if (n)
-    sstar_kernel<<<(n + 255) / 256, 256, 0, s->w.stream>>>(
-        s->w.cand_score, s->w.cand_count, (uint32_t)s->sp.prm.cand_count, n,
-        s_star, has);
-  HIP_TRY(hipEventRecord(s->ev[1], s->w.stream));
-  st = check_status(s->w);
-  if (st) return st;
-  (void)hipEventElapsedTime(&s->fwd_ms, s->ev[0], s->ev[1]);
+  // the workspace's launches go to the caller's stream for an asynchronous
+  // forward; its own stream is put back before returning
+  const hipStream_t own = s->w.stream;
+  const hipStream_t fs = async ? stream : own;
+  s->w.stream = fs;
+  adsr_status st = [&]() -> adsr_status {
+    HIP_TRY(hipMemcpyAsync(s->w.kernels, taps.data(), taps.size() * 4,
+                           hipMemcpyHostToDevice, fs));
+    HIP_TRY(hipMemcpyAsync(s->dk, dk.data(), dk.size() * 8, hipMemcpyHostToDevice, fs));
+    s->sp.lib.f_int = weights;
+    LaunchShape shape{4, 2};
+    const uint32_t n = (uint32_t)l->n_prec;
+    HIP_TRY(hipEventRecord(s->ev[0], fs));
+    adsr_status e = enqueue(s->w, s->sp, n, shape, nullptr, 0, 0, 0, false);
+    if (e) return e;
+    if (n)
+      sstar_kernel<<<(n + 255) / 256, 256, 0, fs>>>(
+          s->w.cand_score, s->w.cand_count, (uint32_t)s->sp.prm.cand_count, n,
+          s_star, has);
+    HIP_TRY(hipEventRecord(s->ev[1], fs));
+    return ADSR_OK;
+  }();
+  s->w.stream = own;
+  if (st) {
+    // work may have been queued: wait for it before the state is reused
+    (void)hipStreamSynchronize(fs);
+    s->have_forward = false;
+    return st;
+  }
   s->have_forward = true;
+  if (async) {
+    s->pending = true;
+    s->pending_stream = fs;
+    return ADSR_OK;
+  }
+  st = check_status(s->w);
+  if (st) {
+    s->have_forward = false;
+    return st;
+  }
+  (void)hipEventElapsedTime(&s->fwd_ms, s->ev[0], s->ev[1]);
   return ADSR_OK;
+}
+
+adsr_status adsr_diff_forward(adsr_diff *h, const float *sigma,
+                              const float *weights, float *s_star,
+                              uint8_t *has) {
+  return diff_forward(reinterpret_cast<DiffState *>(h), sigma, weights, s_star,
+                      has, false, nullptr);
+}
+
+adsr_status adsr_diff_forward_async(adsr_diff *h, const float *sigma,
+                                    const float *weights, float *s_star,
+                                    uint8_t *has, void *stream) {
+  return diff_forward(reinterpret_cast<DiffState *>(h), sigma, weights, s_star,
+                      has, true, (hipStream_t)stream);
 }
 
 adsr_status adsr_diff_backward(adsr_diff *h, const float *grad_s,
@@ -3411,6 +3468,8 @@ adsr_status adsr_diff_backward(adsr_diff *h, const float *grad_s,
   auto *s = reinterpret_cast<DiffState *>(h);
   if (!s || !grad_s || !grad_w || !grad_sigma)
     return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  adsr_status pst = diff_settle(s);
+  if (pst) return pst;
   if (!s->have_forward)
     return set_error(ADSR_ERR_INVALID_ARGUMENT, "backward before forward");
   const adsr_dia *d = s->d;
@@ -3476,8 +3535,11 @@ adsr_status adsr_hinge_loss(const float *s_star, size_t n_rows,
 }
 
 adsr_status adsr_diff_timings(const adsr_diff *h, double *fwd, double *bwd) {
-  const auto *s = reinterpret_cast<const DiffState *>(h);
+  // a pending forward is waited for (the state itself is not const)
+  auto *s = reinterpret_cast<DiffState *>(const_cast<adsr_diff *>(h));
   if (!s) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
+  adsr_status pst = diff_settle(s);
+  if (pst) return pst;
   if (fwd) *fwd = s->fwd_ms;
   if (bwd) *bwd = s->bwd_ms;
   return ADSR_OK;
@@ -3486,6 +3548,8 @@ adsr_status adsr_diff_timings(const adsr_diff *h, double *fwd, double *bwd) {
 void adsr_diff_free(adsr_diff *h) {
   auto *s = reinterpret_cast<DiffState *>(h);
   if (!s) return;
+  // kernels of an asynchronous forward may still use the workspace
+  if (s->pending) (void)hipStreamSynchronize(s->pending_stream);
   ws_free(s->w);
   (void)hipFree(s->dk);
   (void)hipFree(s->contrib);

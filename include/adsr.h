@@ -236,6 +236,17 @@ adsr_status adsr_diff_create(const adsr_dia *d, const adsr_lib *l,
 adsr_status adsr_diff_forward(adsr_diff *s, const float *sigma,
                               const float *weights, float *s_star,
                               uint8_t *has_candidate);
+/* adsr_diff_forward enqueued on `stream` (a hipStream_t; NULL: the null
+ * stream) without waiting for the device, so that the caller can queue
+ * the loss and its gradient behind it.  Argument errors are returned at
+ * once; the status errors adsr_diff_forward reports from the device are
+ * returned by the next adsr_diff_forward(_async), adsr_diff_backward or
+ * adsr_diff_timings call on the state, which waits for this forward
+ * (adsr_diff_free waits too).  weights must stay valid until the forward
+ * has run on `stream`.  No counterpart in the reference.                  */
+adsr_status adsr_diff_forward_async(adsr_diff *s, const float *sigma,
+                                    const float *weights, float *s_star,
+                                    uint8_t *has_candidate, void *stream);
 /* grad_s: device f32[num_precursors] = dLoss/ds_star; outputs: device
  * f32[num_fragments] (fragments that were not selected get 0) and host
  * f64[num_observations].                                                  */

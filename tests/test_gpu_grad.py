@@ -301,3 +301,37 @@ def test_hinge_loss_kernel_matches_torch_ops(env):
 
     with pytest.raises(ValueError):
         hinge_loss(s_dev, t_dev, torch.full_like(d_dev, n), margin)
+
+
+def test_async_forward_matches_sync_and_defers_status(env):
+    """adsr_diff_forward_async (the forward rank_zero_scores uses) queued on
+    torch's stream gives the synchronous forward's s* and has-candidate
+    bits; a device status error of an asynchronous forward (a precursor
+    inside more than four isolation windows) is raised by the next call
+    that waits for it, while the synchronous forward raises it at once."""
+    torch, ads, gen = env["torch"], env["ads"], env["gen"]
+    import alphadia_search_rs._native as N
+    from alphadia_search_rs.autograd import SelectionGradState
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    st = SelectionGradState(d, lib, _params(gen))
+    sig = torch.tensor(env["sigma"])
+    w = torch.tensor(env["weights"], device="cuda")
+    s_sync, h_sync = st.forward(sig, w)
+    s_async, h_async = st.forward(sig, w, asynchronous=True)
+    torch.cuda.synchronize()
+    a, b = s_sync.cpu().numpy(), s_async.cpu().numpy()
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert torch.equal(h_sync, h_async)
+    assert st.timings()[0] > 0.0
+
+    dia = list(gen["dia"])
+    dia[1] = np.full_like(dia[1], 400.0)   # every window covers 400-1000
+    dia[2] = np.full_like(dia[2], 1000.0)
+    d2 = ads.DIAData.from_arrays(*dia)
+    st2 = SelectionGradState(d2, lib, _params(gen))
+    with pytest.raises(N.AdsrError, match="isolation windows"):
+        st2.forward(sig, w)
+    st2.forward(sig, w, asynchronous=True)  # queued; the status is read later
+    with pytest.raises(N.AdsrError, match="isolation windows"):
+        st2.backward(torch.zeros(lib.num_precursors, device="cuda"))
