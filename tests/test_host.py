@@ -279,6 +279,23 @@ def test_diff_rejects_shared_fragment_ranges_and_needs_gpu(run0):
         SelectionGradState(d, ok, ads.SelectionParameters())
 
 
+def test_hinge_loss_and_async_forward_argument_errors():
+    """The hinge loss kernel's entry point needs the device (no host
+    fallback) and the Python loss refuses host tensors; the asynchronous
+    forward rejects a NULL state before touching any device."""
+    import torch
+    import alphadia_search_rs._native as N
+    from alphadia_search_rs.autograd import hinge_loss
+    st = N.lib.adsr_diff_forward_async(None, None, None, None, None, None)
+    assert st == N.ADSR_ERR_INVALID_ARGUMENT
+    if not N.device_available():
+        st = N.lib.adsr_hinge_loss(None, 0, None, None, 1, 1.0, None, None, None)
+        assert st == N.ADSR_ERR_DEVICE
+        assert "gfx950" in N.lib.adsr_last_error().decode()
+    with pytest.raises(ValueError, match="device tensor"):
+        hinge_loss(torch.zeros(4), torch.tensor([0]), torch.tensor([1]))
+
+
 def test_candidate_parquet_round_trip(tmp_path):
     """The file hand-off (scripts/candidate_selection.py:121-209 writes the
     to_arrays columns to parquet): columns, dtypes and values survive a
