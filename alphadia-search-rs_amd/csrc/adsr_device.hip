@@ -854,6 +854,20 @@ __device__ __forceinline__ void argmax_dpp(float &bv, uint32_t &bi) {
   }
 }
 
+// Inclusive prefix maximum over the wave's lanes through DPP: row_shr 1, 2,
+// 4, 8 within each row of 16 lanes, then row_bcast 15 and 31 carry rows
+// 0..2 forward.  Lanes without a source keep their own value (max is
+// idempotent).
+__device__ __forceinline__ uint32_t wave_incl_max_dpp(uint32_t x) {
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x111, 0xf, 0xf, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x112, 0xf, 0xf, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x114, 0xf, 0xf, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x118, 0xf, 0xf, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x142, 0xa, 0xf, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x143, 0xc, 0xf, false));
+  return x;
+}
+
 // Wave-wide arg-max: row_shr 1, 2, 4, 8 reduce each row of 16 lanes into
 // its lane 15; row_bcast 15 and 31 carry rows 0..2 into lane 63.  The
 // result is returned in every lane.
@@ -1162,6 +1176,16 @@ void select_kernel(SelArgs a) {
           const uint32_t gincl = wave_incl_scan(ng);
           const uint32_t gp = gincl - ng;  // first virtual group
           const uint32_t NG = rdlane(gincl, FN - 1);
+          // The item list's LDS is unused during extraction: slots 0-63
+          // hold the tile's fragment-start markers, then each fragment's
+          // first group, packed bounds and first bin, indexed by fragment.
+          uint32_t *const mk = reinterpret_cast<uint32_t *>(l.items);
+          uint32_t *const ft = mk + kWave;
+          if (lane < (int)FN) {
+            ft[lane] = gsr;
+            ft[kWave + lane] = meta;
+            ft[2 * kWave + lane] = rb0;
+          }
           for (uint32_t vb = 0; vb < NG; vb += kWave * kGroupTiles) {
             uint4 raw[kGroupTiles];
             uint32_t tg[kGroupTiles], tfr[kGroupTiles], tvm[kGroupTiles],
@@ -1178,23 +1202,24 @@ void select_kernel(SelArgs a) {
               if (v0 < NG) {
                 const uint32_t vmax = min(v0 + (uint32_t)kWave - 1, NG - 1);
                 const uint64_t lo_m = ballot64(lane < (int)FN && ng > 0 && gp <= v0);
-                const uint64_t hi_m = ballot64(lane < (int)FN && ng > 0 && gp <= vmax);
                 const uint32_t flo = 63u - (uint32_t)__builtin_clzll(lo_m);
-                const uint32_t fhi = 63u - (uint32_t)__builtin_clzll(hi_m);
-                // fragment of this lane's group: the last one starting at
-                // or before it (empty fragments share the next one's start)
-                uint32_t f = flo, fgp = rdlane(gp, flo), fgs = rdlane(gsr, flo);
-                uint32_t fmeta = rdlane(meta, flo), fb0 = rdlane(rb0, flo);
-                for (uint32_t ff = flo + 1; ff <= fhi; ++ff) {
-                  const uint32_t pp = rdlane(gp, ff);
-                  if (pp <= v) {
-                    f = ff;
-                    fgp = pp;
-                    fgs = rdlane(gsr, ff);
-                    fmeta = rdlane(meta, ff);
-                    fb0 = rdlane(rb0, ff);
-                  }
-                }
+                // Fragment of this lane's group: the last non-empty one
+                // starting at or before it (empty fragments share the next
+                // one's start).  Non-empty fragments start at distinct
+                // groups; those starting inside the tile mark 1 + their
+                // index at their start's slot, and each lane takes the
+                // marked slot at or below it (a prefix maximum; lane 0's
+                // slot, never marked, holds 1 + flo).
+                mk[lane] = lane == 0 ? flo + 1u : 0u;
+                __builtin_amdgcn_wave_barrier();
+                if (lane < (int)FN && ng > 0 && gp > v0 && gp <= vmax) mk[gp - v0] = (uint32_t)lane + 1u;
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t f = wave_incl_max_dpp(mk[lane]) - 1u;
+                const uint32_t fgp = (uint32_t)__shfl((int)gp, (int)f, kWave);
+                const uint32_t fgs = ft[f];
+                const uint32_t fmeta = ft[kWave + f];
+                const uint32_t fb0 = ft[2 * kWave + f];
+                __builtin_amdgcn_wave_barrier();
                 if (v < NG) {
                   const uint32_t q = v - fgp;  // group within the fragment
                   tg[u] = fgs + q;
