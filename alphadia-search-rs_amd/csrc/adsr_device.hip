@@ -1796,7 +1796,7 @@ __device__ __forceinline__ void bin_range(const KDia &d, float mz, float tol,
 // Per-wave LDS of the backward: XIC window per selected fragment, its
 // duplicate detector, the hit list and the per-fragment sigma terms.
 __host__ __device__ inline uint32_t bwd_lds_bytes(uint32_t ks) {
-  return (ks * kWin * 4 + ks * kWin + kBwdHitCap * 8 + ks * 8 + 15) / 16 * 16;
+  return (ks * kWin * 4 + ks * 8 + kBwdHitCap * 8 + kWave * 4 + ks * kWin + 15) / 16 * 16;
 }
 
 // One wave per precursor (launched over all rows; rows without a candidate
@@ -1810,8 +1810,11 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
   float *xw = reinterpret_cast<float *>(base_lds);
   double *part = reinterpret_cast<double *>(base_lds + ks * kWin * 4);
   uint2 *hits = reinterpret_cast<uint2 *>(base_lds + ks * kWin * 4 + ks * 8);
+  // fragment-start markers of a tile (see the select kernel's extraction)
+  uint32_t *mk = reinterpret_cast<uint32_t *>(base_lds + ks * kWin * 4 + ks * 8 +
+                                              kBwdHitCap * 8);
   uint8_t *own = reinterpret_cast<uint8_t *>(base_lds + ks * kWin * 4 + ks * 8 +
-                                             kBwdHitCap * 8);
+                                             kBwdHitCap * 8 + kWave * 4);
   if (row >= a.n_rows) return;  // whole wave
   const uint32_t half = a.prm.half;
   // The row's loads are issued in three rounds of independent loads: what
@@ -1909,21 +1912,19 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
       const uint32_t v = v0 + lane;
       const uint32_t vmax = min(v0 + (uint32_t)kWave - 1, NG - 1);
       const uint64_t lo_m = ballot64(lane < (int)nsel && ng > 0 && gp <= v0);
-      const uint64_t hi_m = ballot64(lane < (int)nsel && ng > 0 && gp <= vmax);
       const uint32_t flo = 63u - (uint32_t)__builtin_clzll(lo_m);
-      const uint32_t fhi = 63u - (uint32_t)__builtin_clzll(hi_m);
-      uint32_t f = flo, fgp = rdlane(gp, flo), fgs = rdlane(gsr, flo);
-      uint32_t fmeta = rdlane(meta, flo), fb0 = rdlane(bst, flo);
-      for (uint32_t ff = flo + 1; ff <= fhi; ++ff) {
-        const uint32_t pp = rdlane(gp, ff);
-        if (pp <= v) {
-          f = ff;
-          fgp = pp;
-          fgs = rdlane(gsr, ff);
-          fmeta = rdlane(meta, ff);
-          fb0 = rdlane(bst, ff);
-        }
-      }
+      // fragment of this lane's group: the last non-empty one starting at
+      // or before it, by start markers and a prefix maximum
+      mk[lane] = lane == 0 ? flo + 1u : 0u;
+      __builtin_amdgcn_wave_barrier();
+      if (lane < (int)nsel && ng > 0 && gp > v0 && gp <= vmax) mk[gp - v0] = (uint32_t)lane + 1u;
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t f = wave_incl_max_dpp(mk[lane]) - 1u;
+      const uint32_t fgp = (uint32_t)__shfl((int)gp, (int)f, kWave);
+      const uint32_t fgs = (uint32_t)__shfl((int)gsr, (int)f, kWave);
+      const uint32_t fmeta = (uint32_t)__shfl((int)meta, (int)f, kWave);
+      const uint32_t fb0 = (uint32_t)__shfl((int)bst, (int)f, kWave);
+      __builtin_amdgcn_wave_barrier();
       uint4 raw = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
       uint32_t tg = 0, vm = 0;
       if (v < NG) {
