@@ -19,7 +19,10 @@ Spectra are cycle-major: spectrum (c, w) has index c*W + w, delta_scan_idx w,
 cycle_idx c and rt = c, so the RT index is 0..C-1 with delta_t = 1 and
 tolerances in seconds equal tolerances in cycles.  Each spectrum's peaks are
 sorted by m/z after planted elution peaks are inserted.  Every random stream
-is numpy.random.default_rng(SeedSequence([config_id, stream_id])).
+is numpy.random.default_rng(SeedSequence([config_id, stream_id])); the peaks
+of chunk q of cycles draw from SeedSequence([config_id, stream_id, q]), so
+chunks are generated on threads and the arrays do not depend on the thread
+count.
 """
 from __future__ import annotations
 
@@ -97,7 +100,16 @@ def _rng(cid: int, stream: int) -> np.random.Generator:
     return np.random.default_rng(np.random.SeedSequence([cid, stream]))
 
 
-def generate(spec: RunSpec, progress=None) -> dict:
+def _cpus() -> int:
+    try:
+        import os
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        import os
+        return os.cpu_count() or 1
+
+
+def generate(spec: RunSpec, progress=None, threads: int | None = None) -> dict:
     """Returns {'dia': (10 arrays), 'lib': (17 arrays), ...metadata}."""
     cid = spec.config_id
     C, W = spec.n_cycles, spec.n_windows
@@ -169,6 +181,10 @@ def generate(spec: RunSpec, progress=None) -> dict:
     )
 
     # ---------------- planted elution profiles ----------------
+    # Each planted target gets, at every cycle within 4 sigma of its RT
+    # centre, one peak per fragment in its first isolation window:
+    # m/z = fragment m/z * (1 + U(-3, 3) ppm), intensity = amp * library
+    # intensity * exp(-(cycle - centre)^2 / (2 sigma^2)).
     rp = _rng(cid, 9)
     n_plant = int(round(spec.planted_fraction * P_t))
     planted = np.sort(rp.choice(P_t, n_plant, replace=False)) if n_plant else \
@@ -176,38 +192,35 @@ def generate(spec: RunSpec, progress=None) -> dict:
     amp = rp.lognormal(8.0, 1.0, n_plant)
     span = int(np.floor(4 * spec.plant_sigma))
     offs = np.arange(-span, span + 1)
-    # window of each planted precursor (first window containing it)
     pw = np.searchsorted(win_hi, pmz[planted], side="left")
     pw = np.minimum(pw, W - 1)
-    pl_spec, pl_mz, pl_int = [], [], []
-    jit = _rng(cid, 10)
-    for k, t in enumerate(planted):
-        c_center = float(prt[t])
-        cyc = np.floor(c_center).astype(np.int64) + offs
-        cyc = cyc[(cyc >= 0) & (cyc < C) & (np.abs(cyc - c_center) <= 4 * spec.plant_sigma)]
-        if cyc.size == 0:
-            continue
-        a, b = int(fstart_t[t]), int(fstart_t[t]) + int(nf[t])
-        fm = fmz_t[a:b].astype(np.float64)
-        fi = fint_t[a:b].astype(np.float64)
-        prof = np.exp(-((cyc - c_center) ** 2) / (2 * spec.plant_sigma ** 2))
-        mzs = fm[None, :] * (1.0 + jit.uniform(-3e-6, 3e-6, (cyc.size, fm.size)))
-        ints = amp[k] * fi[None, :] * prof[:, None]
-        pl_spec.append(np.repeat(cyc * W + int(pw[k]), fm.size))
-        pl_mz.append(mzs.reshape(-1))
-        pl_int.append(ints.reshape(-1))
-    if pl_spec:
-        pl_spec = np.concatenate(pl_spec)
-        pl_mz = np.concatenate(pl_mz).astype(np.float32)
-        pl_int = np.concatenate(pl_int).astype(np.float32)
-        order = np.argsort(pl_spec, kind="stable")
-        pl_spec, pl_mz, pl_int = pl_spec[order], pl_mz[order], pl_int[order]
-    else:
-        pl_spec = np.zeros(0, dtype=np.int64)
-        pl_mz = np.zeros(0, dtype=np.float32)
-        pl_int = np.zeros(0, dtype=np.float32)
+    centre = prt[planted].astype(np.float64)
+    cyc = np.floor(centre).astype(np.int64)[:, None] + offs[None, :]
+    ok = (cyc >= 0) & (cyc < C) & (np.abs(cyc - centre[:, None]) <= 4 * spec.plant_sigma)
+    kk, oo = np.nonzero(ok)                      # (planted k, cycle) pairs
+    pc = cyc[kk, oo]
+    prof = np.exp(-((pc - centre[kk]) ** 2) / (2 * spec.plant_sigma ** 2))
+    nfk = nf[planted][kk]                        # fragments of each pair
+    pair = np.repeat(np.arange(kk.size), nfk)
+    fofs = np.arange(int(nfk.sum())) - np.repeat(np.cumsum(nfk) - nfk, nfk)
+    fidx = fstart_t[planted][kk][pair].astype(np.int64) + fofs
+    jit = _rng(cid, 10).uniform(-3e-6, 3e-6, fidx.size)
+    pl_cyc = pc[pair]
+    pl_spec = pl_cyc * W + pw[kk][pair]
+    pl_mz = (fmz_t[fidx].astype(np.float64) * (1.0 + jit)).astype(np.float32)
+    pl_int = (amp[kk][pair] * fint_t[fidx].astype(np.float64) * prof[pair]).astype(np.float32)
+    # grouped by cycle here; sorted by (spectrum, m/z) per chunk below
+    order = np.argsort(pl_cyc, kind="stable")
+    pl_cyc, pl_spec = pl_cyc[order], pl_spec[order]
+    pl_mz, pl_int = pl_mz[order], pl_int[order]
 
-    # ---------------- peaks, chunked over cycles ----------------
+    # ---------------- peaks, in chunks of cycles on threads ----------------
+    # Chunk q of cycles draws from its own streams (SeedSequence([cid, 11,
+    # q]) for m/z, [cid, 12, q] for intensity), so chunks are independent
+    # and the result does not depend on the thread count.  Background m/z is
+    # sorted per spectrum; intensities are i.i.d. and independent of m/z, so
+    # they stay in draw order.  Planted peaks are merged into the sorted
+    # rows (after background peaks of equal m/z).
     n_spec = C * W
     K = spec.peaks_per_spectrum
     pl_count = np.bincount(pl_spec, minlength=n_spec).astype(np.int64)
@@ -217,40 +230,54 @@ def generate(spec: RunSpec, progress=None) -> dict:
     total = int(stop[-1]) if n_spec else 0
     peak_mz = np.empty(total, dtype=np.float32)
     peak_int = np.empty(total, dtype=np.float32)
-    rmz = _rng(cid, 11)
-    rin = _rng(cid, 12)
-    pl_start = np.concatenate([[0], np.cumsum(pl_count)])
-    chunk = max(1, 4_000_000 // max(K * W, 1))
-    for c0 in range(0, C, chunk):
-        c1 = min(C, c0 + chunk)
+    chunk = max(1, 2_000_000 // max(K * W, 1))
+    chunks = [(q, c0, min(C, c0 + chunk)) for q, c0 in enumerate(range(0, C, chunk))]
+    assert not pl_mz.size or float(pl_mz.max()) < 4096.0
+
+    def fill(job):
+        q, c0, c1 = job
         s0, s1 = c0 * W, c1 * W
         ns = s1 - s0
-        extra = int(pl_count[s0:s1].max()) if ns else 0
-        base_mz = rmz.uniform(150.0, 2000.0, (ns, K)).astype(np.float32)
-        base_in = rin.lognormal(6.0, 1.5, (ns, K)).astype(np.float32)
-        if extra:
-            mz2 = np.full((ns, K + extra), np.inf, dtype=np.float32)
-            in2 = np.zeros((ns, K + extra), dtype=np.float32)
-            mz2[:, :K] = base_mz
-            in2[:, :K] = base_in
-            a, b = pl_start[s0], pl_start[s1]
-            rows = pl_spec[a:b] - s0
-            # column of each planted peak within its spectrum
-            first = np.searchsorted(rows, np.arange(ns), side="left")
-            col = K + (np.arange(b - a) - first[rows])
-            mz2[rows, col] = pl_mz[a:b]
-            in2[rows, col] = pl_int[a:b]
-        else:
-            mz2, in2 = base_mz, base_in
-        order = np.argsort(mz2, axis=1, kind="stable")
-        mz_s = np.take_along_axis(mz2, order, axis=1)
-        in_s = np.take_along_axis(in2, order, axis=1)
-        keep = np.isfinite(mz_s)
+        rmz = np.random.default_rng(np.random.SeedSequence([cid, 11, q]))
+        rin = np.random.default_rng(np.random.SeedSequence([cid, 12, q]))
+        base_mz = rmz.random((ns, K), dtype=np.float32)
+        base_mz *= np.float32(1850.0)
+        base_mz += np.float32(150.0)
+        base_mz.sort(axis=1)
+        base_in = rin.standard_normal((ns, K), dtype=np.float32)
+        base_in *= np.float32(1.5)
+        base_in += np.float32(6.0)
+        np.exp(base_in, out=base_in)
         o0, o1 = int(start[s0]), int(stop[s1 - 1])
-        peak_mz[o0:o1] = mz_s[keep]
-        peak_int[o0:o1] = in_s[keep]
-        if progress:
-            progress(c1, C)
+        a, b = np.searchsorted(pl_cyc, [c0, c1], side="left").tolist()
+        if b == a:
+            peak_mz[o0:o1] = base_mz.reshape(-1)
+            peak_int[o0:o1] = base_in.reshape(-1)
+        else:
+            # merge by the exact f64 key (row * 4096 + m/z)
+            pkey = (pl_spec[a:b] - s0).astype(np.float64) * 4096.0 + pl_mz[a:b]
+            po = np.argsort(pkey, kind="stable")
+            pkey = pkey[po]
+            bkey = (np.arange(ns, dtype=np.float64)[:, None] * 4096.0
+                    + base_mz.astype(np.float64)).reshape(-1)
+            pos = np.searchsorted(bkey, pkey, side="right") + np.arange(b - a)
+            is_pl = np.zeros(o1 - o0, dtype=bool)
+            is_pl[pos] = True
+            seg_mz = peak_mz[o0:o1]
+            seg_in = peak_int[o0:o1]
+            seg_mz[pos] = pl_mz[a:b][po]
+            seg_in[pos] = pl_int[a:b][po]
+            seg_mz[~is_pl] = base_mz.reshape(-1)
+            seg_in[~is_pl] = base_in.reshape(-1)
+        return c1
+
+    if chunks:
+        import concurrent.futures as cf
+        nthr = max(1, min(threads or _cpus(), 16, len(chunks)))
+        with cf.ThreadPoolExecutor(nthr) as ex:
+            for c1 in ex.map(fill, chunks):
+                if progress:
+                    progress(c1, C)
     spec_idx = np.arange(n_spec)
     dia = (
         (spec_idx % W).astype(np.int64),                  # delta_scan_idx
