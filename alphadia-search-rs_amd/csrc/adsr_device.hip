@@ -2084,9 +2084,20 @@ __global__ void logdot_kernel(const float *a, uint32_t rows, uint32_t cols,
   out[j] = r;
 }
 
+// logf_exact of every input.  For a positive normal input the fused kernels
+// may take the shorter logf_normal instead; where the two differ the output
+// is the NaN 0x7fc0dead, a value glibc never returns for such an input, so a
+// comparison with glibc checks both paths at once.
 __global__ void logf_kernel(const float *x, uint64_t n, float *y) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) y[t] = logf_exact(x[t], kDevLogfTab);
+  if (t < n) {
+    const float v = x[t];
+    float r = logf_exact(v, kDevLogfTab);
+    if (logf_is_normal(v) &&
+        __float_as_uint(logf_normal(v, kDevLogfTab)) != __float_as_uint(r))
+      r = __uint_as_float(0x7fc0deadu);
+    y[t] = r;
+  }
 }
 
 // src/peak_group_selection/mod.rs:23-64 on one wavefront: flags in parallel,
@@ -2840,15 +2851,22 @@ using namespace adsr;
 // ===========================================================================
 extern "C" int adsr_device_available(void) { return device_present() ? 1 : 0; }
 
+// The registry knows the reference's names (src/simd/mod.rs:53-58: "scalar",
+// and "neon" on aarch64) and "hip".  Only "hip" has an implementation here,
+// so the CPU names answer as an unavailable backend does in the reference
+// (src/simd/mod.rs:105-111); unknown names list the known ones (:113-118).
 extern "C" adsr_status adsr_backend_set(const char *name) {
   clear_error();
-  if (!name || std::strcmp(name, "hip") != 0)
+  static const char *const kKnown[] = {"scalar", "neon", "hip"};
+  bool known = false;
+  for (const char *k : kKnown) known = known || (name && std::strcmp(name, k) == 0);
+  if (!known)
     return set_error(ADSR_ERR_INVALID_ARGUMENT,
-                     "Unknown backend '%s'. Available backends: [\"hip\"]",
+                     "Unknown backend '%s'. Available backends: [\"scalar\", \"neon\", \"hip\"]",
                      name ? name : "(null)");
-  if (!device_present())
-    return set_error(ADSR_ERR_UNSUPPORTED,
-                     "Backend 'hip' is not available on this system");
+  if (std::strcmp(name, "hip") != 0 || !device_present())
+    return set_error(ADSR_ERR_INVALID_ARGUMENT,
+                     "Backend '%s' is not available on this system", name);
   g_backend_override.store(0);
   return ADSR_OK;
 }

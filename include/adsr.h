@@ -62,9 +62,12 @@ const char *adsr_last_error(void);
 /* ---- device / backend registry (src/simd/mod.rs:80-134, src/lib.rs:53-72) */
 /* 1 if a gfx950 device is visible and usable. */
 int adsr_device_available(void);
-/* Backend names: "hip".  set fails with INVALID_ARGUMENT for unknown names
- * and UNSUPPORTED for a known but unavailable backend, like
- * simd::set_backend (src/simd/mod.rs:97-119).                              */
+/* Known backend names: "scalar", "neon" (the reference's, src/simd/mod.rs:
+ * 53-58) and "hip", the only one implemented here.  set fails with
+ * INVALID_ARGUMENT and the reference's messages (src/simd/mod.rs:97-119):
+ * "Unknown backend ..." for other names, "Backend '<name>' is not available
+ * on this system" for "scalar", "neon", and "hip" without a gfx950 device
+ * (both are ValueError in Python, src/lib.rs:58-61).                       */
 adsr_status adsr_backend_set(const char *name);
 void adsr_backend_clear(void);
 /* Writes the current / optimal backend name ("hip" or "none"). */

@@ -87,6 +87,7 @@ def lib():
         sig("orc_select_one", C.c_int, C.POINTER(OrcDia), vp, C.POINTER(OrcParams), sz,
             C.POINTER(sz), C.POINTER(sz), C.POINTER(sz), vp, vp, vp, vp, vp, sz)
         sig("orc_logf", f32, f32)
+        sig("orc_logf_check_range", C.c_uint64, C.c_uint32, sz, vp, C.c_int, C.POINTER(sz))
         sig("orc_lib_build_full", vp, sz, vp, vp, vp, vp, vp, vp, sz, vp, vp, vp, vp, vp)
         u8 = C.c_uint8
         sig("orc_correlation_safe", f32, vp, vp, sz)

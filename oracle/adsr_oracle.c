@@ -535,6 +535,64 @@ void orc_convolution(const float *k, size_t klen, const float *xic,
 /* glibc logf; f32::ln lowers to it on x86-64 Linux (src/score/scalar.rs:19) */
 float orc_logf(float x) { return logf(x); }
 
+/* Checker of a logf implementation over a contiguous range of inputs: y[i]
+ * is the candidate's logf of the float with bits first + i.  Counts the i
+ * where y[i] differs from glibc logf bit for bit (two NaNs count as equal:
+ * the payload of a NaN result is not part of the reference's output), on
+ * nthreads threads, and stores the smallest such i in *first_bad (n when
+ * there is none). */
+typedef struct {
+  uint32_t first;
+  size_t lo, hi;
+  const float *y;
+  uint64_t bad;
+  size_t first_bad;
+} LogfCheckPart;
+
+static void *logf_check_worker(void *arg) {
+  LogfCheckPart *p = (LogfCheckPart *)arg;
+  p->bad = 0;
+  p->first_bad = (size_t)-1;
+  for (size_t i = p->lo; i < p->hi; i++) {
+    uint32_t b = p->first + (uint32_t)i;
+    float x;
+    memcpy(&x, &b, 4);
+    float want = logf(x), got = p->y[i];
+    uint32_t wb, gb;
+    memcpy(&wb, &want, 4);
+    memcpy(&gb, &got, 4);
+    if (wb != gb && !(want != want && got != got)) {
+      if (p->bad == 0) p->first_bad = i;
+      p->bad++;
+    }
+  }
+  return NULL;
+}
+
+uint64_t orc_logf_check_range(uint32_t first, size_t n, const float *y,
+                              int nthreads, size_t *first_bad) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  LogfCheckPart parts[256];
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; t++) {
+    parts[t].first = first;
+    parts[t].lo = n * (size_t)t / (size_t)nthreads;
+    parts[t].hi = n * (size_t)(t + 1) / (size_t)nthreads;
+    parts[t].y = y;
+    pthread_create(&th[t], NULL, logf_check_worker, &parts[t]);
+  }
+  uint64_t bad = 0;
+  size_t fb = n;
+  for (int t = 0; t < nthreads; t++) {
+    pthread_join(th[t], NULL);
+    bad += parts[t].bad;
+    if (parts[t].bad && parts[t].first_bad < fb) fb = parts[t].first_bad;
+  }
+  if (first_bad) *first_bad = fb;
+  return bad;
+}
+
 /* src/score/scalar.rs:4-25 axis_log_dot_product_scalar */
 void orc_axis_log_dot_product(const float *a, size_t rows, size_t cols,
                               const float *w, float *out) {

@@ -220,6 +220,10 @@ int orc_select_one(const orc_dia *d, const orc_lib *l, const orc_params *p,
                    float *conv, float *score, size_t cap_cells);
 
 float orc_logf(float x); /* glibc logf, as called by the reference */
+/* Mismatches of y[i] against glibc logf(bits first + i), i < n (NaN == NaN);
+ * smallest mismatching i in *first_bad (n if none).  Threaded checker. */
+uint64_t orc_logf_check_range(uint32_t first, size_t n, const float *y,
+                              int nthreads, size_t *first_bad);
 
 /* ---- scoring: src/peak_group_scoring (adsr_oracle_score.c) ---------- */
 /* CandidateFeature f32 columns (src/candidate/features.rs:51-96, to_dict_

@@ -1,13 +1,16 @@
-"""Candidate table kept in HBM (adsr_select_device) vs the host table.
+"""Candidate table kept in HBM (adsr_select_device) vs the oracle and the
+host table.
 
 search_device().to_torch() must hold the same nine columns, row for row and
-bit for bit, as search().to_arrays() (src/candidate/entry.rs:124-173), on
-torch's current stream and on a side stream, and for an empty result.
+bit for bit, as the oracle's selection and as search().to_arrays()
+(src/candidate/entry.rs:124-173), on torch's current stream and on a side
+stream, and for an empty result.
 """
 import numpy as np
 import pytest
 
-from conftest import assert_tables_equal, params_for
+import oracle
+from conftest import assert_tables_equal, oracle_params_for, params_for
 
 pytestmark = pytest.mark.gpu
 
@@ -34,7 +37,9 @@ def test_device_table_equals_host_table(ads, run0):
     d = ads.DIAData.from_arrays(*run0["dia"])
     lib = ads.SpecLibFlat.from_arrays(*run0["lib"])
     sel = ads.PeakGroupSelection(params_for(run0["spec"]))
-    want = sel.search(d, lib).to_arrays()
+    want = oracle.select(oracle.Dia(run0["dia"]), oracle.Lib(run0["lib"]),
+                         oracle_params_for(run0["spec"]))
+    assert_tables_equal(sel.search(d, lib).to_arrays(), want, "host table")
     dc = sel.search_device(d, lib)
     assert len(dc) == len(want[0]) > 0
     cols = dc.to_torch()
@@ -57,7 +62,9 @@ def test_device_table_on_side_stream(ads, run0):
     d = ads.DIAData.from_arrays(*run0["dia"])
     lib = ads.SpecLibFlat.from_arrays(*run0["lib"])
     sel = ads.PeakGroupSelection(params_for(run0["spec"], candidate_count=5))
-    want = sel.search(d, lib).to_arrays()
+    want = oracle.select(oracle.Dia(run0["dia"]), oracle.Lib(run0["lib"]),
+                         oracle_params_for(run0["spec"], candidate_count=5))
+    assert_tables_equal(sel.search(d, lib).to_arrays(), want, "host table, top-5")
     dc = sel.search_device(d, lib)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):

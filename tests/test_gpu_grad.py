@@ -83,9 +83,12 @@ def _rank0(table, n):
 class F64Model:
     """The FP64 restatement described in the module docstring."""
 
-    def __init__(self, gen, weights, sigma, apex):
+    def __init__(self, gen, weights, sigma, apex, dia=None, lib_o=None, prm=None):
         spec = gen["spec"]
-        od, ol, prm = _oracle_state(gen, weights, sigma)
+        if dia is None:
+            od, ol, prm = _oracle_state(gen, weights, sigma)
+        else:  # oracle handles already built with these weights and sigma
+            od, ol = dia, lib_o
         lib = gen["lib"]
         self.klen = int(np.ceil(spec.kernel_size / 2.0) * 2)
         self.half = self.klen // 2

@@ -247,10 +247,23 @@ def test_search_fails_loudly_without_gpu(run0):
     lib = ads.SpecLibFlat.from_arrays(*run0["lib"])
     with pytest.raises(N.AdsrError, match="gfx950"):
         ads.PeakGroupSelection(ads.SelectionParameters()).search(d, lib)
-    with pytest.raises(ValueError):
-        ads.set_simd_backend("scalar")
-    with pytest.raises(N.AdsrError):
+    with pytest.raises(ValueError, match="Backend 'hip' is not available on this system"):
         ads.set_simd_backend("hip")
+
+
+def test_backend_registry_names():
+    """src/simd/mod.rs:97-119: known names that cannot run here answer as an
+    unavailable backend; unknown names list the known ones.  Both raise
+    ValueError (src/lib.rs:58-61)."""
+    import alphadia_search_rs as ads
+    for name in ("scalar", "neon"):
+        with pytest.raises(ValueError, match=f"Backend '{name}' is not available on this system"):
+            ads.set_simd_backend(name)
+    with pytest.raises(ValueError, match=r"Unknown backend 'avx9'. Available backends: "
+                                         r"\[\"scalar\", \"neon\", \"hip\"\]"):
+        ads.set_simd_backend("avx9")
+    ads.clear_simd_backend()
+    assert ads.get_current_simd_backend() == ads.get_optimal_simd_backend()
 
 
 def test_diff_rejects_shared_fragment_ranges_and_needs_gpu(run0):

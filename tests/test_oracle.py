@@ -410,6 +410,34 @@ def test_score_kat_and_closed_form():
         assert abs(L().orc_logf(float(x)) - math.log(float(x))) <= 2 ** -22 * max(1.0, math.log(float(x)))
 
 
+def test_logf_range_checker_counts_planted_mismatches():
+    """orc_logf_check_range (the checker of the exhaustive device-ln test):
+    on glibc's own values it finds nothing; planted bit flips are counted
+    and the first one located; a NaN with another payload is no mismatch."""
+    import ctypes as C
+    L = oracle.lib()
+    first, n = 0x3f7ff000, 8192          # around 1.0, both sides
+    x = np.arange(first, first + n, dtype=np.uint32).view(np.float32)
+    y = np.array([L.orc_logf(float(v)) for v in x], np.float32)
+    fb = C.c_size_t(0)
+    yp = y.ctypes.data_as(C.c_void_p)
+    assert L.orc_logf_check_range(first, n, yp, 3, C.byref(fb)) == 0 and fb.value == n
+    yb = y.view(np.uint32)
+    for i in (17, 4000, 8191):
+        yb[i] ^= 1
+    assert L.orc_logf_check_range(first, n, yp, 4, C.byref(fb)) == 3 and fb.value == 17
+    # NaN inputs: glibc's NaN and another payload compare equal
+    nan_first = 0x7fc00000
+    yn = np.full(64, np.float32(np.nan))
+    yn.view(np.uint32)[5] = 0x7fc0dead
+    assert L.orc_logf_check_range(nan_first, 64, yn.ctypes.data_as(C.c_void_p), 2,
+                                  C.byref(fb)) == 0
+    # ... but a number where glibc gives NaN is one
+    yn[9] = 1.0
+    assert L.orc_logf_check_range(nan_first, 64, yn.ctypes.data_as(C.c_void_p), 2,
+                                  C.byref(fb)) == 1 and fb.value == 9
+
+
 # ----------------------------------------------------------------- maxima
 @pytest.mark.parametrize("case", GOLD["local_maxima"], ids=lambda c: c["source"])
 def test_local_maxima_kats(case):
