@@ -1667,6 +1667,31 @@ __global__ __launch_bounds__(256) void export_kernel(
   if (cycle_stop) cycle_stop[i] = min(center + pl + 1, nrt);
 }
 
+// The host table's block (eight u64 columns, then the f32 score; see
+// adsr_candidates) for the compacted rows, n = *total read on the device so
+// that the kernel is queued before the host knows the count.  Columns as in
+// export_kernel.
+__global__ __launch_bounds__(256) void table_kernel(const CandRow *rows,
+                                                    const uint32_t *total,
+                                                    const uint64_t *p_idx,
+                                                    uint64_t pl, uint64_t nrt,
+                                                    uint64_t cap, uint64_t *blk) {
+  const uint64_t n = min((uint64_t)*total, cap);
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CandRow r = rows[i];
+  const uint64_t center = r.center;
+  blk[i] = p_idx[r.row];
+  blk[n + i] = r.rank;
+  blk[2 * n + i] = 0;
+  blk[3 * n + i] = 0;
+  blk[4 * n + i] = 0;
+  blk[5 * n + i] = center;
+  blk[6 * n + i] = center - pl;  // usize arithmetic, mod.rs:176
+  blk[7 * n + i] = min(center + pl + 1, nrt);
+  reinterpret_cast<float *>(blk + 8 * n)[i] = r.score;
+}
+
 // Algorithmic bytes per (row, selected fragment): slice_starts of the bin
 // range, every CSR entry's cycle index, the intensity of entries in the RT
 // window, per valid window.  Reduced with one atomic per block.
@@ -2400,6 +2425,9 @@ struct Workspace {
   float *kernels = nullptr;
   float *rt_tol_pp = nullptr;
   uint32_t *key_scratch = nullptr;
+  uint64_t *table = nullptr;  // host table block on the device (68 B per row)
+  bool pp = false;            // per-precursor RT tolerances
+  size_t nfs = 0;             // key_scratch entries
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipGraphExec_t graph = nullptr;
@@ -2426,15 +2454,38 @@ static void ws_free(Workspace &w) {
   (void)hipFree(w.kernels);
   (void)hipFree(w.rt_tol_pp);
   (void)hipFree(w.key_scratch);
+  (void)hipFree(w.table);
   for (auto &e : w.ev)
     if (e) (void)hipEventDestroy(e);
   if (w.stream) (void)hipStreamDestroy(w.stream);
   w = Workspace();
 }
 
+// Idle workspaces of a library (at most kIdleWorkspaces) are taken by the
+// next search with the same shape; a workspace is used by one search at a
+// time, so concurrent searches each get their own.
+constexpr size_t kIdleWorkspaces = 2;
+
 static adsr_status ws_alloc(Workspace &w, size_t n_rows, size_t ks, size_t cc,
-                            size_t nker, bool pp, size_t n_frag_scratch) {
+                            size_t nker, bool pp, size_t n_frag_scratch,
+                            adsr_lib *pool = nullptr) {
   ws_free(w);
+  if (pool) {
+    std::lock_guard<std::mutex> lk(pool->mu);
+    auto &v = pool->idle_workspaces;
+    for (size_t i = 0; i < v.size(); ++i) {
+      Workspace *c = static_cast<Workspace *>(v[i]);
+      if (c->n_rows == n_rows && c->ks == ks && c->cc == cc && c->nker == nker &&
+          c->pp == pp && c->nfs == n_frag_scratch) {
+        w = *c;
+        delete c;
+        v.erase(v.begin() + (long)i);
+        return ADSR_OK;
+      }
+    }
+  }
+  w.pp = pp;
+  w.nfs = n_frag_scratch;
   w.n_rows = n_rows;
   w.ks = ks;
   w.cc = cc;
@@ -2461,6 +2512,90 @@ static adsr_status ws_alloc(Workspace &w, size_t n_rows, size_t ks, size_t cc,
   HIP_TRY(hipMalloc(&w.rt_tol_pp, (pp ? R : 1) * 4));
   HIP_TRY(hipMalloc(&w.key_scratch, std::max<size_t>(n_frag_scratch, 1) * 4));
   return ADSR_OK;
+}
+
+// Returns a finished search's workspace to its library's idle list (or
+// frees it when the list is full).
+static void ws_recycle(adsr_lib *l, Workspace &w) {
+  if (w.graph) (void)hipGraphExecDestroy(w.graph);
+  if (w.graph_src) (void)hipGraphDestroy(w.graph_src);
+  w.graph = nullptr;
+  w.graph_src = nullptr;
+  if (w.stream) {
+    std::lock_guard<std::mutex> lk(l->mu);
+    if (l->idle_workspaces.size() < kIdleWorkspaces) {
+      l->idle_workspaces.push_back(new Workspace(w));
+      w = Workspace();
+      return;
+    }
+  }
+  ws_free(w);
+}
+
+void release_lib_workspaces(adsr_lib *l) {
+  std::vector<void *> v;
+  {
+    std::lock_guard<std::mutex> lk(l->mu);
+    v.swap(l->idle_workspaces);
+  }
+  for (void *p : v) {
+    Workspace *w = static_cast<Workspace *>(p);
+    ws_free(*w);
+    delete w;
+  }
+}
+
+// ---- page-locked blocks for host candidate tables ------------------------
+// Blocks are kept after release up to kPinnedPoolBytes in total and handed
+// to the next request they fit (capacity at most twice the request).  The
+// pool is never destroyed, so tables freed during interpreter shutdown can
+// still return their blocks.
+constexpr size_t kPinnedPoolBytes = size_t(1) << 30;
+
+struct PinnedPool {
+  std::mutex mu;
+  std::multimap<size_t, void *> idle;
+  size_t idle_bytes = 0;
+};
+
+static PinnedPool &pinned_pool() {
+  static PinnedPool *p = new PinnedPool();
+  return *p;
+}
+
+void *pinned_acquire(size_t bytes, size_t *cap) {
+  const size_t want = (std::max<size_t>(bytes, 1) + (size_t(2) << 20) - 1) &
+                      ~((size_t(2) << 20) - 1);
+  PinnedPool &pp = pinned_pool();
+  {
+    std::lock_guard<std::mutex> lk(pp.mu);
+    auto it = pp.idle.lower_bound(want);
+    if (it != pp.idle.end() && it->first <= 2 * want) {
+      void *p = it->second;
+      *cap = it->first;
+      pp.idle_bytes -= it->first;
+      pp.idle.erase(it);
+      return p;
+    }
+  }
+  void *p = nullptr;
+  if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+  *cap = want;
+  return p;
+}
+
+void pinned_release(void *p, size_t cap) {
+  if (!p) return;
+  PinnedPool &pp = pinned_pool();
+  {
+    std::lock_guard<std::mutex> lk(pp.mu);
+    if (pp.idle_bytes + cap <= kPinnedPoolBytes) {
+      pp.idle.emplace(cap, p);
+      pp.idle_bytes += cap;
+      return;
+    }
+  }
+  (void)hipHostFree(p);
 }
 
 struct SearchPlan {
@@ -2558,9 +2693,14 @@ static adsr_status prepare(const adsr_dia *d, const adsr_lib *l,
     return set_error(ADSR_ERR_UNSUPPORTED,
                      "per-precursor LDS tile of %u bytes exceeds the device",
                      sp.lds_per_wave);
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, 0) == hipSuccess)
-    sp.cus = prop.multiProcessorCount;
+  static const int cus = [] {
+    int n = 0;
+    return hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, 0) ==
+                   hipSuccess && n > 0
+               ? n
+               : 256;
+  }();
+  sp.cus = cus;
   return ADSR_OK;
 }
 
@@ -2717,7 +2857,7 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
 
 static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
                                 const adsr_selection_params *p, Workspace &w,
-                                SearchPlan &sp) {
+                                SearchPlan &sp, bool pooled = false) {
   set_lds_limits();
   size_t ks = 1;
   adsr_status st = prepare(d, l, p, sp, ks);
@@ -2725,7 +2865,8 @@ static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
   const size_t cc = (size_t)std::min<uint64_t>(p->candidate_count, 1u << 20);
   st = ws_alloc(w, l->n_prec, ks, cc, sp.kernels.size(),
                 p->rt_tolerance_per_precursor != nullptr,
-                l->max_raw_fragments > (uint64_t)kStashFrag ? l->n_frag : 0);
+                l->max_raw_fragments > (uint64_t)kStashFrag ? l->n_frag : 0,
+                pooled ? const_cast<adsr_lib *>(l) : nullptr);
   if (st) return st;
   HIP_TRY(hipMemcpy(w.kernels, sp.kernels.data(), sp.kernels.size() * 4,
                     hipMemcpyHostToDevice));
@@ -2755,10 +2896,11 @@ static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
   return ADSR_OK;
 }
 
-static adsr_status check_status(Workspace &w) {
+static adsr_status check_status(Workspace &w, uint32_t *total = nullptr) {
   uint32_t host[3] = {0, 0, 0};
   HIP_TRY(hipMemcpyAsync(host, w.small, 12, hipMemcpyDeviceToHost, w.stream));
   HIP_TRY(hipStreamSynchronize(w.stream));
+  if (total) *total = host[2];
   const uint32_t st = host[1];
   if (st & kStTooManyFragments)
     return set_error(ADSR_ERR_UNSUPPORTED,
@@ -2773,37 +2915,47 @@ static adsr_status check_status(Workspace &w) {
   return ADSR_OK;
 }
 
-static adsr_status fetch(Workspace &w, const adsr_dia *d, const adsr_lib *l,
-                         const adsr_selection_params *p, adsr_candidates **out) {
-  uint32_t total = 0;
-  HIP_TRY(hipMemcpyAsync(&total, w.small + 2, 4, hipMemcpyDeviceToHost,
-                         w.stream));
-  HIP_TRY(hipStreamSynchronize(w.stream));
+// Queues the host table's columns (table_kernel) behind the search, so
+// that one device-to-host copy follows the status read.
+static adsr_status queue_table(Workspace &w, const adsr_dia *d, const adsr_lib *l,
+                               const adsr_selection_params *p) {
+  const size_t cap = std::max<size_t>(w.n_rows, 1) * std::max<size_t>(w.cc, 1);
+  if (!w.table)
+    HIP_TRY(hipMalloc(&w.table, cap * adsr_candidates::kRowBytes));
+  if (w.n_rows)
+    table_kernel<<<(uint32_t)((cap + 255) / 256), 256, 0, w.stream>>>(
+        w.out, w.small + 2, l->dev.p_idx, p->peak_length, d->rt_index.size(), cap,
+        w.table);
+  HIP_TRY(hipGetLastError());
+  return ADSR_OK;
+}
+
+// The table queued by queue_table, `total` rows, into one page-locked block
+// (pageable memory when page-locked memory cannot be had).
+static adsr_status fetch(Workspace &w, const adsr_lib *l, uint32_t total,
+                         adsr_candidates **out) {
   if (l->n_prec == 0) total = 0;
-  std::vector<CandRow> rows(total);
-  if (total)
-    HIP_TRY(hipMemcpy(rows.data(), w.out, (size_t)total * sizeof(CandRow),
-                      hipMemcpyDeviceToHost));
+  const size_t bytes = (size_t)total * adsr_candidates::kRowBytes;
+  size_t cap = 0;
+  bool pinned = true;
+  void *blk = pinned_acquire(bytes, &cap);
+  if (!blk) {
+    pinned = false;
+    cap = std::max<size_t>(bytes, 16);
+    blk = std::malloc(cap);
+    if (!blk) return set_error(ADSR_ERR_OUT_OF_MEMORY, "candidate table");
+  }
   auto *c = new adsr_candidates();
-  c->precursor_idx.resize(total);
-  c->rank.resize(total);
-  c->score.resize(total);
-  c->scan_center.assign(total, 0);
-  c->scan_start.assign(total, 0);
-  c->scan_stop.assign(total, 0);
-  c->cycle_center.resize(total);
-  c->cycle_start.resize(total);
-  c->cycle_stop.resize(total);
-  const uint64_t pl = p->peak_length, nrt = d->rt_index.size();
-  for (size_t i = 0; i < total; ++i) {
-    const CandRow &r = rows[i];
-    const uint64_t center = r.center;
-    c->precursor_idx[i] = l->precursor_idx[r.row];
-    c->rank[i] = r.rank;
-    c->score[i] = r.score;
-    c->cycle_center[i] = center;
-    c->cycle_start[i] = center - pl;  // usize arithmetic, mod.rs:176
-    c->cycle_stop[i] = std::min<uint64_t>(center + pl + 1, nrt);
+  c->pinned = pinned;
+  c->block_cap = cap;
+  c->bind(blk, total);
+  if (bytes) {
+    if (hipMemcpyAsync(blk, w.table, bytes, hipMemcpyDeviceToHost, w.stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(w.stream) != hipSuccess) {
+      delete c;
+      return set_error(ADSR_ERR_DEVICE, "candidate table copy failed");
+    }
   }
   *out = c;
   return ADSR_OK;
@@ -2921,7 +3073,7 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
   if (st) return st;
   Workspace w;
   SearchPlan sp;
-  st = build_search(d, l, p, w, sp);
+  st = build_search(d, l, p, w, sp, true);
   if (st) {
     ws_free(w);
     return st;
@@ -3062,10 +3214,16 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
       *graph_ms = gms / steps;
     }
   }
-  st = check_status(w);
-  if (!st) st = dout ? fetch_device(w, d, l, p, dout) : fetch(w, d, l, p, out);
-  ws_free(w);
-  return st;
+  if (!dout) {
+    st = queue_table(w, d, l, p);
+    if (st) return fail(st);
+  }
+  uint32_t total = 0;
+  st = check_status(w, &total);
+  if (!st) st = dout ? fetch_device(w, d, l, p, dout) : fetch(w, l, total, out);
+  if (st) return fail(st);
+  ws_recycle(l, w);
+  return ADSR_OK;
 }
 
 extern "C" adsr_status adsr_select_device(const adsr_dia *d, const adsr_lib *l,
@@ -3103,29 +3261,39 @@ extern "C" adsr_status adsr_device_candidates_to_host(
   clear_error();
   if (!c || !out) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
   *out = nullptr;
-  std::vector<DevCand> rows(c->n);
-  if (c->n)
-    HIP_TRY(hipMemcpy(rows.data(), c->rows, c->n * sizeof(DevCand),
-                      hipMemcpyDeviceToHost));
-  auto *h = new adsr_candidates();
   const size_t n = c->n;
-  h->precursor_idx.resize(n);
-  h->rank.resize(n);
-  h->score.resize(n);
-  h->scan_center.assign(n, 0);
-  h->scan_start.assign(n, 0);
-  h->scan_stop.assign(n, 0);
-  h->cycle_center.resize(n);
-  h->cycle_start.resize(n);
-  h->cycle_stop.resize(n);
-  for (size_t i = 0; i < n; ++i) {
-    const uint64_t center = rows[i].center;
-    h->precursor_idx[i] = rows[i].pidx;
-    h->rank[i] = rows[i].rank;
-    h->score[i] = rows[i].score;
-    h->cycle_center[i] = center;
-    h->cycle_start[i] = center - c->pl;
-    h->cycle_stop[i] = std::min<uint64_t>(center + c->pl + 1, c->nrt);
+  const size_t bytes = n * adsr_candidates::kRowBytes;
+  size_t cap = 0;
+  bool pinned = true;
+  void *blk = pinned_acquire(bytes, &cap);
+  if (!blk) {
+    pinned = false;
+    cap = std::max<size_t>(bytes, 16);
+    blk = std::malloc(cap);
+    if (!blk) return set_error(ADSR_ERR_OUT_OF_MEMORY, "candidate table");
+  }
+  auto *h = new adsr_candidates();
+  h->pinned = pinned;
+  h->block_cap = cap;
+  h->bind(blk, n);
+  if (n) {
+    // the columns expanded on the device, then one copy
+    uint64_t *dblk = nullptr;
+    if (hipMalloc(&dblk, bytes) != hipSuccess) {
+      delete h;
+      return set_error(ADSR_ERR_OUT_OF_MEMORY, "device table block");
+    }
+    export_kernel<<<(uint32_t)((n + 255) / 256), 256>>>(
+        (const DevCand *)c->rows, n, c->pl, c->nrt, dblk, dblk + n,
+        reinterpret_cast<float *>(dblk + 8 * n), dblk + 2 * n, dblk + 3 * n,
+        dblk + 4 * n, dblk + 5 * n, dblk + 6 * n, dblk + 7 * n);
+    const bool ok = hipGetLastError() == hipSuccess &&
+                    hipMemcpy(blk, dblk, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+    (void)hipFree(dblk);
+    if (!ok) {
+      delete h;
+      return set_error(ADSR_ERR_DEVICE, "candidate table copy failed");
+    }
   }
   *out = h;
   return ADSR_OK;
@@ -5204,7 +5372,7 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   const KDia kd = make_kdia(d);
   st = upload_lib_scoring(l, kd);
   if (st) return st;
-  const size_t n = hc ? hc->precursor_idx.size() : dc->n;
+  const size_t n = hc ? hc->n : dc->n;
   const uint64_t nrt = d->rt_index.size();
   // the longest cycle range of the table bounds the LDS per candidate
   uint64_t lmax = 0;
@@ -5289,9 +5457,9 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
     auto up = [&](void *dst, const void *src, size_t bytes) {
       return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, w.stream) == hipSuccess;
     };
-    if (!up(cpidx, hc->precursor_idx.data(), n * 8) || !up(crank, hc->rank.data(), n * 8) ||
-        !up(cs, hc->cycle_start.data(), n * 8) || !up(ce, hc->cycle_stop.data(), n * 8) ||
-        !up(cc, hc->cycle_center.data(), n * 8) || !up(csc, hc->score.data(), n * 4))
+    if (!up(cpidx, hc->precursor_idx, n * 8) || !up(crank, hc->rank, n * 8) ||
+        !up(cs, hc->cycle_start, n * 8) || !up(ce, hc->cycle_stop, n * 8) ||
+        !up(cc, hc->cycle_center, n * 8) || !up(csc, hc->score, n * 4))
       return fail(set_error(ADSR_ERR_DEVICE, "candidate upload failed"));
   } else if (n) {
     export_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, w.stream>>>(

@@ -577,6 +577,7 @@ extern "C" adsr_status adsr_lib_create(
 
 extern "C" void adsr_lib_free(adsr_lib *l) {
   if (!l) return;
+  release_lib_workspaces(l);
   device_lib_free(l->dev);
   delete l;
 }
@@ -592,7 +593,7 @@ extern "C" size_t adsr_lib_num_fragments(const adsr_lib *l) {
 // CandidateCollection -- src/candidate/entry.rs:41-193
 // ===========================================================================
 extern "C" size_t adsr_candidates_len(const adsr_candidates *c) {
-  return c ? c->score.size() : 0;
+  return c ? c->n : 0;
 }
 
 extern "C" adsr_status adsr_candidates_export(
@@ -602,9 +603,9 @@ extern "C" adsr_status adsr_candidates_export(
     uint64_t *cycle_stop) {
   clear_error();
   if (!c) return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL handle");
-  const size_t n = c->score.size();
-  auto put = [n](auto *dst, const auto &src) {
-    if (dst && n) std::memcpy(dst, src.data(), n * sizeof(src[0]));
+  const size_t n = c->n;
+  auto put = [n](auto *dst, const auto *src) {
+    if (dst && n) std::memcpy(dst, src, n * sizeof(src[0]));
   };
   put(precursor_idx, c->precursor_idx);
   put(rank, c->rank);
@@ -629,17 +630,17 @@ extern "C" adsr_status adsr_candidates_from_arrays(
   if (n && (!precursor_idx || !rank || !score || !scan_center || !scan_start ||
             !scan_stop || !cycle_center || !cycle_start || !cycle_stop))
     return set_error(ADSR_ERR_INVALID_ARGUMENT, "column arrays are NULL");
+  void *blk = std::malloc(std::max<size_t>(n * adsr_candidates::kRowBytes, 16));
+  if (!blk) return set_error(ADSR_ERR_OUT_OF_MEMORY, "candidate table");
   auto *c = new adsr_candidates();
+  c->bind(blk, n);
+  const uint64_t *src[8] = {precursor_idx, rank, scan_center, scan_start,
+                            scan_stop, cycle_center, cycle_start, cycle_stop};
+  uint64_t *dst[8] = {c->precursor_idx, c->rank, c->scan_center, c->scan_start,
+                      c->scan_stop, c->cycle_center, c->cycle_start, c->cycle_stop};
   if (n) {
-    c->precursor_idx.assign(precursor_idx, precursor_idx + n);
-    c->rank.assign(rank, rank + n);
-    c->score.assign(score, score + n);
-    c->scan_center.assign(scan_center, scan_center + n);
-    c->scan_start.assign(scan_start, scan_start + n);
-    c->scan_stop.assign(scan_stop, scan_stop + n);
-    c->cycle_center.assign(cycle_center, cycle_center + n);
-    c->cycle_start.assign(cycle_start, cycle_start + n);
-    c->cycle_stop.assign(cycle_stop, cycle_stop + n);
+    for (int i = 0; i < 8; ++i) std::memcpy(dst[i], src[i], n * 8);
+    std::memcpy(c->score, score, n * 4);
   }
   *out = c;
   return ADSR_OK;

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <atomic>
+#include <cstdlib>
 #include <cstddef>
 #include <cstdint>
 #include <mutex>
@@ -152,7 +153,19 @@ struct adsr_lib {
   bool frag_ranges_disjoint = true;
   adsr::DeviceLib dev;
   std::mutex mu;
+  // search workspaces kept for the next search on this library (each one
+  // used by one search at a time; freed with the library)
+  std::vector<void *> idle_workspaces;
 };
+
+namespace adsr {
+// Frees the idle search workspaces of a library (adsr_device.hip).
+void release_lib_workspaces(adsr_lib *l);
+// Page-locked host blocks for candidate tables, recycled through a bounded
+// pool: acquire returns a block of at least `bytes` and its capacity.
+void *pinned_acquire(size_t bytes, size_t *cap);
+void pinned_release(void *p, size_t cap);
+}  // namespace adsr
 
 // A candidate table left in HBM by adsr_select_device: `rows` holds n
 // device records (precursor_idx, rank, score, cycle centre).
@@ -185,9 +198,34 @@ struct adsr_quantified {
   double plan_ms = 0, kernel_ms = 0;
 };
 
+// CandidateCollection (src/candidate/entry.rs:5-16, 41-193): the nine
+// to_arrays columns in one block, eight u64 columns then the f32 score,
+// n entries each (68 bytes per row).  A selection's table is written into a
+// page-locked block by one device-to-host copy; from_arrays uses malloc.
 struct adsr_candidates {
-  // src/candidate/entry.rs:5-16
-  std::vector<uint64_t> precursor_idx, rank, scan_center, scan_start,
-      scan_stop, cycle_center, cycle_start, cycle_stop;
-  std::vector<float> score;
+  size_t n = 0;
+  uint64_t *precursor_idx = nullptr, *rank = nullptr, *scan_center = nullptr,
+           *scan_start = nullptr, *scan_stop = nullptr, *cycle_center = nullptr,
+           *cycle_start = nullptr, *cycle_stop = nullptr;
+  float *score = nullptr;
+  void *block = nullptr;
+  size_t block_cap = 0;
+  bool pinned = false;
+
+  static constexpr size_t kRowBytes = 8 * 8 + 4;
+  // points the columns into `blk`, laid out for n rows
+  void bind(void *blk, size_t rows) {
+    n = rows;
+    block = blk;
+    uint64_t *u = static_cast<uint64_t *>(blk);
+    uint64_t **cols[8] = {&precursor_idx, &rank, &scan_center, &scan_start,
+                          &scan_stop, &cycle_center, &cycle_start, &cycle_stop};
+    for (int i = 0; i < 8; ++i) *cols[i] = u + (size_t)i * rows;
+    score = reinterpret_cast<float *>(u + 8 * rows);
+  }
+  ~adsr_candidates() {
+    if (!block) return;
+    if (pinned) adsr::pinned_release(block, block_cap);
+    else std::free(block);
+  }
 };

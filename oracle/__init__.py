@@ -115,7 +115,7 @@ def lib():
         sig("orc_score", None, C.POINTER(OrcDia), vp, f32, sz, sz, vp, vp, vp, vp, vp,
             vp, vp, C.c_int)
         sig("orc_quantify", None, C.POINTER(OrcDia), vp, f32, sz, sz, vp, vp, vp, vp, sz,
-            vp, vp, vp, vp, vp, vp, vp, vp)
+            vp, vp, vp, vp, vp, vp, vp, vp, C.c_int)
         _lib = L
     return _lib
 
@@ -367,7 +367,7 @@ def score(dia: Dia, library: Lib, candidates, mass_tolerance=7.0, top_k_fragment
 
 
 def quantify(dia: Dia, library: Lib, lib_arrays, candidates, tolerance_ppm=7.0,
-             top_k_fragments=10000):
+             top_k_fragments=10000, nthreads=0):
     """PeakGroupQuantification.quantify (src/peak_group_quantification/mod.rs:
     39-168): (precursor dict, fragment dict) as SpecLibFlatQuantified.
     to_dict_arrays (src/speclib_flat_quantified/mod.rs:126-211).  The
@@ -390,7 +390,8 @@ def quantify(dia: Dia, library: Lib, lib_arrays, candidates, tolerance_ppm=7.0,
     rto = np.zeros(max(n, 1), np.float32)
     lib().orc_quantify(dia.h, library.h, float(tolerance_ppm), int(top_k_fragments), n,
                        p(k[0]), p(k[1]), p(k[2]), p(k[3]), ks, p(row), p(nf), p(raw),
-                       p(inten), p(mzo), p(corr), p(err), p(rto))
+                       p(inten), p(mzo), p(corr), p(err), p(rto),
+                       nthreads or (os.cpu_count() or 1))
     prec = {key: [] for key in ("precursor_idx", "mz_library", "mz", "rt_library", "rt",
                                 "naa", "rank", "rt_observed", "flat_frag_start_idx",
                                 "flat_frag_stop_idx")}

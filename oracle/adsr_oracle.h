@@ -292,7 +292,7 @@ void orc_quantify(const orc_dia *d, const orc_lib *l, float tolerance_ppm,
                   const uint64_t *cycle_start, const uint64_t *cycle_stop,
                   const uint64_t *cycle_center, size_t ks, size_t *row,
                   size_t *nf, size_t *raw, float *intensity, float *mz_obs,
-                  float *corr, float *mass_err, float *rt_observed);
+                  float *corr, float *mass_err, float *rt_observed, int nthreads);
 /* PeakGroupScoring.score over n candidates: features [ORC_N_FEATURES][n]
  * (column-major), found[i] = 0 where precursor_idx is not in the library.
  * The library must come from orc_lib_build_full. */

@@ -613,26 +613,41 @@ void orc_score(const orc_dia *d, const orc_lib *l, float mass_tolerance,
  * correlation with the filtered median profile and mass error (ppm).  The
  * intensity > 0 filter (precursor_quantified/mod.rs:26-94) and the flat
  * layout are left to the caller. */
-void orc_quantify(const orc_dia *d, const orc_lib *l, float tolerance_ppm,
-                  size_t top_k, size_t n, const uint64_t *precursor_idx,
-                  const uint64_t *cycle_start, const uint64_t *cycle_stop,
-                  const uint64_t *cycle_center, size_t ks, size_t *row,
-                  size_t *nf, size_t *raw, float *intensity, float *mz_obs,
-                  float *corr, float *mass_err, float *rt_observed) {
-  for (size_t ci = 0; ci < n; ci++) {
+typedef struct {
+  const orc_dia *d;
+  const orc_lib *l;
+  float tol;
+  size_t top_k, n, ks;
+  const uint64_t *pidx, *start, *stop, *center;
+  size_t *row, *nf, *raw;
+  float *intensity, *mz_obs, *corr, *mass_err, *rt_observed;
+  size_t next;
+  pthread_mutex_t mu;
+} quant_ctx;
+
+static void quantify_one(const quant_ctx *q, size_t ci) {
+  const orc_dia *d = q->d;
+  const orc_lib *l = q->l;
+  const size_t ks = q->ks;
+  size_t *row = q->row, *nf = q->nf, *raw = q->raw;
+  float *intensity = q->intensity, *mz_obs = q->mz_obs, *corr = q->corr,
+        *mass_err = q->mass_err, *rt_observed = q->rt_observed;
+  const float tolerance_ppm = q->tol;
+  const size_t top_k = q->top_k;
+  {
     row[ci] = (size_t)-1;
     nf[ci] = 0;
-    uint64_t key = precursor_idx[ci];
+    uint64_t key = q->pidx[ci];
     size_t lo = 0, hi = l->n_prec;
     while (lo < hi) {
       size_t mid = lo + (hi - lo) / 2;
       if (l->precursor_idx[mid] < key) lo = mid + 1;
       else hi = mid;
     }
-    if (lo >= l->n_prec || l->precursor_idx[lo] != key) continue;
-    uint64_t c0 = cycle_start[ci], c1 = cycle_stop[ci], cc = cycle_center[ci];
+    if (lo >= l->n_prec || l->precursor_idx[lo] != key) return;
+    uint64_t c0 = q->start[ci], c1 = q->stop[ci], cc = q->center[ci];
     uint64_t L = c1 - c0; /* mod.rs:93-98 */
-    if (L == 0 || cc < c0 || cc >= c1) continue;
+    if (L == 0 || cc < c0 || cc >= c1) return;
     size_t fs = l->frag_start[lo], nraw = l->frag_stop[lo] - fs;
     size_t *ord = malloc((nraw ? nraw : 1) * sizeof(size_t));
     size_t F = orc_filter_sort_fragments(nraw, l->fragment_mz + fs,
@@ -665,4 +680,56 @@ void orc_quantify(const orc_dia *d, const orc_lib *l, float tolerance_ppm,
     free(ord); free(fmz); free(xic); free(dmz); free(norm); free(filt);
     free(med); free(mo); free(er);
   }
+}
+
+static void *quant_worker(void *arg) {
+  quant_ctx *q = (quant_ctx *)arg;
+  for (;;) {
+    pthread_mutex_lock(&q->mu);
+    size_t a = q->next;
+    q->next += 256;
+    pthread_mutex_unlock(&q->mu);
+    if (a >= q->n) break;
+    size_t b = a + 256 < q->n ? a + 256 : q->n;
+    for (size_t i = a; i < b; i++) quantify_one(q, i);
+  }
+  return NULL;
+}
+
+/* Candidates are independent (src/peak_group_quantification/mod.rs:57-71
+ * maps them with par_iter, order kept by collect); each writes only its own
+ * slots, on nthreads threads. */
+void orc_quantify(const orc_dia *d, const orc_lib *l, float tolerance_ppm,
+                  size_t top_k, size_t n, const uint64_t *precursor_idx,
+                  const uint64_t *cycle_start, const uint64_t *cycle_stop,
+                  const uint64_t *cycle_center, size_t ks, size_t *row,
+                  size_t *nf, size_t *raw, float *intensity, float *mz_obs,
+                  float *corr, float *mass_err, float *rt_observed, int nthreads) {
+  quant_ctx q;
+  memset(&q, 0, sizeof(q));
+  q.d = d;
+  q.l = l;
+  q.tol = tolerance_ppm;
+  q.top_k = top_k;
+  q.n = n;
+  q.ks = ks;
+  q.pidx = precursor_idx;
+  q.start = cycle_start;
+  q.stop = cycle_stop;
+  q.center = cycle_center;
+  q.row = row;
+  q.nf = nf;
+  q.raw = raw;
+  q.intensity = intensity;
+  q.mz_obs = mz_obs;
+  q.corr = corr;
+  q.mass_err = mass_err;
+  q.rt_observed = rt_observed;
+  pthread_mutex_init(&q.mu, NULL);
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 64) nthreads = 64;
+  pthread_t th[64];
+  for (int t = 0; t < nthreads; t++) pthread_create(&th[t], NULL, quant_worker, &q);
+  for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  pthread_mutex_destroy(&q.mu);
 }

@@ -110,7 +110,8 @@ def test_workload3_full_size_gradient_on_pair_subset(ads, run2):
     the FP64 gradient must match FP64 central differences (h = 1e-4)."""
     import torch
     from alphadia_search_rs.autograd import SelectionGradState, hinge_loss, rank_zero_scores
-    from test_gpu_grad import F64Model, _close, _hinge64, _rank0, H, RTOL
+    from oracle.grad64 import F64Model, hinge64 as _hinge64
+    from test_gpu_grad import _close, _rank0, H, RTOL
 
     gen, d, lib, p = run2["gen"], run2["d"], run2["lib"], run2["p"]
     P = lib.num_precursors

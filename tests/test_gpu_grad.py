@@ -1,5 +1,5 @@
 """Differentiable selection (adsr_diff_* through alphadia_search_rs.autograd)
-against an FP64 restatement of the rank-0 score.
+against an FP64 restatement of the rank-0 score (oracle/grad64.py).
 
 Model, per precursor p with rank-0 apex a (fixed by the forward pass),
 selected fragments f, XIC X (exact f32 sums from the oracle) and the kernel
@@ -27,6 +27,7 @@ import pytest
 
 import oracle
 import synthetic
+from oracle.grad64 import F64Model, hinge64 as _hinge64, oracle_state as _oracle_state
 
 pytestmark = pytest.mark.gpu
 
@@ -60,16 +61,6 @@ def _params(gen, **extra):
     return p
 
 
-def _oracle_state(gen, weights, sigma):
-    lib = list(gen["lib"])
-    lib[10] = weights
-    od, ol = oracle.Dia(gen["dia"]), oracle.Lib(tuple(lib))
-    cfg = gen["spec"].selection_config()
-    prm = oracle.params(**cfg, rt_tolerance_per_precursor=gen["rt_tolerance_per_precursor"],
-                        sigma_per_window=sigma)
-    return od, ol, prm
-
-
 def _rank0(table, n):
     pid, rank, score, cc = table[0], table[1], table[2], table[6]
     s = np.zeros(n, np.float32)
@@ -78,71 +69,6 @@ def _rank0(table, n):
     s[pid[m].astype(np.int64)] = score[m]
     c[pid[m].astype(np.int64)] = cc[m].astype(np.int64)
     return s, c
-
-
-class F64Model:
-    """The FP64 restatement described in the module docstring."""
-
-    def __init__(self, gen, weights, sigma, apex, dia=None, lib_o=None, prm=None):
-        spec = gen["spec"]
-        if dia is None:
-            od, ol, prm = _oracle_state(gen, weights, sigma)
-        else:  # oracle handles already built with these weights and sigma
-            od, ol = dia, lib_o
-        lib = gen["lib"]
-        self.klen = int(np.ceil(spec.kernel_size / 2.0) * 2)
-        self.half = self.klen // 2
-        self.rows = []  # (row, window, fragment indices, X window f64)
-        fs, fe = lib[6].astype(np.int64), lib[7].astype(np.int64)
-        for r in np.flatnonzero(apex >= 0):
-            one = oracle.select_one(od, ol, prm, int(r), max_cells=64 * 4096)
-            assert one is not None
-            c0 = one["cycle_start"]
-            a = int(apex[r]) - c0
-            assert self.half <= a < one["xic"].shape[1] - self.half
-            sel = oracle.filter_sort_fragments(
-                lib[9][fs[r]:fe[r]], weights[fs[r]:fe[r]], lib[14][fs[r]:fe[r]],
-                lib[16][fs[r]:fe[r]], top_k=spec.top_k_fragments)
-            assert np.array_equal(lib[9][fs[r]:fe[r]][sel], one["fragment_mz"])
-            obs = od.valid_observations(float(lib[2][r]))
-            X = one["xic"][:, a - self.half:a + self.half + 1].astype(np.float64)
-            self.rows.append((int(r), int(obs[0]), fs[r] + sel, X))
-        self.n_prec = len(lib[0])
-        self.n_frag = len(lib[10])
-
-    def taps(self, s):
-        x = np.arange(-self.half, self.half, dtype=np.float64)
-        e = np.exp(-x * x / (2.0 * s * s))
-        k = e / e.sum()
-        dk = k * (x * x - (k * x * x).sum()) / (s ** 3)
-        return k[: self.half + 1], dk[: self.half + 1]
-
-    def _y(self, X, k):
-        h = self.half
-        # Y = X[a] k[h] + sum_d (X[a-d] + X[a+d]) k[h-d]; X column h is the apex
-        sym = np.concatenate([X[:, :h] + X[:, h + 1:][:, ::-1], X[:, h:h + 1]], axis=1)
-        return sym @ k
-
-    def s_star(self, sigma, w):
-        s = np.zeros(self.n_prec)
-        for r, o, fi, X in self.rows:
-            k, _ = self.taps(float(sigma[o]))
-            s[r] = (w[fi] * np.log1p(self._y(X, k))).sum()
-        return s
-
-    def grads(self, sigma, w, g):
-        gw = np.zeros(self.n_frag)
-        gs = np.zeros(len(sigma))
-        for r, o, fi, X in self.rows:
-            k, dk = self.taps(float(sigma[o]))
-            y = self._y(X, k)
-            gw[fi] = g[r] * np.log1p(y)
-            gs[o] += g[r] * (w[fi] * self._y(X, dk) / (1.0 + y)).sum()
-        return gw, gs
-
-
-def _hinge64(s, pairs, m):
-    return np.maximum(0.0, m - (s[pairs[:, 0]] - s[pairs[:, 1]])).sum()
 
 
 def _close(got, ref, what):
