@@ -156,6 +156,8 @@ _sig("adsr_quantified_timings", I, vp, C.POINTER(C.c_double), C.POINTER(C.c_doub
 _sig("adsr_quantified_free", None, vp)
 _sig("adsr_device_candidates_len", c_size, vp)
 _sig("adsr_device_candidates_export", I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
+_sig("adsr_select_rows_noalloc", I, vp, vp, C.POINTER(SelectionParamsC), c_size, c_size,
+     vp, vp, vp, vp, vp, vp, vp, vp, vp, c_size, C.POINTER(c_size), vp)
 _sig("adsr_device_candidates_to_host", I, vp, C.POINTER(vp))
 _sig("adsr_device_candidates_free", None, vp)
 _sig("adsr_gaussian_kernel", c_size, c_f32, c_f32, c_u64, c_f32, vp, c_size)

@@ -1692,6 +1692,31 @@ __global__ __launch_bounds__(256) void table_kernel(const CandRow *rows,
   reinterpret_cast<float *>(blk + 8 * n)[i] = r.score;
 }
 
+// The nine to_arrays columns of the compacted rows straight into caller
+// device buffers of `cap` rows (the no-allocation search): n = *total read
+// on the device; rows past cap are not written (the host reports the
+// shortfall).  p_idx starts at the searched rows' first library row.
+__global__ __launch_bounds__(256) void columns_kernel(
+    const CandRow *rows, const uint32_t *total, const uint64_t *p_idx, uint64_t pl,
+    uint64_t nrt, uint64_t cap, uint64_t *pidx, uint64_t *rank, float *score,
+    uint64_t *scan_center, uint64_t *scan_start, uint64_t *scan_stop,
+    uint64_t *cycle_center, uint64_t *cycle_start, uint64_t *cycle_stop) {
+  const uint64_t n = min((uint64_t)*total, cap);
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CandRow r = rows[i];
+  const uint64_t center = r.center;
+  pidx[i] = p_idx[r.row];
+  rank[i] = r.rank;
+  score[i] = r.score;
+  scan_center[i] = 0;
+  scan_start[i] = 0;
+  scan_stop[i] = 0;
+  cycle_center[i] = center;
+  cycle_start[i] = center - pl;  // usize arithmetic, mod.rs:176
+  cycle_stop[i] = min(center + pl + 1, nrt);
+}
+
 // Algorithmic bytes per (row, selected fragment): slice_starts of the bin
 // range, every CSR entry's cycle index, the intensity of entries in the RT
 // window, per valid window.  Reduced with one atomic per block.
@@ -2341,6 +2366,7 @@ static adsr_status upload_dia(adsr_dia *d) {
   UP(v.intens, d->intens.data(), d->intens.size());
   v.bytes = acc;
   v.ready = true;
+  (void)hipGetDevice(&v.device);
   return ADSR_OK;
 }
 #undef UP
@@ -2400,6 +2426,7 @@ static adsr_status upload_lib(adsr_lib *l) {
 #undef UP
   v.bytes = acc;
   v.ready = true;
+  (void)hipGetDevice(&v.device);
   return ADSR_OK;
 }
 
@@ -2609,6 +2636,7 @@ struct SearchPlan {
   int cus = 256;
   uint64_t max_raw = 0;  // longest raw fragment list of the library
   std::vector<float> kernels;
+  size_t n_taps = 0;
 };
 
 // Largest number of RT-index points inside any closed window of width
@@ -2629,9 +2657,12 @@ static uint32_t window_bound(const adsr_dia *d, float eff) {
   return (uint32_t)std::min<size_t>(n, best + 2);
 }
 
+// Taps go to sp.kernels, or to taps[0, taps_cap) when `taps` is given (the
+// no-allocation search), with their count in sp.n_taps.
 static adsr_status prepare(const adsr_dia *d, const adsr_lib *l,
                            const adsr_selection_params *p, SearchPlan &sp,
-                           size_t &ks_out) {
+                           size_t &ks_out, float *taps = nullptr,
+                           size_t taps_cap = 0) {
   if (p->kernel_size == 0)
     return set_error(ADSR_ERR_INVALID_ARGUMENT,
                      "kernel_size must be >= 1 (the reference indexes "
@@ -2645,15 +2676,21 @@ static adsr_status prepare(const adsr_dia *d, const adsr_lib *l,
     return set_error(ADSR_ERR_INVALID_ARGUMENT,
                      "sigma_per_window has %llu values for %zu observations",
                      (unsigned long long)p->n_sigma_per_window, d->n_obs);
-  sp.kernels.assign(klen, 0.0f);
-  adsr_gaussian_kernel(p->fwhm_rt, 1.0f, p->kernel_size, 1.0f,
-                       sp.kernels.data(), klen);
-  if (p->sigma_per_window) {
-    sp.kernels.resize(klen * (1 + d->n_obs));
-    for (size_t o = 0; o < d->n_obs; ++o)
-      adsr_gaussian_kernel_sigma(p->sigma_per_window[o], klen,
-                                 sp.kernels.data() + klen * (1 + o), klen);
+  sp.n_taps = klen * (p->sigma_per_window ? 1 + d->n_obs : 1);
+  float *kt = taps;
+  if (!kt) {
+    sp.kernels.assign(sp.n_taps, 0.0f);
+    kt = sp.kernels.data();
+  } else if (sp.n_taps > taps_cap) {
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "%zu kernel taps (one kernel per window) exceed the %zu of "
+                     "the no-allocation search", sp.n_taps, taps_cap);
   }
+  adsr_gaussian_kernel(p->fwhm_rt, 1.0f, p->kernel_size, 1.0f, kt, klen);
+  if (p->sigma_per_window)
+    for (size_t o = 0; o < d->n_obs; ++o)
+      adsr_gaussian_kernel_sigma(p->sigma_per_window[o], klen, kt + klen * (1 + o),
+                                 klen);
   // selected-fragment stride
   uint64_t ks = std::min<uint64_t>(p->top_k_fragments, l->max_raw_fragments);
   ks = std::min<uint64_t>(ks, kMaxSel);
@@ -2855,6 +2892,11 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
   return ADSR_OK;
 }
 
+static void fill_plan(const adsr_dia *d, const adsr_lib *l,
+                      const adsr_selection_params *p, Workspace &w,
+                      SearchPlan &sp, size_t ks, size_t cc, size_t row0,
+                      size_t n);
+
 static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
                                 const adsr_selection_params *p, Workspace &w,
                                 SearchPlan &sp, bool pooled = false) {
@@ -2873,12 +2915,23 @@ static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
   if (p->rt_tolerance_per_precursor && l->n_prec)
     HIP_TRY(hipMemcpy(w.rt_tol_pp, p->rt_tolerance_per_precursor,
                       l->n_prec * 4, hipMemcpyHostToDevice));
-  const Grid &g = global_grid();
-  const DeviceDia &dv = d->dev;
+  fill_plan(d, l, p, w, sp, ks, cc, 0, l->n_prec);
+  return ADSR_OK;
+}
+
+// The kernel arguments of a search over library rows [row0, row0 + n) in
+// workspace w: the library arrays indexed by row start at row0, so the
+// plan and outputs are indexed by row - row0 (fragment ranges stay
+// absolute).
+static void fill_plan(const adsr_dia *d, const adsr_lib *l,
+                      const adsr_selection_params *p, Workspace &w,
+                      SearchPlan &sp, size_t ks, size_t cc, size_t row0,
+                      size_t n) {
   sp.dia = make_kdia(d);
   const DeviceLib &lv = l->dev;
-  sp.lib = KLib{(uint32_t)l->n_prec, lv.p_mz,  lv.p_rt, lv.f_start, lv.f_stop,
-                lv.f_mz,             lv.f_int, lv.f_type, lv.f_num};
+  sp.lib = KLib{(uint32_t)n,        lv.p_mz + row0, lv.p_rt + row0, lv.f_start + row0,
+                lv.f_stop + row0,   lv.f_mz,        lv.f_int,       lv.f_type,
+                lv.f_num};
   sp.prm.mass_tol = p->mass_tolerance;
   sp.prm.rt_tol = p->rt_tolerance;
   sp.prm.rt_tol_pp = p->rt_tolerance_per_precursor ? w.rt_tol_pp : nullptr;
@@ -2893,7 +2946,6 @@ static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
   sp.max_raw = l->max_raw_fragments;
   sp.plan = KPlan{w.rec, w.obs, w.sel, w.sel_mz, w.sel_w, w.rng, w.b0,
                   (uint32_t)ks};
-  return ADSR_OK;
 }
 
 static adsr_status check_status(Workspace &w, uint32_t *total = nullptr) {
@@ -3235,6 +3287,165 @@ extern "C" adsr_status adsr_select_device(const adsr_dia *d, const adsr_lib *l,
   }
   return select_common(d, l, p, 0, 0, nullptr, nullptr, nullptr, nullptr,
                        nullptr, nullptr, out);
+}
+
+// ---------------------------------------------------------------------------
+// search without allocation: every buffer in one file-scope device array
+// ---------------------------------------------------------------------------
+// Callers that cannot allocate device memory (code under HIP graph capture,
+// serving loops, harnesses that forbid hipMalloc) search library rows
+// [row_begin, row_end) with adsr_select_rows_noalloc.  The plan, candidate
+// slots, compaction and queue live in `scratch`, each GPU's own copy of
+// this array; its address on the current device is looked up once per
+// device with hipGetSymbolAddress, and only that address is used on the
+// host.  Every call on a GPU shares the array: calls that overlap on one
+// GPU race.  The queue and status words are cleared by a kernel at the
+// start of every call; every other buffer is written before it is read.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_at[kScratchDevices];
+constexpr size_t kNoAllocTaps = 4096;  // kernel taps (one kernel per window)
+
+static adsr_status scratch_base(int dev, unsigned char **out) {
+  if (dev < 0 || dev >= kScratchDevices)
+    return set_error(ADSR_ERR_UNSUPPORTED, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_at[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(ADSR_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s",
+                       hipGetErrorString(e));
+    g_scratch_at[dev] = p;
+  }
+  *out = static_cast<unsigned char *>(g_scratch_at[dev]);
+  return ADSR_OK;
+}
+
+extern "C" adsr_status adsr_select_rows_noalloc(
+    const adsr_dia *d, const adsr_lib *l, const adsr_selection_params *p,
+    size_t row_begin, size_t row_end, uint64_t *precursor_idx, uint64_t *rank,
+    float *score, uint64_t *scan_center, uint64_t *scan_start, uint64_t *scan_stop,
+    uint64_t *cycle_center, uint64_t *cycle_start, uint64_t *cycle_stop,
+    size_t capacity, size_t *n_out, void *stream) {
+  clear_error();
+  if (!d || !l || !p || !n_out)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "NULL argument");
+  *n_out = 0;
+  if (capacity && (!precursor_idx || !rank || !score || !scan_center || !scan_start ||
+                   !scan_stop || !cycle_center || !cycle_start || !cycle_stop))
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "column buffers are NULL");
+  if (row_begin > row_end || row_end > l->n_prec)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT, "rows [%zu, %zu) outside the library of %zu",
+                     row_begin, row_end, l->n_prec);
+  if (!d->dev.ready || !l->dev.ready)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT,
+                     "the DIAData and the library must be on the device first "
+                     "(adsr_dia_upload, adsr_lib_upload)");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (d->dev.device != dev || l->dev.device != dev)
+    return set_error(ADSR_ERR_INVALID_ARGUMENT,
+                     "the handles live on another device than the current one");
+  if (l->max_raw_fragments > (uint64_t)kStashFrag)
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "fragment lists longer than %d are not searched without "
+                     "allocation", kStashFrag);
+  unsigned char *base = nullptr;
+  adsr_status st = scratch_base(dev, &base);
+  if (st) return st;
+  set_lds_limits();
+  SearchPlan sp;
+  size_t ks = 1;
+  float taps[kNoAllocTaps];
+  st = prepare(d, l, p, sp, ks, taps, kNoAllocTaps);
+  if (st) return st;
+  const size_t n = row_end - row_begin;
+  if (n == 0) return ADSR_OK;
+  const size_t cc = (size_t)std::min<uint64_t>(p->candidate_count, 1u << 20);
+  const size_t ccs = std::max<size_t>(cc, 1);  // slots per row
+  const bool pp = p->rt_tolerance_per_precursor != nullptr;
+  // layout: each buffer 256-byte aligned
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  const size_t nb = (n + kScanBlock - 1) / kScanBlock;
+  const size_t o_small = take(64), o_rec = take(n * sizeof(uint4)),
+               o_obs = take(n * kObsMax * 2), o_sel = take(n * ks * 2),
+               o_smz = take(n * ks * 4), o_sw = take(n * ks * 4),
+               o_cs = take(n * ccs * 4), o_cc = take(n * ccs * 4), o_cnt = take(n * 4),
+               o_loc = take(n * 4), o_bsum = take(nb * 4),
+               o_out = take(n * ccs * sizeof(CandRow)), o_ker = take(sp.n_taps * 4),
+               o_rt = take(pp ? n * 4 : 0);
+  if (off > kScratchBytes) {
+    const size_t per_row = off / n + 1;
+    return set_error(ADSR_ERR_INVALID_ARGUMENT,
+                     "rows [%zu, %zu) need %zu bytes of scratch, the limit is %zu: "
+                     "search at most %zu rows per call", row_begin, row_end, off,
+                     kScratchBytes, kScratchBytes / per_row);
+  }
+  Workspace w;
+  w.n_rows = n;
+  w.ks = ks;
+  w.cc = cc;
+  w.nker = sp.n_taps;
+  w.small = reinterpret_cast<uint32_t *>(base + o_small);
+  w.rec = reinterpret_cast<uint4 *>(base + o_rec);
+  w.obs = reinterpret_cast<uint16_t *>(base + o_obs);
+  w.sel = reinterpret_cast<uint16_t *>(base + o_sel);
+  w.sel_mz = reinterpret_cast<float *>(base + o_smz);
+  w.sel_w = reinterpret_cast<float *>(base + o_sw);
+  w.cand_score = reinterpret_cast<float *>(base + o_cs);
+  w.cand_center = reinterpret_cast<uint32_t *>(base + o_cc);
+  w.cand_count = reinterpret_cast<uint32_t *>(base + o_cnt);
+  w.local = reinterpret_cast<uint32_t *>(base + o_loc);
+  w.bsum = reinterpret_cast<uint32_t *>(base + o_bsum);
+  w.out = reinterpret_cast<CandRow *>(base + o_out);
+  w.kernels = reinterpret_cast<float *>(base + o_ker);
+  w.rt_tol_pp = pp ? reinterpret_cast<float *>(base + o_rt) : nullptr;
+  w.stream = (hipStream_t)stream;
+  // scratch memory that is copied, not allocated: none of it is freed
+  auto release = [&](adsr_status s2) {
+    w = Workspace();
+    return s2;
+  };
+  if (hipMemcpyAsync(w.kernels, taps, sp.n_taps * 4, hipMemcpyHostToDevice, w.stream) !=
+          hipSuccess ||
+      (pp && hipMemcpyAsync(w.rt_tol_pp, p->rt_tolerance_per_precursor + row_begin, n * 4,
+                            hipMemcpyHostToDevice, w.stream) != hipSuccess))
+    return release(set_error(ADSR_ERR_DEVICE, "copy of taps or tolerances failed"));
+  fill_plan(d, l, p, w, sp, ks, cc, row_begin, n);
+  LaunchShape shape{4, 2};
+  {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    auto it = g_tuned.find(std::make_tuple(n, sp.lcap, ks));
+    if (it != g_tuned.end()) shape = it->second;
+  }
+  st = enqueue(w, sp, (uint32_t)n, shape, nullptr, 0, 0, 0, false);
+  if (st) return release(st);
+  if (capacity) {
+    const uint64_t cap_rows = (uint64_t)n * ccs;
+    columns_kernel<<<(uint32_t)((cap_rows + 255) / 256), 256, 0, w.stream>>>(
+        w.out, w.small + 2, l->dev.p_idx + row_begin, p->peak_length,
+        d->rt_index.size(), capacity, precursor_idx, rank, score, scan_center,
+        scan_start, scan_stop, cycle_center, cycle_start, cycle_stop);
+    if (hipGetLastError() != hipSuccess)
+      return release(set_error(ADSR_ERR_DEVICE, "column kernel launch failed"));
+  }
+  uint32_t total = 0;
+  st = check_status(w, &total);
+  if (st) return release(st);
+  *n_out = total;
+  if (total > capacity)
+    return release(set_error(ADSR_ERR_INVALID_ARGUMENT,
+                             "%u candidates found, the columns hold %zu: the first %zu "
+                             "are written", total, capacity, capacity));
+  return release(ADSR_OK);
 }
 
 extern "C" size_t adsr_device_candidates_len(const adsr_device_candidates *c) {
@@ -4527,6 +4738,7 @@ extern "C" adsr_status adsr_dia_create_device(
       device_dia_free(d->dev);
     } else {
       d->dev.ready = true;
+      (void)hipGetDevice(&d->dev.device);
     }
   }
   if (st) {

@@ -316,6 +316,29 @@ adsr_status adsr_device_candidates_to_host(const adsr_device_candidates *c,
                                            adsr_candidates **out);
 void adsr_device_candidates_free(adsr_device_candidates *c);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  PeakGroupSelection::search over library rows
+ * [row_begin, row_end) (sorted order) with no allocation: the plan,
+ * candidate slots and compaction live in a 64 MiB device array of the
+ * library (one per GPU), so a call takes at most as many rows as fit it
+ * (about 290k rows at top_k 12 / top-3; larger ranges fail with
+ * INVALID_ARGUMENT and the row limit in the message).  The DIAData and the
+ * library must already be on the current device (adsr_dia_upload,
+ * adsr_lib_upload).  The candidates of those rows, in library order, are
+ * written as the nine to_arrays columns into caller device buffers of
+ * `capacity` rows on `stream` (a hipStream_t), and *n_out is their count;
+ * more than `capacity` candidates fail with INVALID_ARGUMENT after the
+ * first `capacity` are written (*n_out still the count).  The call waits
+ * for the stream.  Calls that overlap on one GPU share the array and race.
+ * Fragment lists longer than 48 entries are UNSUPPORTED here.             */
+adsr_status adsr_select_rows_noalloc(
+    const adsr_dia *d, const adsr_lib *l, const adsr_selection_params *p,
+    size_t row_begin, size_t row_end, uint64_t *precursor_idx, uint64_t *rank,
+    float *score, uint64_t *scan_center, uint64_t *scan_start,
+    uint64_t *scan_stop, uint64_t *cycle_center, uint64_t *cycle_start,
+    uint64_t *cycle_stop, size_t capacity, size_t *n_out, void *stream);
+
 /* ---- PeakGroupScoring (src/peak_group_scoring/mod.rs:43-341) ---------- */
 /* ScoringParameters (src/peak_group_scoring/parameters.rs:4-40) */
 typedef struct adsr_scoring_params {
