@@ -650,6 +650,7 @@ struct SelArgs {
   uint32_t *defer_count;
   const uint32_t *rows;
   const uint32_t *n_rows_at;
+  uint32_t row0;  // library row of row 0 (chunked searches): defer list entries
 };
 
 // Per-wave LDS.  XIC rows have kXPad zero floats on both sides so that a
@@ -746,9 +747,12 @@ __device__ __forceinline__ WaveLds carve(char *base, uint32_t L, uint32_t nsel,
 }
 
 // small[0]: queue, [1]: status, [3]: queue of the long-row launch,
-// [4]: deferred row count ([2], the candidate total, is written by the scan)
+// [4]: deferred row count, [8 + k]: queue of chunk k of a chunked search
+// ([2], the candidate total, is written by the scan)
+constexpr int kSmallWords = 32;
+constexpr int kMaxChunks = kSmallWords - 8;
 __global__ void reset_queue_kernel(uint32_t *small) {
-  if (threadIdx.x < 5 && threadIdx.x != 2) small[threadIdx.x] = 0;
+  if (threadIdx.x < kSmallWords && threadIdx.x != 2) small[threadIdx.x] = 0;
 }
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t lane) {
@@ -1032,7 +1036,7 @@ void select_kernel(SelArgs a) {
           // deferred to the launch over long rows.  Every lane adds (lane 0
           // one, the others zero), so lane 0's old value is the row's slot.
           const uint32_t slot = uni(atomicAdd(a.defer_count, lane == 0 ? 1u : 0u));
-          a.defer_rows[slot] = row;
+          a.defer_rows[slot] = a.row0 + row;
         }
         if (more) {
           stage2();
@@ -2457,6 +2461,10 @@ struct Workspace {
   size_t nfs = 0;             // key_scratch entries
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // chunked searches: plans on stream2, chunk k's select waits for ev_plan[k]
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_plan[kMaxChunks] = {};
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_src = nullptr;
 };
@@ -2484,8 +2492,32 @@ static void ws_free(Workspace &w) {
   (void)hipFree(w.table);
   for (auto &e : w.ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : w.ev_plan)
+    if (e) (void)hipEventDestroy(e);
+  if (w.ev_fork) (void)hipEventDestroy(w.ev_fork);
+  if (w.stream2) (void)hipStreamDestroy(w.stream2);
   if (w.stream) (void)hipStreamDestroy(w.stream);
   w = Workspace();
+}
+
+// Libraries of more than kChunkAbove rows are searched in chunks of at most
+// kChunkRows rows (ADSR_CHUNK_ROWS overrides the chunk size; 0 turns
+// chunking off): chunk k + 1's plan runs on a second stream while chunk k
+// is selected, and one compaction over all rows keeps library order.
+constexpr uint32_t kChunkAbove = 1u << 20;
+constexpr uint32_t kChunkRows = 1u << 19;
+
+static uint32_t search_chunks(uint32_t n_rows) {
+  static const long env = [] {
+    const char *e = std::getenv("ADSR_CHUNK_ROWS");
+    return e ? std::strtol(e, nullptr, 10) : -1L;
+  }();
+  uint32_t rows = kChunkRows;
+  if (env == 0) return 1;
+  if (env > 0) rows = (uint32_t)std::min<long>(env, 0x7fffffffL);
+  else if (n_rows <= kChunkAbove) return 1;
+  const uint64_t k = ((uint64_t)n_rows + rows - 1) / rows;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(k, kMaxChunks));
 }
 
 // Idle workspaces of a library (at most kIdleWorkspaces) are taken by the
@@ -2521,6 +2553,11 @@ static adsr_status ws_alloc(Workspace &w, size_t n_rows, size_t ks, size_t cc,
   const size_t nb = (R + kScanBlock - 1) / kScanBlock;
   HIP_TRY(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
   for (auto &e : w.ev) HIP_TRY(hipEventCreate(&e));
+  if (search_chunks((uint32_t)n_rows) > 1) {
+    HIP_TRY(hipStreamCreateWithFlags(&w.stream2, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&w.ev_fork, hipEventDisableTiming));
+    for (auto &e : w.ev_plan) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   HIP_TRY(hipMalloc(&w.rec, R * sizeof(uint4)));
   HIP_TRY(hipMalloc(&w.obs, R * kObsMax * 2));
   HIP_TRY(hipMalloc(&w.sel, R * ks * 2));
@@ -2533,7 +2570,7 @@ static adsr_status ws_alloc(Workspace &w, size_t n_rows, size_t ks, size_t cc,
   HIP_TRY(hipMalloc(&w.cand_count, R * 4));
   HIP_TRY(hipMalloc(&w.local, R * 4));
   HIP_TRY(hipMalloc(&w.bsum, nb * 4));
-  HIP_TRY(hipMalloc(&w.small, 64));
+  HIP_TRY(hipMalloc(&w.small, kSmallWords * 4));
   HIP_TRY(hipMalloc(&w.out, R * std::max<size_t>(cc, 1) * sizeof(CandRow)));
   HIP_TRY(hipMalloc(&w.kernels, nker * 4));
   HIP_TRY(hipMalloc(&w.rt_tol_pp, (pp ? R : 1) * 4));
@@ -2772,25 +2809,66 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
   reset_queue_kernel<<<1, 64, 0, s>>>(w.small);
   // fragment lists of at most 64 entries: a lane group per row; longer
   // ones: one thread per row with an LDS stash / global scratch
-  auto launch_plan = [&]() {
+  auto plan_rows = [&](const KLib &lb, const KParams &pm, const KPlan &pl, uint32_t n,
+                       hipStream_t st) {
     constexpr uint32_t wpb = kPlanThreads / kWave;  // rows per block: wpb * 64 / LP
     if (sp.max_raw <= 16)
-      plan_group_kernel<16><<<(n_rows + 4 * wpb - 1) / (4 * wpb), kPlanThreads, 0, s>>>(
-          sp.dia, sp.lib, sp.prm, sp.plan, status);
+      plan_group_kernel<16><<<(n + 4 * wpb - 1) / (4 * wpb), kPlanThreads, 0, st>>>(
+          sp.dia, lb, pm, pl, status);
     else if (sp.max_raw <= 32)
-      plan_group_kernel<32><<<(n_rows + 2 * wpb - 1) / (2 * wpb), kPlanThreads, 0, s>>>(
-          sp.dia, sp.lib, sp.prm, sp.plan, status);
+      plan_group_kernel<32><<<(n + 2 * wpb - 1) / (2 * wpb), kPlanThreads, 0, st>>>(
+          sp.dia, lb, pm, pl, status);
     else if (sp.max_raw <= 64)
-      plan_group_kernel<64><<<(n_rows + wpb - 1) / wpb, kPlanThreads, 0, s>>>(
-          sp.dia, sp.lib, sp.prm, sp.plan, status);
+      plan_group_kernel<64><<<(n + wpb - 1) / wpb, kPlanThreads, 0, st>>>(
+          sp.dia, lb, pm, pl, status);
     else
-      plan_kernel<kStashFrag><<<(n_rows + kWave - 1) / kWave, kWave, 0, s>>>(
-          sp.dia, sp.lib, sp.prm, sp.plan, w.key_scratch, status);
+      plan_kernel<kStashFrag><<<(n + kWave - 1) / kWave, kWave, 0, st>>>(
+          sp.dia, lb, pm, pl, w.key_scratch, status);
   };
+  auto launch_plan = [&]() { plan_rows(sp.lib, sp.prm, sp.plan, n_rows, s); };
+  // rows [r0, r0 + n) as a search of their own (library arrays, tolerances
+  // and plan slots offset by r0; fragment ranges stay absolute)
+  auto chunk_args = [&](uint32_t r0, uint32_t n, KLib &lb, KParams &pm, KPlan &pl) {
+    lb = sp.lib;
+    lb.n_prec = n;
+    lb.p_mz += r0;
+    lb.p_rt += r0;
+    lb.f_start += r0;
+    lb.f_stop += r0;
+    pm = sp.prm;
+    if (pm.rt_tol_pp) pm.rt_tol_pp += r0;
+    pl = sp.plan;
+    pl.rec += r0;
+    pl.obs += (size_t)r0 * kObsMax;
+    pl.sel += (size_t)r0 * pl.ks;
+    pl.sel_mz += (size_t)r0 * pl.ks;
+    pl.sel_w += (size_t)r0 * pl.ks;
+    if (pl.rng) pl.rng += (size_t)r0 * kObsMax * pl.ks;
+    if (pl.b0) pl.b0 += (size_t)r0 * kObsMax * pl.ks;
+  };
+  const uint32_t nchunk =
+      (w.stream2 && !dbg && !with_bins && !plan_only) ? search_chunks(n_rows) : 1u;
+  const uint32_t crows = (n_rows + nchunk - 1) / nchunk;
   if (n_rows && plan_only) {
     launch_plan();
   } else if (n_rows) {
-    launch_plan();
+    if (nchunk > 1) {
+      // the plans of all chunks on the second stream, each chunk's select
+      // waiting for its plan below
+      HIP_TRY(hipEventRecord(w.ev_fork, s));
+      HIP_TRY(hipStreamWaitEvent(w.stream2, w.ev_fork, 0));
+      for (uint32_t k = 0; k < nchunk; ++k) {
+        const uint32_t r0 = k * crows, n = std::min(crows, n_rows - r0);
+        KLib lb;
+        KParams pm;
+        KPlan pl;
+        chunk_args(r0, n, lb, pm, pl);
+        plan_rows(lb, pm, pl, n, w.stream2);
+        HIP_TRY(hipEventRecord(w.ev_plan[k], w.stream2));
+      }
+    } else {
+      launch_plan();
+    }
     // the select kernel finds bin and CSR ranges itself; the per-window
     // range table is only built for the byte count and the backward pass
     if (with_bins) {
@@ -2817,6 +2895,7 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
     a.dbg_stride = dbg_stride;
     a.dbg_row0 = dbg_row0;
     a.dbg_rows = dbg_rows;
+    a.row0 = 0;
     while (shape.waves > 1 &&
            (size_t)sp.lds_per_wave * shape.waves > kMaxDynLds)
       shape.waves >>= 1;
@@ -2850,10 +2929,10 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
             : (shape.waves == 1   ? (const void *)select_kernel<1, 0>
                : shape.waves == 2 ? (const void *)select_kernel<2, 0>
                                   : (const void *)select_kernel<4, 0>);
-    auto launch_select = [&](const SelArgs &x) {
+    auto launch_select_on = [&](const SelArgs &x, hipStream_t s) {
       const size_t lds = (size_t)x.lds_per_wave * shape.waves;
       const int per_cu = occupancy_blocks(fn, shape.waves * kWave, lds);
-      const uint64_t want = ((uint64_t)n_rows + shape.chunk - 1) / shape.chunk;
+      const uint64_t want = ((uint64_t)x.n_rows + shape.chunk - 1) / shape.chunk;
       const uint32_t blocks = (uint32_t)std::max<uint64_t>(
           1, std::min<uint64_t>((uint64_t)per_cu * sp.cus,
                                 (want + shape.waves - 1) / shape.waves));
@@ -2867,8 +2946,35 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
         else select_kernel<4, 0><<<blocks, 256, lds, s>>>(x);
       }
     };
+    auto launch_select = [&](const SelArgs &x) { launch_select_on(x, s); };
     if (timed_select) HIP_TRY(hipEventRecord(w.ev[2], s));
-    if (split) {
+    if (nchunk > 1) {
+      // chunk k's rows through the first select launch once its plan is
+      // done; long rows of every chunk go to the one long-row launch
+      const SelArgs b = a;
+      for (uint32_t k = 0; k < nchunk; ++k) {
+        const uint32_t r0 = k * crows, n = std::min(crows, n_rows - r0);
+        SelArgs x = a;
+        chunk_args(r0, n, x.lib, x.prm, x.plan);
+        x.n_rows = n;
+        x.queue = w.small + 8 + k;
+        x.cand_score += (size_t)r0 * sp.prm.cand_count;
+        x.cand_center += (size_t)r0 * sp.prm.cand_count;
+        x.cand_count += r0;
+        x.row0 = r0;
+        if (split) x.lds_per_wave = fit;
+        HIP_TRY(hipStreamWaitEvent(s, w.ev_plan[k], 0));
+        launch_select(x);
+      }
+      if (split) {
+        SelArgs y = b;
+        y.split_l = sp.lcap;
+        y.queue = w.small + 3;
+        y.rows = w.local;
+        y.n_rows_at = w.small + 4;
+        launch_select(y);
+      }
+    } else if (split) {
       SelArgs b = a;
       a.lds_per_wave = fit;
       launch_select(a);
@@ -3375,7 +3481,7 @@ extern "C" adsr_status adsr_select_rows_noalloc(
     return at;
   };
   const size_t nb = (n + kScanBlock - 1) / kScanBlock;
-  const size_t o_small = take(64), o_rec = take(n * sizeof(uint4)),
+  const size_t o_small = take(kSmallWords * 4), o_rec = take(n * sizeof(uint4)),
                o_obs = take(n * kObsMax * 2), o_sel = take(n * ks * 2),
                o_smz = take(n * ks * 4), o_sw = take(n * ks * 4),
                o_cs = take(n * ccs * 4), o_cc = take(n * ccs * 4), o_cnt = take(n * 4),
