@@ -19,6 +19,7 @@ in libadsr.so (adsr_diff_forward / adsr_diff_backward); there is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 import torch
@@ -58,6 +59,9 @@ class SelectionGradState:
         check(N.lib.adsr_diff_create(dia._h, lib._h, C.byref(c), C.byref(out)))
         self._h = out.value
         self._fwd_done = False
+        # forwards run so far: a backward belongs to the forward whose apexes,
+        # fragment selections, weights and taps the state holds
+        self.forward_count = 0
 
     def __del__(self, _free=N.lib.adsr_diff_free):
         # the free function is bound at definition: module globals may be
@@ -103,13 +107,23 @@ class SelectionGradState:
             check(N.lib.adsr_diff_forward(self._h, N.ptr(sig), _dev_ptr(w),
                                           _dev_ptr(s_star), has_p))
         self._fwd_done = True
+        self.forward_count += 1
         return s_star, (has.bool() if has is not None else None)
 
-    def backward(self, grad_s: torch.Tensor):
+    def backward(self, grad_s: torch.Tensor, forward_id: int | None = None):
         """Return (grad_weights f32[n_fragments] on the device,
-        grad_sigma f64[n_observations] on the host) for dLoss/ds* = grad_s."""
+        grad_sigma f64[n_observations] on the host) for dLoss/ds* = grad_s,
+        at the last forward.  ``forward_id``: the ``forward_count`` right
+        after the forward the gradient is for; another forward since then
+        raises RuntimeError (the state holds only the last forward)."""
         if not self._fwd_done:
             raise RuntimeError("backward before forward")
+        if forward_id is not None and forward_id != self.forward_count:
+            raise RuntimeError(
+                f"backward of forward {forward_id}, but forward {self.forward_count} "
+                "ran on this SelectionGradState since: its apexes and fragment "
+                "selections replaced those of the earlier forward; use one state "
+                "per forward that is differentiated")
         g = grad_s.detach().to(torch.float32).contiguous()
         if not g.is_cuda or g.numel() != self.num_precursors:
             raise ValueError("grad_s must be a device tensor with one value "
@@ -120,6 +134,12 @@ class SelectionGradState:
         check(N.lib.adsr_diff_backward(self._h, _dev_ptr(g), _dev_ptr(gw),
                                        N.ptr(gs)))
         return gw, torch.from_numpy(gs)
+
+    def settle(self):
+        """Wait for the last forward and raise its device status error, if
+        any (an asynchronous forward reports errors here or in the next
+        call that waits for it)."""
+        self.timings()
 
     def timings(self):
         """Device milliseconds of the last forward and backward pass."""
@@ -137,24 +157,47 @@ class RankZeroScore(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, state: SelectionGradState, sigma, weights):
-        s_star, _ = state.forward(sigma, weights, with_has=False, asynchronous=True)
+        # queued without waiting only when a backward will follow: its
+        # device status is then read by that backward; a forward nothing
+        # differentiates waits and raises at once
+        grad = bool(ctx.needs_input_grad[1] or ctx.needs_input_grad[2])
+        s_star, _ = state.forward(sigma, weights, with_has=False, asynchronous=grad)
         ctx.state = state
+        ctx.forward_id = state.forward_count
         ctx.sigma_meta = (sigma.dtype, sigma.device)
         return s_star
 
     @staticmethod
     def backward(ctx, grad_s):
-        gw, gs = ctx.state.backward(grad_s)
+        gw, gs = ctx.state.backward(grad_s, forward_id=ctx.forward_id)
         dt, dev = ctx.sigma_meta
         return None, gs.to(device=dev, dtype=dt), gw
 
 
 def rank_zero_scores(state: SelectionGradState, sigma, weights):
-    """s* per precursor (rows in candidate-table order), differentiable."""
+    """s* per precursor (rows in candidate-table order), differentiable.
+
+    When sigma or the weights require a gradient, the forward is queued on
+    torch's current stream without waiting: a device status error (for
+    example a precursor in more than four isolation windows) is raised by
+    the backward, by ``state.settle()`` or by the next call on the state
+    that waits.  Otherwise the forward waits and raises at once.  One
+    backward per forward: a second forward on the same state before the
+    first one's backward makes that backward raise."""
     return RankZeroScore.apply(state, sigma, weights)
 
 
+# id(targets) -> (weakrefs to the targets and decoys tensors, versions, row
+# count) of a checked pair list; only tensors used as passed in are kept, and
+# an entry is dropped with its targets tensor, so a new tensor in a reused
+# block is checked again
 _checked_pairs: dict = {}
+
+
+def _forget(key):
+    def drop(_ref):
+        _checked_pairs.pop(key, None)
+    return drop
 
 
 def _pair_indices(t, device) -> torch.Tensor:
@@ -163,24 +206,28 @@ def _pair_indices(t, device) -> torch.Tensor:
     return t.to(device=device, dtype=torch.int64).contiguous()
 
 
-def _check_pairs(t: torch.Tensor, d: torch.Tensor, n_rows: int) -> None:
-    """Raise on an index outside [0, n_rows); an index tensor pair is
-    checked once while its storage and version stay the same (the kernel
-    itself skips such pairs, so a stale entry cannot make it read out of
-    bounds)."""
+def _check_pairs(t: torch.Tensor, d: torch.Tensor, n_rows: int,
+                 cacheable: bool) -> None:
+    """Raise on an index outside [0, n_rows).  A pair of index tensors the
+    caller passed as they are (no conversion) is checked once while both
+    tensor objects live and their versions stay the same; converted
+    temporaries are checked every time (the kernel itself skips pairs out
+    of range, so it never reads out of bounds)."""
     if t.shape != d.shape or t.dim() != 1:
         raise ValueError("targets and decoys must be 1-D and of equal length")
-    key = (t.data_ptr(), t._version, d.data_ptr(), d._version, t.numel(), n_rows)
-    if key in _checked_pairs:
-        return
+    if cacheable:
+        hit = _checked_pairs.get(id(t))
+        if (hit is not None and hit[0]() is t and hit[1]() is d
+                and hit[2:] == (t._version, d._version, n_rows)):
+            return
     if t.numel():
         lo = torch.minimum(t.min(), d.min())
         hi = torch.maximum(t.max(), d.max())
         if bool(lo < 0) or bool(hi >= n_rows):
             raise ValueError(f"pair indices must lie in [0, {n_rows})")
-    if len(_checked_pairs) > 64:
-        _checked_pairs.clear()
-    _checked_pairs[key] = True
+    if cacheable:
+        _checked_pairs[id(t)] = (weakref.ref(t, _forget(id(t))), weakref.ref(d),
+                                 t._version, d._version, n_rows)
 
 
 class HingeLoss(torch.autograd.Function):
@@ -199,7 +246,7 @@ class HingeLoss(torch.autograd.Function):
         n = s.numel()
         t = _pair_indices(targets, s.device)
         d = _pair_indices(decoys, s.device)
-        _check_pairs(t, d, n)
+        _check_pairs(t, d, n, cacheable=t is targets and d is decoys)
         terms = torch.empty(t.numel(), dtype=torch.float32, device=s.device)
         g = torch.zeros(n, dtype=torch.float32, device=s.device)
         stream = C.c_void_p(torch.cuda.current_stream(s.device).cuda_stream)

@@ -1183,6 +1183,9 @@ void select_kernel(SelArgs a) {
           // The item list's LDS is unused during extraction: slots 0-63
           // hold the tile's fragment-start markers, then each fragment's
           // first group, packed bounds and first bin, indexed by fragment.
+          static_assert(kItemCap * 2 >= 4 * kWave * 4,
+                        "the item list's LDS holds the tile markers and the "
+                        "fragment table during extraction");
           uint32_t *const mk = reinterpret_cast<uint32_t *>(l.items);
           uint32_t *const ft = mk + kWave;
           if (lane < (int)FN) {
@@ -1205,6 +1208,8 @@ void select_kernel(SelArgs a) {
               tb0[u] = 0;
               if (v0 < NG) {
                 const uint32_t vmax = min(v0 + (uint32_t)kWave - 1, NG - 1);
+                // v0 < NG: the first non-empty fragment of [fa, fb) has
+                // gp = 0 <= v0 (the ones before it are empty), so lo_m != 0
                 const uint64_t lo_m = ballot64(lane < (int)FN && ng > 0 && gp <= v0);
                 const uint32_t flo = 63u - (uint32_t)__builtin_clzll(lo_m);
                 // Fragment of this lane's group: the last non-empty one

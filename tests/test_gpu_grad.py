@@ -264,3 +264,45 @@ def test_async_forward_matches_sync_and_defers_status(env):
     st2.forward(sig, w, asynchronous=True)  # queued; the status is read later
     with pytest.raises(N.AdsrError, match="isolation windows"):
         st2.backward(torch.zeros(lib.num_precursors, device="cuda"))
+
+
+def test_backward_of_an_earlier_forward_raises(env):
+    """The state holds the last forward's apexes and fragment selections: a
+    backward through an earlier forward on the same state raises instead of
+    returning the later forward's gradient; the last one's still works."""
+    torch, ads, gen = env["torch"], env["ads"], env["gen"]
+    from alphadia_search_rs.autograd import SelectionGradState, rank_zero_scores
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    st = SelectionGradState(d, lib, _params(gen))
+    sig = torch.tensor(env["sigma"], requires_grad=True)
+    w = torch.tensor(env["weights"], device="cuda", requires_grad=True)
+    s1 = rank_zero_scores(st, sig, w)
+    s2 = rank_zero_scores(st, sig * 1.01, w)
+    with pytest.raises(RuntimeError, match="ran on this SelectionGradState since"):
+        s1.sum().backward()
+    s2.sum().backward()
+    assert w.grad is not None and int((w.grad != 0).sum()) > 0
+
+
+def test_forward_without_grad_raises_at_once(env):
+    """A forward nothing differentiates waits for the device: its status
+    error (a precursor in more than four isolation windows) is raised by
+    rank_zero_scores itself, and settle() raises a deferred one."""
+    torch, ads, gen = env["torch"], env["ads"], env["gen"]
+    import alphadia_search_rs._native as N
+    from alphadia_search_rs.autograd import SelectionGradState, rank_zero_scores
+    dia = list(gen["dia"])
+    dia[1] = np.full_like(dia[1], 400.0)
+    dia[2] = np.full_like(dia[2], 1000.0)
+    d2 = ads.DIAData.from_arrays(*dia)
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    st = SelectionGradState(d2, lib, _params(gen))
+    sig = torch.tensor(env["sigma"])
+    w = torch.tensor(env["weights"], device="cuda")
+    with torch.no_grad():
+        with pytest.raises(N.AdsrError, match="isolation windows"):
+            rank_zero_scores(st, sig, w)
+    st.forward(sig, w, asynchronous=True)
+    with pytest.raises(N.AdsrError, match="isolation windows"):
+        st.settle()

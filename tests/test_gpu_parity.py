@@ -476,3 +476,92 @@ def test_concurrent_searches_match_sequential(ads, run0):
     for i in range(len(params)):
         assert got[i] is not None, f"thread {i} did not finish"
         assert_tables_equal(got[i], want[i], f"concurrent variant {variants[i]}")
+
+
+def _dense_fragment_run():
+    """40 selected fragments per precursor, 20 of them on masses with a peak
+    in each of cycles 20-29 (one or two 8-entry CSR groups per fragment) and
+    20 at 0.5 Th above them, where no peak lies (empty CSR ranges, or a
+    stray background peak), interleaved in m/z order: many fragments start
+    inside one 64-group extraction tile, with empty ones between them and
+    at tile edges."""
+    rng = np.random.default_rng(31)
+    C, W, K = 60, 2, 30
+    win_lo = np.array([400.0, 700.0], np.float32)
+    win_hi = np.array([700.0, 1000.0], np.float32)
+    planted = np.sort(rng.uniform(300.0, 1900.0, 20)).astype(np.float32)
+    spec_mz, spec_in, starts, stops = [], [], [], []
+    pos = 0
+    for c in range(C):
+        for w in range(W):
+            mz = rng.uniform(150.0, 2000.0, K).astype(np.float32)
+            it = rng.lognormal(6.0, 1.5, K).astype(np.float32)
+            if 20 <= c < 30 and w == 0:
+                prof = np.float32(np.exp(-0.5 * ((c - 24.5) / 2.0) ** 2))
+                mz = np.concatenate([mz, planted])
+                it = np.concatenate([it, (rng.uniform(500, 5000, 20) * prof).astype(np.float32)])
+            o = np.argsort(mz, kind="stable")
+            spec_mz.append(mz[o])
+            spec_in.append(it[o])
+            starts.append(pos)
+            pos += len(mz)
+            stops.append(pos)
+    n_spec = C * W
+    idx = np.arange(n_spec)
+    dia = ((idx % W).astype(np.int64), win_lo[idx % W], win_hi[idx % W],
+           np.array(starts, np.int64), np.array(stops, np.int64),
+           (idx // W).astype(np.int64), (idx // W).astype(np.float32),
+           np.concatenate(spec_mz).astype(np.float32),
+           np.concatenate(spec_in).astype(np.float32),
+           np.zeros((W, 1, 1, 1), np.float32))
+    P, F = 64, 40
+    fmz = np.empty((P, F), np.float32)
+    fmz[:, 0::2] = planted
+    fmz[:, 1::2] = planted + np.float32(0.5)  # between planted masses, no peak
+    fint = rng.uniform(0.01, 1.0, (P, F)).astype(np.float32)
+    NF = P * F
+    fs = (np.arange(P) * F).astype(np.uint64)
+    local = np.tile(np.arange(F), P)
+    ones = np.ones(NF, np.uint8)
+    lib = (np.arange(P, dtype=np.uint64), np.full(P, 550.0, np.float32),
+           np.full(P, 550.0, np.float32), np.full(P, 25.0, np.float32),
+           np.full(P, 25.0, np.float32), np.full(P, 10, np.uint8), fs,
+           fs + np.uint64(F), fmz.reshape(-1), fmz.reshape(-1), fint.reshape(-1), ones,
+           ones.copy(), np.zeros(NF, np.uint8), (2 + local // 2).astype(np.uint8),
+           ones.copy(), np.where(local % 2 == 0, 98, 121).astype(np.uint8))
+    return dia, lib
+
+
+def test_many_fragments_per_tile_with_empty_ones_bitwise(ads):
+    import ctypes
+    import alphadia_search_rs._native as N
+    dia, lib_arrays = _dense_fragment_run()
+    cfg = dict(fwhm_rt=4.7096, kernel_size=20, peak_length=5, mass_tolerance=5.0,
+               rt_tolerance=100.0, candidate_count=3, top_k_fragments=40)
+    d = ads.DIAData.from_arrays(*dia)
+    lib = ads.SpecLibFlat.from_arrays(*lib_arrays)
+    p = ads.SelectionParameters()
+    p.update(cfg)
+    n = lib.num_precursors
+    stride = 64
+    scores = np.zeros((n, stride), np.float32)
+    cs = np.zeros(n, np.uint64)
+    ce = np.zeros(n, np.uint64)
+    c = p._c()
+    N.check(N.lib.adsr_select_scores(d._h, lib._h, ctypes.byref(c), 0, n, stride,
+                                     N.ptr(scores), N.ptr(cs), N.ptr(ce)))
+    od, ol = oracle.Dia(dia), oracle.Lib(lib_arrays)
+    prm = oracle.params(**cfg)
+    for r in range(n):
+        one = oracle.select_one(od, ol, prm, r)
+        assert one is not None and one["fragment_mz"].size == 40
+        L = one["cycle_stop"] - one["cycle_start"]
+        hit = one["xic"].any(axis=1)
+        assert hit.sum() >= 20 and (~hit).sum() >= 15  # planted hit, the others empty
+        got = scores[r, :L]
+        bad = np.flatnonzero(got.view(np.uint32) != one["score"].view(np.uint32))
+        assert bad.size == 0, f"row {r}: score differs at cycles {bad[:8]}"
+    got = ads.PeakGroupSelection(p).search(d, lib).to_arrays()
+    want = oracle.select(od, ol, prm)
+    assert len(want[0]) > 0
+    assert_tables_equal(got, want, "40 fragments, empty ones interleaved")

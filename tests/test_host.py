@@ -338,3 +338,36 @@ def test_candidate_parquet_round_trip(tmp_path):
     empty = ads.CandidateCollection()
     empty.to_parquet(tmp_path / "empty.parquet")
     assert ads.CandidateCollection.read_parquet(tmp_path / "empty.parquet").len() == 0
+
+
+def test_pair_check_cache_is_per_live_tensor():
+    """hinge_loss checks pair indices against the row count once per pair of
+    index tensors passed as they are; an entry dies with its tensors, so a
+    new tensor that reuses a freed block (same data pointer, version 0) is
+    checked again, and converted temporaries are never cached."""
+    torch = pytest.importorskip("torch")
+    from alphadia_search_rs.autograd import _check_pairs, _checked_pairs
+    t = torch.arange(10, dtype=torch.int64)
+    d = torch.arange(10, dtype=torch.int64)
+    _check_pairs(t, d, 10, cacheable=True)
+    assert id(t) in _checked_pairs
+    ptr = t.data_ptr()
+    del t, d
+    import gc
+    gc.collect()
+    bad = torch.full((10,), 99, dtype=torch.int64)  # often the freed block again
+    good = torch.arange(10, dtype=torch.int64)
+    with pytest.raises(ValueError, match="pair indices"):
+        _check_pairs(bad, good, 10, cacheable=True)
+    # an in-place change bumps the version: checked again
+    t2 = torch.arange(10, dtype=torch.int64)
+    d2 = torch.arange(10, dtype=torch.int64)
+    _check_pairs(t2, d2, 10, cacheable=True)
+    t2[3] = 50
+    with pytest.raises(ValueError, match="pair indices"):
+        _check_pairs(t2, d2, 10, cacheable=True)
+    # not cacheable: nothing kept
+    t3 = torch.arange(10, dtype=torch.int64)
+    _check_pairs(t3, t3.clone(), 10, cacheable=False)
+    assert id(t3) not in _checked_pairs
+    assert ptr >= 0
