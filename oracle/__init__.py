@@ -366,14 +366,92 @@ def score(dia: Dia, library: Lib, candidates, mass_tolerance=7.0, top_k_fragment
     return out
 
 
+# A planted mistake in the quantification layout below, for the tests that
+# show the layout's pins catch it (0: none; set only by those tests).
+LAYOUT_MUTANT = 0
+
+_PQ_FRAGMENT_KEYS = ("fragment_mz_library", "fragment_mz", "fragment_intensity",
+                     "fragment_cardinality", "fragment_charge", "fragment_loss_type",
+                     "fragment_number", "fragment_position", "fragment_type",
+                     "fragment_mz_observed", "fragment_correlation_observed",
+                     "fragment_mass_error_observed")
+_PQ_PRECURSOR_KEYS = ("precursor_idx", "mz_library", "mz", "rt_library", "rt", "naa",
+                      "rank", "rt_observed")
+
+
+def filter_fragments_by_intensity(pq: dict, intensity_threshold: float):
+    """PrecursorQuantified::filter_fragments_by_intensity
+    (src/precursor_quantified/mod.rs:29-94): the fragments whose intensity is
+    above the threshold (strictly), in order, every fragment column alike;
+    None when no fragment is left.  pq maps the field names of
+    PrecursorQuantified (src/precursor_quantified/mod.rs:6-27) to scalars
+    and per-fragment arrays."""
+    it = np.asarray(pq["fragment_intensity"], np.float32)
+    keep = it >= np.float32(intensity_threshold) if LAYOUT_MUTANT == 1 else \
+        it > np.float32(intensity_threshold)
+    if not keep.any():
+        return None
+    out = {k: pq[k] for k in _PQ_PRECURSOR_KEYS}
+    for k in _PQ_FRAGMENT_KEYS:
+        out[k] = np.asarray(pq[k])[keep]
+    return out
+
+
+def speclib_flat_quantified(pqs):
+    """SpecLibFlatQuantified::from_precursor_quantified_vec
+    (src/speclib_flat_quantified/mod.rs:213-330) and to_dict_arrays
+    (:126-211): fragments concatenated in input order with each precursor's
+    [start, stop) and its index and rank repeated per fragment; then the
+    precursor columns (start and stop included) reordered by a stable sort on
+    precursor_idx, the fragment columns left as they are.  Returns the two
+    dicts of to_dict_arrays."""
+    ptypes = {"precursor_idx": np.uint64, "mz_library": np.float32, "mz": np.float32,
+              "rt_library": np.float32, "rt": np.float32, "naa": np.uint8,
+              "rank": np.uint64, "rt_observed": np.float32,
+              "flat_frag_start_idx": np.uint64, "flat_frag_stop_idx": np.uint64}
+    ftypes = {"mz_library": np.float32, "mz": np.float32, "intensity": np.float32,
+              "cardinality": np.uint8, "charge": np.uint8, "loss_type": np.uint8,
+              "number": np.uint8, "position": np.uint8, "type": np.uint8,
+              "mz_observed": np.float32, "correlation_observed": np.float32,
+              "mass_error_observed": np.float32, "precursor_idx": np.uint64,
+              "rank": np.uint64}
+    prec = {k: [] for k in ptypes}
+    frag = {k: [] for k in ftypes}
+    pos = 0
+    for pq in pqs:
+        for k in _PQ_PRECURSOR_KEYS:
+            prec[k].append(pq[k])
+        nfr = len(pq["fragment_mz"])
+        prec["flat_frag_start_idx"].append(pos)
+        pos += nfr
+        prec["flat_frag_stop_idx"].append(pos)
+        for k in _PQ_FRAGMENT_KEYS:
+            frag[k[len("fragment_"):]].append(np.asarray(pq[k]))
+        frag["precursor_idx"].append(np.full(nfr, pq["precursor_idx"], np.uint64))
+        frag["rank"].append(np.full(nfr, pq["rank"], np.uint64))
+    pout = {k: np.asarray(v, dtype=ptypes[k]) for k, v in prec.items()}
+    fout = {k: (np.concatenate(v).astype(ftypes[k]) if v else np.zeros(0, ftypes[k]))
+            for k, v in frag.items()}
+    if LAYOUT_MUTANT == 2:  # planted: fragments reordered with their precursors
+        srt = np.argsort(pout["precursor_idx"], kind="stable")
+        parts = [np.arange(int(a), int(b)) for a, b in
+                 zip(pout["flat_frag_start_idx"][srt], pout["flat_frag_stop_idx"][srt])]
+        idx = np.concatenate(parts) if parts else np.zeros(0, np.int64)
+        fout = {k: v[idx] for k, v in fout.items()}
+    srt = np.argsort(pout["precursor_idx"], kind="stable")  # sort_by_key, stable
+    pout = {k: v[srt] for k, v in pout.items()}
+    return pout, fout
+
+
 def quantify(dia: Dia, library: Lib, lib_arrays, candidates, tolerance_ppm=7.0,
              top_k_fragments=10000, nthreads=0):
     """PeakGroupQuantification.quantify (src/peak_group_quantification/mod.rs:
     39-168): (precursor dict, fragment dict) as SpecLibFlatQuantified.
     to_dict_arrays (src/speclib_flat_quantified/mod.rs:126-211).  The
-    per-candidate work is the C restatement; the intensity > 0 filter
-    (src/precursor_quantified/mod.rs:26-94) and the flat layout
-    (speclib_flat_quantified/mod.rs:213-330) are restated here."""
+    per-candidate work (:86-166) is the C restatement orc_quantify; the
+    PrecursorQuantified of each candidate, the intensity > 0 filter and the
+    flat layout are filter_fragments_by_intensity and
+    speclib_flat_quantified above."""
     (pidx, pmzl, pmz, prtl, prt, naa, fs, fe, fmzl, fmz, fint, card, chg, loss,
      fnum, fpos, ftype) = lib_arrays
     order = np.argsort(pidx, kind="stable")  # SpecLibFlat's stable sort
@@ -392,55 +470,27 @@ def quantify(dia: Dia, library: Lib, lib_arrays, candidates, tolerance_ppm=7.0,
                        p(k[0]), p(k[1]), p(k[2]), p(k[3]), ks, p(row), p(nf), p(raw),
                        p(inten), p(mzo), p(corr), p(err), p(rto),
                        nthreads or (os.cpu_count() or 1))
-    prec = {key: [] for key in ("precursor_idx", "mz_library", "mz", "rt_library", "rt",
-                                "naa", "rank", "rt_observed", "flat_frag_start_idx",
-                                "flat_frag_stop_idx")}
-    frag = {key: [] for key in ("mz_library", "mz", "intensity", "cardinality", "charge",
-                                "loss_type", "number", "position", "type", "mz_observed",
-                                "correlation_observed", "mass_error_observed",
-                                "precursor_idx", "rank")}
-    pos = 0
     none = np.iinfo(np.uintp).max
+    pqs = []
     for i in range(n):
         if row[i] == none:
             continue
         sl = slice(i * ks, i * ks + int(nf[i]))
-        keep = np.flatnonzero(inten[sl] > 0.0)
-        if keep.size == 0:  # filter_fragments_by_intensity -> None
-            continue
         r = int(order[int(row[i])])  # the row in the caller's array order
-        rw = raw[sl][keep].astype(np.int64)
-        for key, src in (("precursor_idx", pidx), ("mz_library", pmzl), ("mz", pmz),
-                         ("rt_library", prtl), ("rt", prt), ("naa", naa)):
-            prec[key].append(src[r])
-        prec["rank"].append(rank[i])
-        prec["rt_observed"].append(rto[i])
-        prec["flat_frag_start_idx"].append(pos)
-        pos += keep.size
-        prec["flat_frag_stop_idx"].append(pos)
-        for key, src in (("mz_library", fmzl), ("mz", fmz), ("cardinality", card),
-                         ("charge", chg), ("loss_type", loss), ("number", fnum),
-                         ("position", fpos), ("type", ftype)):
-            frag[key].append(np.asarray(src)[rw])
-        frag["intensity"].append(inten[sl][keep])
-        frag["mz_observed"].append(mzo[sl][keep])
-        frag["correlation_observed"].append(corr[sl][keep])
-        frag["mass_error_observed"].append(err[sl][keep])
-        frag["precursor_idx"].append(np.full(keep.size, pidx[r], np.uint64))
-        frag["rank"].append(np.full(keep.size, rank[i], np.uint64))
-    ptypes = {"precursor_idx": np.uint64, "mz_library": np.float32, "mz": np.float32,
-              "rt_library": np.float32, "rt": np.float32, "naa": np.uint8,
-              "rank": np.uint64, "rt_observed": np.float32,
-              "flat_frag_start_idx": np.uint64, "flat_frag_stop_idx": np.uint64}
-    pout = {key: np.asarray(v, dtype=ptypes[key]) for key, v in prec.items()}
-    srt = np.argsort(pout["precursor_idx"], kind="stable")  # sort_by_key, stable
-    pout = {key: v[srt] for key, v in pout.items()}
-    ftypes = {"mz_library": np.float32, "mz": np.float32, "intensity": np.float32,
-              "cardinality": np.uint8, "charge": np.uint8, "loss_type": np.uint8,
-              "number": np.uint8, "position": np.uint8, "type": np.uint8,
-              "mz_observed": np.float32, "correlation_observed": np.float32,
-              "mass_error_observed": np.float32, "precursor_idx": np.uint64,
-              "rank": np.uint64}
-    fout = {key: (np.concatenate(v).astype(ftypes[key]) if v else np.zeros(0, ftypes[key]))
-            for key, v in frag.items()}
-    return pout, fout
+        rw = raw[sl].astype(np.int64)
+        pq = {"precursor_idx": pidx[r], "mz_library": pmzl[r], "mz": pmz[r],
+              "rt_library": prtl[r], "rt": prt[r], "naa": naa[r], "rank": rank[i],
+              "rt_observed": rto[i]}
+        for key, src in (("fragment_mz_library", fmzl), ("fragment_mz", fmz),
+                         ("fragment_cardinality", card), ("fragment_charge", chg),
+                         ("fragment_loss_type", loss), ("fragment_number", fnum),
+                         ("fragment_position", fpos), ("fragment_type", ftype)):
+            pq[key] = np.asarray(src)[rw]
+        pq["fragment_intensity"] = inten[sl]
+        pq["fragment_mz_observed"] = mzo[sl]
+        pq["fragment_correlation_observed"] = corr[sl]
+        pq["fragment_mass_error_observed"] = err[sl]
+        kept = filter_fragments_by_intensity(pq, 0.0)  # mod.rs:166
+        if kept is not None:
+            pqs.append(kept)
+    return speclib_flat_quantified(pqs)

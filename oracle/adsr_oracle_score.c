@@ -612,7 +612,11 @@ void orc_score(const orc_dia *d, const orc_lib *l, float mass_tolerance,
  * fragment order) and the observed intensity (sum over cycles), m/z,
  * correlation with the filtered median profile and mass error (ppm).  The
  * intensity > 0 filter (precursor_quantified/mod.rs:26-94) and the flat
- * layout are left to the caller. */
+ * layout are left to the caller.
+ * Pinned by tests/test_oracle_quant.py: an independent float64 numpy
+ * implementation on 300 rank-0 candidates of workload 0 (intensity within
+ * L * 2^-24, correlation within 1e-4, m/z and ppm error bit for bit, the
+ * rt, the filter and the layout exactly), with planted mistakes 13-15. */
 typedef struct {
   const orc_dia *d;
   const orc_lib *l;
@@ -666,15 +670,26 @@ static void quantify_one(const quant_ctx *q, size_t ci) {
     orc_normalize_profiles(xic, F, L, 1, norm);
     size_t Ff = orc_filter_non_zero(norm, F, L, filt);
     orc_median_axis_0(filt, Ff, L, med);
+#if ORC_MUTANT == 14 /* planted: median over every profile, not the non-zero ones */
+    orc_median_axis_0(norm, F, L, med);
+#endif
     for (size_t f = 0; f < F; f++) {
       size_t s = ci * ks + f;
       raw[s] = fs + ord[f];
       corr[s] = orc_correlation_safe(norm + f * L, med, L); /* :121-128 */
+#if ORC_MUTANT == 13 /* planted: the last cycle left out of the sum */
+      intensity[s] = orc_ndarray_sum(xic + f * L, L - 1);
+#else
       intensity[s] = orc_ndarray_sum(xic + f * L, L);      /* :131 */
+#endif
       mz_obs[s] = mo[f];
       mass_err[s] = er[f];
     }
+#if ORC_MUTANT == 15 /* planted: RT of the first cycle instead of the centre */
+    rt_observed[ci] = d->rt[c0];
+#else
     rt_observed[ci] = d->rt[cc];
+#endif
     row[ci] = lo;
     nf[ci] = F;
     free(ord); free(fmz); free(xic); free(dmz); free(norm); free(filt);

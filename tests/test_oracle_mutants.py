@@ -11,6 +11,7 @@ import pytest
 
 import oracle
 import test_oracle as T
+import test_oracle_quant as TQ
 import test_oracle_score as TS
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -41,6 +42,10 @@ MUTANTS = {
     10: ("document frequency not floored at 1", [TS.test_idf_kats]),
     11: ("unweighted running m/z", [TS.test_dense_xic_mz_kats]),
     12: ("profiles normalised by the centre cycle only", ["features"]),
+    # quantification (orc_quantify); "quantify" is its float64 check
+    13: ("observed intensity without the last cycle", ["quantify"]),
+    14: ("median over every profile, the all-zero ones too", ["quantify"]),
+    15: ("rt_observed of the first cycle", ["quantify"]),
 }
 
 
@@ -60,9 +65,12 @@ def builds(tmp_path_factory):
 
 
 def _run_check(check, run0):
+    # handles built by the library loaded now (a mutant or the oracle)
     if check == "features":
-        # handles built by the library loaded now (a mutant or the oracle)
         TS.test_score_features_against_float64_implementation(
+            run0, (oracle.Dia(run0["dia"]), oracle.Lib(run0["lib"])))
+    elif check == "quantify":
+        TQ.test_quantify_against_float64_implementation(
             run0, (oracle.Dia(run0["dia"]), oracle.Lib(run0["lib"])))
     else:
         check()
