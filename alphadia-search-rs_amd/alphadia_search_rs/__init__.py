@@ -490,12 +490,12 @@ class CandidateCollection:
             check(N.lib.adsr_candidates_export(self._h, *[ptr(a) for a in cols]))
         return tuple(cols)
 
-    # The file hand-off of scripts/candidate_selection.py:121-209 writes the
-    # nine to_arrays columns to parquet (there renamed frame_* and rescaled
-    # by the run's cycle length, with the library's elution group and decoy
-    # columns merged in, which is alphaDIA's side).  Here the columns keep
-    # their to_arrays names and dtypes, so that a table read back is the
-    # same CandidateCollection.
+    # Two file hand-offs.  to_arrow/to_parquet keep the nine to_arrays
+    # columns, names and dtypes, so that a table read back is the same
+    # CandidateCollection.  to_candidates_frame/to_candidates_parquet (below)
+    # write the table of scripts/candidate_selection.py:121-209 instead:
+    # frame_* names, rescaled by the run's cycle length, with the library's
+    # elution group and decoy columns merged in.
     COLUMNS = ("precursor_idx", "rank", "score", "scan_center", "scan_start",
                "scan_stop", "cycle_center", "cycle_start", "cycle_stop")
 
@@ -523,6 +523,47 @@ class CandidateCollection:
     def read_parquet(path) -> "CandidateCollection":
         import pyarrow.parquet as pq
         return CandidateCollection.from_arrow(pq.read_table(path))
+
+    # scripts/candidate_selection.py:121-168 (parse_candidates): the table
+    # alphaDIA reads.  The cycle columns become frame_*, multiplied by the
+    # run's cycle length (spectra per cycle); scan_start/center are 0 and
+    # scan_stop 1; the library's elution_group_idx and decoy columns are
+    # joined by precursor_idx (a left merge: every candidate row kept, in
+    # order).
+    SCRIPT_COLUMNS = ("precursor_idx", "rank", "score", "scan_center", "scan_start",
+                      "scan_stop", "frame_center", "frame_start", "frame_stop")
+
+    def to_candidates_frame(self, precursor_df, cycle_len):
+        """The pandas DataFrame of parse_candidates: the nine columns named
+        as in SCRIPT_COLUMNS, merged (how="left", on precursor_idx) with
+        ``precursor_df[["precursor_idx", "elution_group_idx", "decoy"]]``,
+        frame_* multiplied by ``cycle_len``, scan_start = scan_center = 0 and
+        scan_stop = 1.  ``precursor_df`` is a DataFrame or a mapping of
+        columns; ``cycle_len`` is ``spectrum_df["cycle_idx"].max() + 1``.
+        The arithmetic is the script's own pandas expressions, so the
+        resulting dtypes follow the type of ``cycle_len`` as they do there."""
+        import pandas as pd
+        df = pd.DataFrame(dict(zip(self.SCRIPT_COLUMNS, self.to_arrays())))
+        lib = precursor_df if isinstance(precursor_df, pd.DataFrame) \
+            else pd.DataFrame(dict(precursor_df))
+        missing = [c for c in ("precursor_idx", "elution_group_idx", "decoy")
+                   if c not in lib.columns]
+        if missing:
+            raise ValueError(f"precursor_df lacks columns {missing}")
+        df = df.merge(lib[["precursor_idx", "elution_group_idx", "decoy"]],
+                      on="precursor_idx", how="left")
+        df["frame_start"] = df["frame_start"] * cycle_len
+        df["frame_stop"] = df["frame_stop"] * cycle_len
+        df["frame_center"] = df["frame_center"] * cycle_len
+        df["scan_start"] = 0
+        df["scan_stop"] = 1
+        df["scan_center"] = 0
+        return df
+
+    def to_candidates_parquet(self, path, precursor_df, cycle_len) -> None:
+        """Writes ``to_candidates_frame(precursor_df, cycle_len)`` to a
+        parquet file, as scripts/candidate_selection.py:203-206 does."""
+        self.to_candidates_frame(precursor_df, cycle_len).to_parquet(path)
 
 
 class DeviceCandidateCollection:

@@ -340,6 +340,55 @@ def test_candidate_parquet_round_trip(tmp_path):
     assert ads.CandidateCollection.read_parquet(tmp_path / "empty.parquet").len() == 0
 
 
+@pytest.mark.parametrize("cycle_len", [np.int64(7), 7], ids=["numpy-int64", "int"])
+def test_script_candidate_frame(tmp_path, cycle_len):
+    """scripts/candidate_selection.py:121-168: frame_* = cycle_* x cycle
+    length, scan_start = scan_center = 0, scan_stop = 1, the library's
+    elution_group_idx and decoy joined by precursor_idx with every
+    candidate row kept in order (a precursor missing from the library gets
+    NaN), and the parquet file reads back equal."""
+    import pandas as pd
+    import alphadia_search_rs as ads
+    rng = np.random.default_rng(5)
+    n = 500
+    pidx = rng.integers(0, 300, n).astype(np.uint64)
+    cyc = [rng.integers(0, 5000, n).astype(np.uint64) for _ in range(3)]
+    cols = [pidx, rng.integers(0, 3, n).astype(np.uint64),
+            rng.standard_normal(n).astype(np.float32),
+            np.zeros(n, np.uint64), np.zeros(n, np.uint64), np.zeros(n, np.uint64),
+            *cyc]
+    c = ads.CandidateCollection.from_arrays(*cols)
+    lib_idx = rng.permutation(np.arange(290, dtype=np.uint64))  # 290..299 absent
+    lib = pd.DataFrame({"precursor_idx": lib_idx,
+                        "elution_group_idx": (lib_idx // 2).astype(np.int64),
+                        "decoy": (lib_idx % 2).astype(np.int8),
+                        "charge": np.ones(290, np.int8)})
+    df = c.to_candidates_frame(lib, cycle_len)
+    assert list(df.columns) == list(ads.CandidateCollection.SCRIPT_COLUMNS) + [
+        "elution_group_idx", "decoy"]
+    assert len(df) == n
+    assert np.array_equal(df["precursor_idx"].to_numpy(), pidx)
+    assert np.array_equal(df["score"].to_numpy(), cols[2])
+    for name, v in zip(("frame_center", "frame_start", "frame_stop"), cyc):
+        got = df[name].to_numpy()
+        assert got.dtype == (pd.Series(v) * cycle_len).dtype
+        assert np.array_equal(got.astype(np.float64), v.astype(np.float64) * 7)
+    assert (df["scan_start"] == 0).all() and (df["scan_center"] == 0).all()
+    assert (df["scan_stop"] == 1).all()
+    known = pidx < 290
+    assert np.array_equal(df["elution_group_idx"].to_numpy()[known],
+                          (pidx[known] // 2).astype(np.float64))
+    assert np.array_equal(df["decoy"].to_numpy()[known], (pidx[known] % 2).astype(np.float64))
+    assert df["elution_group_idx"][~known].isna().all()
+    path = tmp_path / "candidates.parquet"
+    c.to_candidates_parquet(path, lib, cycle_len)
+    pd.testing.assert_frame_equal(pd.read_parquet(path), df)
+    with pytest.raises(ValueError, match="decoy"):
+        c.to_candidates_frame(lib.drop(columns=["decoy"]), cycle_len)
+    dict_lib = {k: lib[k].to_numpy() for k in lib.columns}
+    pd.testing.assert_frame_equal(c.to_candidates_frame(dict_lib, cycle_len), df)
+
+
 def test_pair_check_cache_is_per_live_tensor():
     """hinge_loss checks pair indices against the row count once per pair of
     index tensors passed as they are; an entry dies with its tensors, so a
