@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libadsr.so")
+# ADSR_LIBRARY selects another build of the same library (the sanitizer
+# build of the CPU tests); by default the in-tree libadsr.so.
+LIB_PATH = os.environ.get("ADSR_LIBRARY") or os.path.join(_HERE, "libadsr.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(

@@ -64,9 +64,25 @@ enum StatusBits : uint32_t {
   do {                                                                  \
     hipError_t _e = (expr);                                             \
     if (_e != hipSuccess)                                               \
-      return set_error(ADSR_ERR_DEVICE, "%s failed: %s", #expr,         \
-                       hipGetErrorString(_e));                          \
+      return set_error(_e == hipErrorOutOfMemory ? ADSR_ERR_OUT_OF_MEMORY \
+                                                 : ADSR_ERR_DEVICE,     \
+                       "%s failed: %s", #expr, hipGetErrorString(_e));  \
   } while (0)
+
+// Every device allocation of the library: hipMalloc, or out of memory for
+// the allocation ADSR_FAIL_ALLOC names.
+template <class T>
+static hipError_t dev_malloc(T **p, size_t bytes) {
+  if (alloc_should_fail()) {
+    *p = nullptr;
+    return hipErrorOutOfMemory;
+  }
+  return hipMalloc(reinterpret_cast<void **>(p), bytes);
+}
+
+static void *host_malloc(size_t bytes) {
+  return alloc_should_fail() ? nullptr : std::malloc(bytes);
+}
 
 // ---------------------------------------------------------------------------
 // kernel argument blocks
@@ -2264,7 +2280,7 @@ static adsr_status dmalloc_copy(T **dst, const T *src, size_t n, size_t &acc,
                                 size_t pad = 0) {
   *dst = nullptr;
   const size_t bytes = std::max<size_t>(n + pad, 1) * sizeof(T);
-  HIP_TRY(hipMalloc((void **)dst, bytes));
+  HIP_TRY(dev_malloc((void **)dst, bytes));
   if (n) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
   if (pad) HIP_TRY(hipMemset(*dst + n, 0xff, pad * sizeof(T)));
   acc += bytes;
@@ -2475,6 +2491,10 @@ struct Workspace {
 };
 
 static void ws_free(Workspace &w) {
+  // work of a failed search may still be queued: it ends before its
+  // buffers are freed
+  if (w.stream2) (void)hipStreamSynchronize(w.stream2);
+  if (w.stream) (void)hipStreamSynchronize(w.stream);
   if (w.graph) (void)hipGraphExecDestroy(w.graph);
   if (w.graph_src) (void)hipGraphDestroy(w.graph_src);
   (void)hipFree(w.rec);
@@ -2563,23 +2583,23 @@ static adsr_status ws_alloc(Workspace &w, size_t n_rows, size_t ks, size_t cc,
     HIP_TRY(hipEventCreateWithFlags(&w.ev_fork, hipEventDisableTiming));
     for (auto &e : w.ev_plan) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
-  HIP_TRY(hipMalloc(&w.rec, R * sizeof(uint4)));
-  HIP_TRY(hipMalloc(&w.obs, R * kObsMax * 2));
-  HIP_TRY(hipMalloc(&w.sel, R * ks * 2));
-  HIP_TRY(hipMalloc(&w.sel_mz, R * ks * 4));
-  HIP_TRY(hipMalloc(&w.sel_w, R * ks * 4));
-  HIP_TRY(hipMalloc(&w.rng, R * kObsMax * ks * sizeof(uint2)));
-  HIP_TRY(hipMalloc(&w.b0, R * kObsMax * ks * 4));
-  HIP_TRY(hipMalloc(&w.cand_score, R * std::max<size_t>(cc, 1) * 4));
-  HIP_TRY(hipMalloc(&w.cand_center, R * std::max<size_t>(cc, 1) * 4));
-  HIP_TRY(hipMalloc(&w.cand_count, R * 4));
-  HIP_TRY(hipMalloc(&w.local, R * 4));
-  HIP_TRY(hipMalloc(&w.bsum, nb * 4));
-  HIP_TRY(hipMalloc(&w.small, kSmallWords * 4));
-  HIP_TRY(hipMalloc(&w.out, R * std::max<size_t>(cc, 1) * sizeof(CandRow)));
-  HIP_TRY(hipMalloc(&w.kernels, nker * 4));
-  HIP_TRY(hipMalloc(&w.rt_tol_pp, (pp ? R : 1) * 4));
-  HIP_TRY(hipMalloc(&w.key_scratch, std::max<size_t>(n_frag_scratch, 1) * 4));
+  HIP_TRY(dev_malloc(&w.rec, R * sizeof(uint4)));
+  HIP_TRY(dev_malloc(&w.obs, R * kObsMax * 2));
+  HIP_TRY(dev_malloc(&w.sel, R * ks * 2));
+  HIP_TRY(dev_malloc(&w.sel_mz, R * ks * 4));
+  HIP_TRY(dev_malloc(&w.sel_w, R * ks * 4));
+  HIP_TRY(dev_malloc(&w.rng, R * kObsMax * ks * sizeof(uint2)));
+  HIP_TRY(dev_malloc(&w.b0, R * kObsMax * ks * 4));
+  HIP_TRY(dev_malloc(&w.cand_score, R * std::max<size_t>(cc, 1) * 4));
+  HIP_TRY(dev_malloc(&w.cand_center, R * std::max<size_t>(cc, 1) * 4));
+  HIP_TRY(dev_malloc(&w.cand_count, R * 4));
+  HIP_TRY(dev_malloc(&w.local, R * 4));
+  HIP_TRY(dev_malloc(&w.bsum, nb * 4));
+  HIP_TRY(dev_malloc(&w.small, kSmallWords * 4));
+  HIP_TRY(dev_malloc(&w.out, R * std::max<size_t>(cc, 1) * sizeof(CandRow)));
+  HIP_TRY(dev_malloc(&w.kernels, nker * 4));
+  HIP_TRY(dev_malloc(&w.rt_tol_pp, (pp ? R : 1) * 4));
+  HIP_TRY(dev_malloc(&w.key_scratch, std::max<size_t>(n_frag_scratch, 1) * 4));
   return ADSR_OK;
 }
 
@@ -2648,7 +2668,9 @@ void *pinned_acquire(size_t bytes, size_t *cap) {
     }
   }
   void *p = nullptr;
-  if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+  if (alloc_should_fail() ||
+      hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess)
+    return nullptr;
   *cap = want;
   return p;
 }
@@ -3084,7 +3106,7 @@ static adsr_status queue_table(Workspace &w, const adsr_dia *d, const adsr_lib *
                                const adsr_selection_params *p) {
   const size_t cap = std::max<size_t>(w.n_rows, 1) * std::max<size_t>(w.cc, 1);
   if (!w.table)
-    HIP_TRY(hipMalloc(&w.table, cap * adsr_candidates::kRowBytes));
+    HIP_TRY(dev_malloc(&w.table, cap * adsr_candidates::kRowBytes));
   if (w.n_rows)
     table_kernel<<<(uint32_t)((cap + 255) / 256), 256, 0, w.stream>>>(
         w.out, w.small + 2, l->dev.p_idx, p->peak_length, d->rt_index.size(), cap,
@@ -3105,7 +3127,7 @@ static adsr_status fetch(Workspace &w, const adsr_lib *l, uint32_t total,
   if (!blk) {
     pinned = false;
     cap = std::max<size_t>(bytes, 16);
-    blk = std::malloc(cap);
+    blk = host_malloc(cap);
     if (!blk) return set_error(ADSR_ERR_OUT_OF_MEMORY, "candidate table");
   }
   auto *c = new adsr_candidates();
@@ -3137,7 +3159,7 @@ static adsr_status fetch_device(Workspace &w, const adsr_dia *d,
   c->n = total;
   c->pl = p->peak_length;
   c->nrt = d->rt_index.size();
-  if (hipMalloc((void **)&c->rows, std::max<size_t>(total, 1) * sizeof(DevCand)) !=
+  if (dev_malloc((void **)&c->rows, std::max<size_t>(total, 1) * sizeof(DevCand)) !=
       hipSuccess) {
     delete c;
     return set_error(ADSR_ERR_OUT_OF_MEMORY, "device candidate table");
@@ -3591,7 +3613,7 @@ extern "C" adsr_status adsr_device_candidates_to_host(
   if (!blk) {
     pinned = false;
     cap = std::max<size_t>(bytes, 16);
-    blk = std::malloc(cap);
+    blk = host_malloc(cap);
     if (!blk) return set_error(ADSR_ERR_OUT_OF_MEMORY, "candidate table");
   }
   auto *h = new adsr_candidates();
@@ -3601,7 +3623,7 @@ extern "C" adsr_status adsr_device_candidates_to_host(
   if (n) {
     // the columns expanded on the device, then one copy
     uint64_t *dblk = nullptr;
-    if (hipMalloc(&dblk, bytes) != hipSuccess) {
+    if (dev_malloc(&dblk, bytes) != hipSuccess) {
       delete h;
       return set_error(ADSR_ERR_OUT_OF_MEMORY, "device table block");
     }
@@ -3660,7 +3682,7 @@ extern "C" adsr_status adsr_select_algorithmic_bytes(
   if (!st) st = enqueue(w, sp, n, LaunchShape(), nullptr, 0, 0, 0, false, false, true);
   if (!st) st = check_status(w);
   unsigned long long *acc = nullptr;
-  if (!st && hipMalloc(&acc, 32) != hipSuccess)
+  if (!st && dev_malloc(&acc, 32) != hipSuccess)
     st = set_error(ADSR_ERR_OUT_OF_MEMORY, "accumulator");
   if (!st) {
     unsigned long long zero[4] = {0, 0, 0, 0};
@@ -3711,7 +3733,7 @@ extern "C" adsr_status adsr_select_scores(const adsr_dia *dc,
   const size_t nr = row_end - row_begin;
   float *dbg = nullptr;
   const size_t bytes = std::max<size_t>(nr * max_cycles, 1) * 4;
-  if (hipMalloc(&dbg, bytes) != hipSuccess) {
+  if (dev_malloc(&dbg, bytes) != hipSuccess) {
     ws_free(w);
     return set_error(ADSR_ERR_OUT_OF_MEMORY, "debug buffer");
   }
@@ -3894,13 +3916,13 @@ adsr_status adsr_diff_create(const adsr_dia *dc, const adsr_lib *lc,
     return e;
   };
   if (st) return fail(st);
-  if (hipMalloc(&s->dk, nk * (half + 1) * 8) != hipSuccess ||
-      hipMalloc(&s->contrib, std::max<size_t>(l->n_prec, 1) * 8) != hipSuccess ||
-      hipMalloc(&s->gsig, std::max<size_t>(d->n_obs, 1) * 8) != hipSuccess ||
-      hipMalloc(&s->gpart, std::max<size_t>(d->n_obs, 1) *
+  if (dev_malloc(&s->dk, nk * (half + 1) * 8) != hipSuccess ||
+      dev_malloc(&s->contrib, std::max<size_t>(l->n_prec, 1) * 8) != hipSuccess ||
+      dev_malloc(&s->gsig, std::max<size_t>(d->n_obs, 1) * 8) != hipSuccess ||
+      dev_malloc(&s->gpart, std::max<size_t>(d->n_obs, 1) *
                                ((l->n_prec + kSigmaChunk - 1) / kSigmaChunk + 1) * 8) !=
           hipSuccess ||
-      hipMalloc(&s->bq, 64) != hipSuccess)
+      dev_malloc(&s->bq, 64) != hipSuccess)
     return fail(set_error(ADSR_ERR_OUT_OF_MEMORY, "diff buffers"));
   for (auto &e : s->ev)
     if (hipEventCreate(&e) != hipSuccess)
@@ -4101,8 +4123,8 @@ void adsr_diff_free(adsr_diff *h) {
 template <class F>
 static adsr_status with_buffers(size_t in_bytes, size_t out_bytes, F &&fn) {
   void *din = nullptr, *dout = nullptr;
-  HIP_TRY(hipMalloc(&din, std::max<size_t>(in_bytes, 16)));
-  if (hipMalloc(&dout, std::max<size_t>(out_bytes, 16)) != hipSuccess) {
+  HIP_TRY(dev_malloc(&din, std::max<size_t>(in_bytes, 16)));
+  if (dev_malloc(&dout, std::max<size_t>(out_bytes, 16)) != hipSuccess) {
     (void)hipFree(din);
     return set_error(ADSR_ERR_OUT_OF_MEMORY, "device buffer");
   }
@@ -4645,12 +4667,14 @@ namespace {
 struct BuildBufs {
   std::vector<void *> p;
   ~BuildBufs() {
+    // a build that failed part-way may have kernels queued on them
+    (void)hipDeviceSynchronize();
     for (void *q : p) (void)hipFree(q);
   }
   template <class T>
   adsr_status alloc(T **out, size_t n) {
     *out = nullptr;
-    HIP_TRY(hipMalloc((void **)out, std::max<size_t>(n, 1) * sizeof(T)));
+    HIP_TRY(dev_malloc((void **)out, std::max<size_t>(n, 1) * sizeof(T)));
     p.push_back(*out);
     return ADSR_OK;
   }
@@ -4756,9 +4780,9 @@ static adsr_status build_csr_device(adsr_dia *d,
   BTRY(tmp.alloc(&htot, 1));
   // CSR arrays of the handle
   const size_t len = (size_t)B + 1;
-  HIP_TRY(hipMalloc((void **)&v.slice_starts, std::max<size_t>(n_obs * len, 1) * 4));
-  HIP_TRY(hipMalloc((void **)&v.cycles, (total + 8) * 2));
-  HIP_TRY(hipMalloc((void **)&v.intens, std::max<uint64_t>(total, 1) * 4));
+  HIP_TRY(dev_malloc((void **)&v.slice_starts, std::max<size_t>(n_obs * len, 1) * 4));
+  HIP_TRY(dev_malloc((void **)&v.cycles, (total + 8) * 2));
+  HIP_TRY(dev_malloc((void **)&v.intens, std::max<uint64_t>(total, 1) * 4));
   acc += n_obs * len * 4 + (total + 8) * 2 + total * 4;
   HIP_TRY(hipMemset(v.cycles + total, 0xff, 8 * 2));
   if (n_obs) HIP_TRY(hipMemset(v.slice_starts, 0, n_obs * len * 4));
@@ -5659,7 +5683,7 @@ static adsr_status upload_lib_scoring(adsr_lib *l, const KDia &kd) {
   adsr_status st = dmalloc_copy(&v.p_naa, l->precursor_naa.data(), l->n_prec, acc);
   if (!st) st = dmalloc_copy(&v.f_mzl, l->fragment_mz_library.data(), l->n_frag, acc);
   if (st) return st;
-  HIP_TRY(hipMalloc((void **)&v.idf_counts, (size_t)kd.n_grid * 4));
+  HIP_TRY(dev_malloc((void **)&v.idf_counts, (size_t)kd.n_grid * 4));
   HIP_TRY(hipMemset(v.idf_counts, 0, (size_t)kd.n_grid * 4));
   if (l->n_frag)
     idf_count_kernel<<<(uint32_t)((l->n_frag + 255) / 256), 256>>>(kd, v.f_mzl,

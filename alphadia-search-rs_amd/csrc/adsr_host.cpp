@@ -35,6 +35,20 @@ adsr_status set_error(adsr_status st, const char *fmt, ...) {
 
 void clear_error() { g_last_error.clear(); }
 
+// ADSR_FAIL_ALLOC=n (n >= 1) makes the n-th allocation of the process that
+// goes through this check (device buffers, page-locked and host candidate
+// tables) report out of memory, so that tests can drive every error path
+// of an allocation without exhausting memory; unset or 0: never.
+bool alloc_should_fail() {
+  static const long target = [] {
+    const char *e = std::getenv("ADSR_FAIL_ALLOC");
+    return e ? std::strtol(e, nullptr, 10) : 0L;
+  }();
+  if (target <= 0) return false;
+  static std::atomic<long> count{0};
+  return count.fetch_add(1) + 1 == target;
+}
+
 static inline uint32_t f2u(float x) {
   uint32_t u;
   std::memcpy(&u, &x, 4);
@@ -630,7 +644,9 @@ extern "C" adsr_status adsr_candidates_from_arrays(
   if (n && (!precursor_idx || !rank || !score || !scan_center || !scan_start ||
             !scan_stop || !cycle_center || !cycle_start || !cycle_stop))
     return set_error(ADSR_ERR_INVALID_ARGUMENT, "column arrays are NULL");
-  void *blk = std::malloc(std::max<size_t>(n * adsr_candidates::kRowBytes, 16));
+  void *blk = alloc_should_fail()
+                  ? nullptr
+                  : std::malloc(std::max<size_t>(n * adsr_candidates::kRowBytes, 16));
   if (!blk) return set_error(ADSR_ERR_OUT_OF_MEMORY, "candidate table");
   auto *c = new adsr_candidates();
   c->bind(blk, n);

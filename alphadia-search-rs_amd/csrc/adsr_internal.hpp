@@ -39,6 +39,8 @@ uint32_t grid_find_closest(const Grid &g, float x);
 // ---- errors -----------------------------------------------------------------
 adsr_status set_error(adsr_status st, const char *fmt, ...);
 void clear_error();
+// True for the allocation that ADSR_FAIL_ALLOC names (a test hook).
+bool alloc_should_fail();
 
 // ---- device-resident copies -------------------------------------------------
 struct DeviceDia {

@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# ADSR_ORACLE_LIBRARY selects another build of the oracle (the sanitizer
+# build of the CPU tests); by default oracle/liboracle.so.
+LIB_PATH = os.environ.get("ADSR_ORACLE_LIBRARY") or os.path.join(HERE, "liboracle.so")
 
 vp = C.c_void_p
 sz = C.c_size_t
@@ -207,8 +209,9 @@ class Lib:
     def __init__(self, arrays):
         (pidx, _pmzl, pmz, _prtl, prt, naa, fs, fe, fmzl, fmz, fint,
          _card, _chg, _loss, fnum, _pos, ftype) = arrays
-        k = [np.ascontiguousarray(a) for a in (pidx, pmz, prt, naa, fs, fe, fmz, fmzl,
-                                               fint, fnum, ftype)]
+        # naa and fragment_mz_library are optional: None is passed as NULL
+        k = [None if a is None else np.ascontiguousarray(a)
+             for a in (pidx, pmz, prt, naa, fs, fe, fmz, fmzl, fint, fnum, ftype)]
         self.h = lib().orc_lib_build_full(len(k[0]), p(k[0]), p(k[1]), p(k[2]), p(k[3]),
                                           p(k[4]), p(k[5]), len(k[6]), p(k[6]), p(k[7]),
                                           p(k[8]), p(k[9]), p(k[10]))

@@ -420,3 +420,35 @@ def test_pair_check_cache_is_per_live_tensor():
     _check_pairs(t3, t3.clone(), 10, cacheable=False)
     assert id(t3) not in _checked_pairs
     assert ptr >= 0
+
+
+def _child(code, env_extra):
+    import subprocess
+    import sys
+    env = dict(os.environ, **env_extra)
+    env["PYTHONPATH"] = os.pathsep.join(
+        [os.path.join(ROOT, "alphadia-search-rs_amd"), ROOT, env.get("PYTHONPATH", "")])
+    return subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                          text=True, timeout=120)
+
+
+def test_forced_allocation_failure_reports_out_of_memory():
+    """ADSR_FAIL_ALLOC=n makes the library's n-th allocation fail: the call
+    that makes it returns ADSR_ERR_OUT_OF_MEMORY with a message (MemoryError
+    in Python), and the calls before and after it succeed."""
+    code = """
+import numpy as np, alphadia_search_rs as ads
+cols = [np.arange(5, dtype=np.uint64)] * 2 + [np.ones(5, np.float32)] + [np.arange(5, dtype=np.uint64)] * 6
+out = []
+for i in range(3):
+    try:
+        out.append(str(ads.CandidateCollection.from_arrays(*cols).len()))
+    except MemoryError as e:
+        out.append("MemoryError:" + str(e))
+print("|".join(out))
+"""
+    r = _child(code, {"ADSR_FAIL_ALLOC": "2"})
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "5|MemoryError:candidate table|5"
+    r = _child(code, {"ADSR_FAIL_ALLOC": "0"})
+    assert r.returncode == 0 and r.stdout.strip() == "5|5|5", r.stderr
