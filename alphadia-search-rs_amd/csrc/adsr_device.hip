@@ -1092,15 +1092,16 @@ void select_kernel(SelArgs a) {
       // Hit key: t (12 bits) | fragment (6) << 12 | window (2) << 18.
       uint32_t nh = 0;
       uint32_t gA = 0, gG = 0;  // fragment group being accumulated
-      // Adds hits [0, nh) of fragments [gA, gA + gG) into X; the list holds
-      // intensities (loaded) or window-relative entries of window `base`.
-      auto accumulate = [&](bool loaded, uint64_t base) {
-        for (uint32_t c = 0; c < nh; c += kWave) {
+      // Adds the hits in list slots [h0, h1) of fragments [gA, gA + gG)
+      // into X; the list holds intensities (loaded) or window-relative
+      // entries of window `base`.
+      auto accumulate = [&](bool loaded, uint64_t base, uint32_t h0, uint32_t h1) {
+        for (uint32_t c = h0; c < h1; c += kWave) {
           const uint32_t idx = c + lane;
           uint2 h = make_uint2(0u, 0u);
-          if (idx < nh) h = l.hits[idx];
+          if (idx < h1) h = l.hits[idx];
           const uint32_t f = (h.y >> 12) & 63u, t = h.y & 0xfffu;
-          const bool v = idx < nh && f >= gA && f < gA + gG;
+          const bool v = idx < h1 && f >= gA && f < gA + gG;
           const uint32_t fl = v ? f - gA : 0u;
           float inten = 0.0f;
           if (v) inten = loaded ? __builtin_bit_cast(float, h.x) : a.dia.intens[base + h.x];
@@ -1286,8 +1287,36 @@ void select_kernel(SelArgs a) {
                   }
                 }
               }
-              // list slots in (lane, k) order = entry order; lanes 0-31
-              // first, then 32-63 (at most 32 x 8 hits each)
+              // list slots in (lane, k) order = entry order.  When the
+              // tile's hits fit the list, each lane writes its own hits
+              // (one iteration per hit of the busiest lane); otherwise lanes
+              // 0-31 first, then 32-63 (at most 32 x 8 hits each), so that
+              // an overflow is met half a tile at a time.
+              {
+                uint32_t pre, tot;
+                list_slots(hm, true, pre, tot);
+                if (tot == 0) continue;
+                if (nh + tot <= (uint32_t)kHitCap) {
+                  uint32_t pos = nh + pre, mm = hm;
+                  const uint32_t key = (j << 18) | (tfr[u] << 12);
+                  while (mm) {
+                    const uint32_t k = (uint32_t)__builtin_ctz(mm);
+                    mm &= mm - 1u;
+                    const uint32_t w = k < 4u ? (k < 2u ? raw[u].x : raw[u].y)
+                                              : (k < 6u ? raw[u].z : raw[u].w);
+                    const uint32_t cy = (w >> ((k & 1u) * 16u)) & 0xffffu;
+                    l.hits[pos] = make_uint2(8 * tg[u] + k - bo, key | (cy - c0));
+                    ++pos;
+                  }
+                  nh += tot;
+                  __builtin_amdgcn_wave_barrier();
+                  if (!deferred) {
+                    accumulate(false, base, 0, nh);
+                    nh = 0;
+                  }
+                  continue;
+                }
+              }
 #pragma unroll
               for (int hh = 0; hh < 2; ++hh) {
                 uint32_t pre, tot;
@@ -1320,14 +1349,14 @@ void select_kernel(SelArgs a) {
                 nh += tot;
                 __builtin_amdgcn_wave_barrier();
                 if (!deferred && nh > (uint32_t)(kHitCap - kTileHits)) {
-                  accumulate(false, base);
+                  accumulate(false, base, 0, nh);
                   nh = 0;
                 }
               }
             }
           }
           if (!deferred && nh) {
-            accumulate(false, base);
+            accumulate(false, base, 0, nh);
             nh = 0;
           }
           __builtin_amdgcn_wave_barrier();
@@ -1375,14 +1404,28 @@ void select_kernel(SelArgs a) {
       }
 
       if (more) stage2();
+      // With one window the list is in fragment order, so each group's
+      // hits follow the previous group's: [hcur, hend)
+      uint32_t hcur = 0;
       for (uint32_t g0 = 0, G = 0; g0 < nsel && nobs > 0; g0 += G) {
         const bool listed = g0 < fstop;
         G = min(l.gcap, (listed ? fstop : nsel) - g0);
         zero_group(G);
         gA = g0;
         gG = G;
-        if (listed) {
-          accumulate(true, 0);
+        if (listed && nobs == 1) {
+          uint32_t hend = hcur;
+          for (;;) {
+            const uint32_t idx = hend + (uint32_t)lane;
+            const bool in = idx < nh && ((l.hits[idx].y >> 12) & 63u) < g0 + G;
+            const uint32_t cnt = (uint32_t)__builtin_popcountll(ballot64(in));
+            hend += cnt;
+            if (cnt < (uint32_t)kWave) break;
+          }
+          accumulate(true, 0, hcur, hend);
+          hcur = hend;
+        } else if (listed) {
+          accumulate(true, 0, 0, nh);
         } else {
           nh = 0;
           extract(g0, g0 + G, false);
