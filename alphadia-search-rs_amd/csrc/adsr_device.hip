@@ -1460,7 +1460,8 @@ void select_kernel(SelArgs a) {
                 const float wf = l.w[g0 + f];
                 float t = 0.0f;
                 if (ballot64(act && !logf_is_normal(y + 1.0f)) == 0) {
-                  if (act) t = logf_normal(y + 1.0f, ltab) * wf;
+                  const float lq = logf_normal(y + 1.0f, ltab);
+                  if (act) t = lq * wf;
                 } else {
                   if (act) t = logf_exact(y + 1.0f, ltab) * wf;
                 }
@@ -1483,24 +1484,34 @@ void select_kernel(SelArgs a) {
             else conv4_any(row, i0, half, l.kern, y);
             const float wf = l.w[g0 + f];
             // outputs outside [half, L - half) stay zero (scalar.rs:30-38)
-            float t[4];
+            float t[4], u[4];
             bool act[4];
-            bool odd = false;
+            // all four arguments of ln positive normal floats: their bit
+            // patterns, as signed integers, lie in [0x00800000, 0x7f800000)
+            // (negative values, zeros, subnormals, infinities and NaN fall
+            // outside); otherwise the exact ln handles every case
+            int blo = 0x7fffffff, bhi = 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const uint32_t pos = i0 + q;
               const bool inside = pos >= half && pos + half < L;
               act[q] = v && inside && y[q] != 0.0f;
-              odd = odd || (act[q] && !logf_is_normal(y[q] + 1.0f));
+              u[q] = y[q] + 1.0f;
+              const int b = __builtin_bit_cast(int, u[q]);
+              blo = min(blo, b);
+              bhi = max(bhi, b);
             }
+            const bool odd = v && (blo < 0x00800000 || bhi >= 0x7f800000);
             if (ballot64(odd) == 0) {
 #pragma unroll
-              for (int q = 0; q < 4; ++q)
-                t[q] = act[q] ? logf_normal(y[q] + 1.0f, ltab) * wf : 0.0f;
+              for (int q = 0; q < 4; ++q) {
+                const float lq = logf_normal(u[q], ltab);
+                t[q] = act[q] ? lq * wf : 0.0f;
+              }
             } else {
 #pragma unroll
               for (int q = 0; q < 4; ++q)
-                t[q] = act[q] ? logf_exact(y[q] + 1.0f, ltab) * wf : 0.0f;
+                t[q] = act[q] ? logf_exact(u[q], ltab) * wf : 0.0f;
             }
             const uint32_t flo = rdlane(item >> 10, 0);
             const uint32_t last = min(n - c, (uint32_t)kWave) - 1;
