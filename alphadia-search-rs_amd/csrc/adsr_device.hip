@@ -58,6 +58,7 @@ enum StatusBits : uint32_t {
   kStTooManyFragments = 1u,
   kStTooManyWindows = 2u,
   kStWindowTooLong = 4u,
+  kStSelectFallback = 8u,  // top-k selection needs median of medians
 };
 
 #define HIP_TRY(expr)                                                   \
@@ -253,18 +254,138 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 // ---------------------------------------------------------------------------
 // plan: one thread per precursor row
 // ---------------------------------------------------------------------------
+// Rust 1.88 select_nth_unstable_by(k, |a, b| b.I.partial_cmp(&a.I)) for
+// 0 < k < len - 1 (src/speclib_flat/mod.rs:286-292): core::slice::select's
+// partition_at_index_loop, sequential, over a view V of fragment indices
+// (get/set by position, less(a, b) = I[b] < I[a]).  Slices of at most 16
+// elements are insertion-sorted; longer ones are partitioned around a
+// median-of-3 pivot (recursive for 64 and more elements) by the branchless
+// cyclic Lomuto partition of the unstable quicksort.  Returns false when the
+// 16 partition rounds run out (the reference then switches to median of
+// medians, which is not restated here; the caller reports it).
+template <class V>
+__device__ void rs_insertion_sort(const V &v, uint32_t o, uint32_t n) {
+  for (uint32_t i = 1; i < n; ++i) {
+    const uint32_t tmp = v.get(o + i);
+    uint32_t j = i;
+    while (j > 0 && v.less(tmp, v.get(o + j - 1))) {
+      v.set(o + j, v.get(o + j - 1));
+      --j;
+    }
+    v.set(o + j, tmp);
+  }
+}
+
+template <class V>
+__device__ uint32_t rs_median3(const V &v, uint32_t a, uint32_t b, uint32_t c) {
+  const bool x = v.less(v.get(a), v.get(b));
+  const bool y = v.less(v.get(a), v.get(c));
+  if (x == y) return (v.less(v.get(b), v.get(c)) != x) ? c : b;
+  return a;
+}
+
+template <int D, class V>
+__device__ uint32_t rs_median3_rec(const V &v, uint32_t a, uint32_t b, uint32_t c,
+                                   uint32_t n) {
+  if constexpr (D > 0) {
+    if (n * 8 >= 64) {
+      const uint32_t n8 = n / 8;
+      a = rs_median3_rec<D - 1>(v, a, a + n8 * 4, a + n8 * 7, n8);
+      b = rs_median3_rec<D - 1>(v, b, b + n8 * 4, b + n8 * 7, n8);
+      c = rs_median3_rec<D - 1>(v, c, c + n8 * 4, c + n8 * 7, n8);
+    }
+  }
+  return rs_median3(v, a, b, c);
+}
+
+// partition of v[o, o + len) around the element at o + pivot; `le` selects
+// the comparison !less(pivot, e) used for elements equal to an ancestor
+// pivot.  Returns the number of elements placed before the pivot.
+template <class V>
+__device__ uint32_t rs_partition(const V &v, uint32_t o, uint32_t len, uint32_t pivot,
+                                 bool le) {
+  uint32_t t = v.get(o);
+  v.set(o, v.get(o + pivot));
+  v.set(o + pivot, t);
+  const uint32_t pv = v.get(o);
+  const uint32_t r = o + 1, n = len - 1;
+  uint32_t num_lt = 0;
+  if (n > 0) {
+    const uint32_t gap_value = v.get(r);
+    uint32_t gap = 0;
+    for (uint32_t right = 1; right <= n; ++right) {
+      const uint32_t e = right == n ? gap_value : v.get(r + right);
+      const bool lt = le ? !v.less(pv, e) : v.less(e, pv);
+      v.set(r + gap, v.get(r + num_lt));
+      v.set(r + num_lt, e);
+      gap = right;
+      num_lt += lt ? 1u : 0u;
+    }
+  }
+  t = v.get(o);
+  v.set(o, v.get(o + num_lt));
+  v.set(o + num_lt, t);
+  return num_lt;
+}
+
+template <class V>
+__device__ bool rs_select(const V &v, uint32_t len, uint32_t index) {
+  uint32_t o = 0;
+  int limit = 16;
+  bool have_anc = false;
+  uint32_t anc = 0;
+  for (;;) {
+    if (len <= 16) {
+      if (len >= 2) rs_insertion_sort(v, o, len);
+      return true;
+    }
+    if (limit == 0) return false;
+    --limit;
+    const uint32_t d8 = len / 8;
+    const uint32_t pivot =
+        len < 64 ? rs_median3(v, o, o + d8 * 4, o + d8 * 7)
+                 : rs_median3_rec<6>(v, o, o + d8 * 4, o + d8 * 7, d8);
+    if (have_anc && !v.less(anc, v.get(pivot))) {
+      const uint32_t mid = rs_partition(v, o, len, pivot - o, true) + 1;
+      if (mid > index) return true;
+      o += mid;
+      len -= mid;
+      index -= mid;
+      have_anc = false;
+      continue;
+    }
+    const uint32_t mid = rs_partition(v, o, len, pivot - o, false);
+    if (mid == index) return true;
+    if (mid < index) {
+      anc = v.get(o + mid);
+      have_anc = true;
+      o += mid + 1;
+      len -= mid + 1;
+      index -= mid + 1;
+    } else {
+      len = mid;
+    }
+  }
+}
+
 // Fragment filter and order of src/speclib_flat/mod.rs:240-345 with the
 // Rust 1.88 select_nth_unstable_by behaviour:
 //   k >= kept        all kept fragments, kept order
 //   k == 0           none
 //   k == kept - 1    drop the first fragment of minimum intensity; the last
 //                    kept fragment takes its place (max_index + swap)
-//   otherwise        stable order by intensity descending, first k
-//                    (insertion sort for <= 16 elements; longer slices use
-//                    introselect, whose order at equal intensities is not
-//                    reproduced)
+//   otherwise        rs_select over the kept fragments in order, first k
+//                    (for at most 16 kept: the stable order by intensity)
 // followed by the stable sort by fragment m/z.  Accessor A gives the
 // intensity of fragment j (-inf when the fragment is filtered out) and mz.
+template <class A>
+struct KeyView {  // positions in A's key storage
+  const A &acc;
+  __device__ uint32_t get(uint32_t p) const { return acc.key(p); }
+  __device__ void set(uint32_t p, uint32_t j) const { acc.set_key(p, j); }
+  __device__ bool less(uint32_t a, uint32_t b) const { return acc.I(b) < acc.I(a); }
+};
+
 template <class A>
 __device__ uint32_t select_fragments(const A &acc, uint32_t n, uint32_t top_k,
                                      uint16_t *sel, float *sel_mz,
@@ -279,6 +400,30 @@ __device__ uint32_t select_fragments(const A &acc, uint32_t n, uint32_t top_k,
     return 0;
   }
   const int mode = (k >= kept) ? 0 : (k == kept - 1 ? 1 : 2);
+  if (mode == 2) {
+    // the key storage holds the kept fragments' indices by position
+    uint32_t q = 0;
+    for (uint32_t j = 0; j < n; ++j)
+      if (acc.I(j) > 0.0f) acc.set_key(q++, j);
+    if (!rs_select(KeyView<A>{acc}, kept, k)) {
+      status |= kStSelectFallback;
+      return 0;
+    }
+    for (uint32_t p = 0; p < k; ++p) {
+      const uint32_t j = acc.key(p);
+      const float mj = acc.mz(j);
+      uint32_t pos = 0;
+      for (uint32_t p2 = 0; p2 < k; ++p2) {
+        if (p2 == p) continue;
+        const float mi = acc.mz(acc.key(p2));
+        if (mi < mj || (!(mi > mj) && !(mi < mj) && p2 < p)) ++pos;
+      }
+      sel[pos] = (uint16_t)j;
+      sel_mz[pos] = mj;
+      sel_w[pos] = f_int[j];
+    }
+    return k;
+  }
   uint32_t drop = 0xffffffffu, last = 0, drop_pos = 0;
   if (mode == 1) {
     float mn = 0.0f;
@@ -302,14 +447,7 @@ __device__ uint32_t select_fragments(const A &acc, uint32_t n, uint32_t top_k,
     const float v = acc.I(j);
     uint32_t key = 0xffffffffu;
     if (v > 0.0f) {
-      if (mode == 2) {
-        uint32_t r = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-          const float u = acc.I(i);
-          if (u > v || (u == v && i < j)) ++r;
-        }
-        key = r < k ? r : 0xffffffffu;
-      } else if (mode == 1) {
+      if (mode == 1) {
         key = (j == drop) ? 0xffffffffu
                           : ((j == last && last != drop) ? drop_pos : pos_kept);
       } else {
@@ -446,6 +584,15 @@ __global__ __launch_bounds__(kWave) void plan_kernel(KDia dia, KLib lib,
 // ---------------------------------------------------------------------------
 constexpr int kPlanThreads = 256;  // 4 waves per block: more waves per CU than 1-wave blocks
 
+// rs_select's view of a group's lanes: positions in g_key, intensities g_I
+struct GroupView {
+  uint32_t *key;
+  const float *I;
+  __device__ uint32_t get(uint32_t p) const { return key[p]; }
+  __device__ void set(uint32_t p, uint32_t j) const { key[p] = j; }
+  __device__ bool less(uint32_t a, uint32_t b) const { return I[b] < I[a]; }
+};
+
 template <int LP>
 __global__ __launch_bounds__(kPlanThreads) void plan_group_kernel(KDia dia, KLib lib,
                                                                   KParams prm, KPlan plan,
@@ -572,9 +719,30 @@ __global__ __launch_bounds__(kPlanThreads) void plan_group_kernel(KDia dia, KLib
     drop_pos = (uint32_t)__shfl((int)pos_kept, (int)(gb + (mi & (LP - 1))), kWave);
   }
   __builtin_amdgcn_wave_barrier();
+  // more than 16 kept fragments (mode 2): the group's first lane runs the
+  // selection over the kept fragments' lanes, in order, in g_key; each
+  // lane then reads its position from the inverse permutation
+  const bool big = live && nsel > 0 && mode == 2 && nkept > 16;
+  if (big && gl == 0) {
+    uint32_t q = 0;
+    for (uint32_t i = 0; i < nraw; ++i)
+      if (g_I[gb + i] > 0.0f) g_key[gb + q++] = i;
+    const GroupView gv{g_key + gb, g_I + gb};
+    if (!rs_select(gv, nkept, k)) st |= kStSelectFallback;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t at = big && (uint32_t)gl < nkept ? g_key[lane] : 0xffffffffu;
+  __builtin_amdgcn_wave_barrier();
+  if (big) g_key[lane] = 0xffffffffu;
+  __builtin_amdgcn_wave_barrier();
+  if (at != 0xffffffffu && (uint32_t)gl < k && !(st & kStSelectFallback))
+    g_key[gb + at] = (uint32_t)gl;
+  __builtin_amdgcn_wave_barrier();
   uint32_t key = 0xffffffffu;
   if (kept && nsel > 0) {
-    if (mode == 2) {
+    if (mode == 2 && big) {
+      key = g_key[lane];
+    } else if (mode == 2) {
       uint32_t r = 0;
       for (uint32_t i = 0; i < nraw; ++i) {
         const float u = g_I[gb + i];
@@ -3151,6 +3319,11 @@ static adsr_status check_status(Workspace &w, uint32_t *total = nullptr) {
   if (st & kStWindowTooLong)
     return set_error(ADSR_ERR_UNSUPPORTED,
                      "an RT window exceeds %d cycles", kLCapMax);
+  if (st & kStSelectFallback)
+    return set_error(ADSR_ERR_UNSUPPORTED,
+                     "a precursor's top-k fragment selection needs the median-of-"
+                     "medians fallback of select_nth_unstable (more than 16 "
+                     "partition rounds)");
   return ADSR_OK;
 }
 
@@ -5831,6 +6004,10 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
     if (host[1] & kStTooManyFragments)
       return fail(set_error(ADSR_ERR_UNSUPPORTED,
                             "a precursor selects more than %d fragments", kMaxSel));
+    if (host[1] & kStSelectFallback)
+      return fail(set_error(ADSR_ERR_UNSUPPORTED,
+                            "a precursor's top-k fragment selection needs the "
+                            "median-of-medians fallback of select_nth_unstable"));
   }
   // candidate columns on the device
   BuildBufs tmp;

@@ -9,7 +9,10 @@
  *   - Rust 1.88 core::slice::binary_search_by (branchless; Ok() returns the
  *     LAST of several equal elements),
  *   - Rust 1.88 core::slice::select::partition_at_index (min/max swap for
- *     index 0 / len-1, insertion sort for len <= 16),
+ *     index 0 / len-1, then introselect: insertion sort for len <= 16,
+ *     recursive median-of-3 pivots, the branchless cyclic Lomuto partition
+ *     of core::slice::sort::unstable::quicksort, median of medians after
+ *     16 rounds),
  *   - ndarray 0.16.1 numeric_util::unrolled_fold (8 partial sums),
  *   - glibc logf/expf, called directly.
  * Build with -ffp-contract=off and without fast-math.
@@ -651,6 +654,234 @@ size_t orc_find_local_maxima(const float *a, size_t len, size_t offset,
  * a < b  <=>  I[b] < I[a]  (descending intensity; NaN compares Equal).   */
 static int frag_less(const float *I, size_t a, size_t b) { return I[b] < I[a]; }
 
+/* Rust 1.88 core::slice::select (select_nth_unstable_by) on an array of
+ * fragment indices ordered by frag_less.  `le` selects the comparison
+ * !is_less(b, a) that partition_at_index_loop uses for the elements equal
+ * to an ancestor pivot. */
+typedef struct {
+  const float *I;
+  int le;
+} sel_cmp;
+
+static int sel_lt(const sel_cmp *c, size_t a, size_t b) {
+  return c->le ? !frag_less(c->I, b, a) : frag_less(c->I, a, b);
+}
+
+static void swap_sz(size_t *v, size_t a, size_t b) {
+  size_t t = v[a];
+  v[a] = v[b];
+  v[b] = t;
+}
+
+/* core::slice::sort::shared::smallsort::insertion_sort_shift_left(v, 1) */
+static void sel_insertion_sort(size_t *v, size_t n, const float *I) {
+  for (size_t i = 1; i < n; i++) {
+    size_t tmp = v[i], j = i;
+    while (j > 0 && frag_less(I, tmp, v[j - 1])) {
+      v[j] = v[j - 1];
+      j--;
+    }
+    v[j] = tmp;
+  }
+}
+
+/* core::slice::sort::shared::pivot::median3: the position of the median */
+static size_t sel_median3(const size_t *v, const float *I, size_t a, size_t b,
+                          size_t c) {
+  int x = frag_less(I, v[a], v[b]);
+  int y = frag_less(I, v[a], v[c]);
+  if (x == y) {
+    int z = frag_less(I, v[b], v[c]);
+    return (z ^ x) ? c : b;
+  }
+  return a;
+}
+
+/* pivot::median3_rec over three runs of n elements starting at a, b, c */
+static size_t sel_median3_rec(const size_t *v, const float *I, size_t a,
+                              size_t b, size_t c, size_t n) {
+  if (n * 8 >= 64) {
+    size_t n8 = n / 8;
+    a = sel_median3_rec(v, I, a, a + n8 * 4, a + n8 * 7, n8);
+    b = sel_median3_rec(v, I, b, b + n8 * 4, b + n8 * 7, n8);
+    c = sel_median3_rec(v, I, c, c + n8 * 4, c + n8 * 7, n8);
+  }
+  return sel_median3(v, I, a, b, c);
+}
+
+/* pivot::choose_pivot (len >= 8) */
+static size_t sel_choose_pivot(const size_t *v, size_t len, const float *I) {
+  size_t d8 = len / 8;
+  size_t a = 0, b = d8 * 4, c = d8 * 7;
+  if (len < 64) return sel_median3(v, I, a, b, c);
+  return sel_median3_rec(v, I, a, b, c, d8);
+}
+
+/* quicksort::partition with partition_lomuto_branchless_cyclic: the pivot
+ * is swapped to the front, the rest is partitioned by a cyclic permutation
+ * that lifts the first element out and processes it last, then the pivot
+ * is swapped to its place.  Returns the number of elements less than it. */
+static size_t sel_partition(size_t *v, size_t len, size_t pivot,
+                            const sel_cmp *c) {
+  if (len == 0) return 0;
+  swap_sz(v, 0, pivot);
+  const size_t pv = v[0];
+  size_t *r = v + 1;
+  const size_t n = len - 1;
+  size_t num_lt = 0;
+  if (n > 0) {
+    const size_t gap_value = r[0];
+    size_t gap = 0;
+    for (size_t right = 1; right <= n; right++) {
+      const int last = right == n;
+      const size_t elem = last ? gap_value : r[right];
+      const int lt = sel_lt(c, elem, pv);
+      r[gap] = r[num_lt];
+      r[num_lt] = elem;
+      gap = right;
+      num_lt += (size_t)lt;
+    }
+  }
+  swap_sz(v, 0, num_lt);
+  return num_lt;
+}
+
+static size_t sel_max_index(const size_t *v, size_t n, const float *I) {
+  size_t m = 0;
+  for (size_t t = 1; t < n; t++)
+    if (frag_less(I, v[m], v[t])) m = t;
+  return m;
+}
+
+static size_t sel_min_index(const size_t *v, size_t n, const float *I) {
+  size_t m = 0;
+  for (size_t t = 1; t < n; t++)
+    if (frag_less(I, v[t], v[m])) m = t;
+  return m;
+}
+
+/* select::median_idx, ninther, median_of_ninthers, median_of_medians */
+static size_t sel_median_idx(const size_t *v, const float *I, size_t a, size_t b,
+                             size_t c) {
+  if (frag_less(I, v[c], v[a])) {
+    size_t t = a;
+    a = c;
+    c = t;
+  }
+  if (frag_less(I, v[c], v[b])) return c;
+  if (frag_less(I, v[b], v[a])) return a;
+  return b;
+}
+
+static void sel_ninther(size_t *v, const float *I, size_t a, size_t b, size_t c,
+                        size_t d, size_t e, size_t f, size_t g, size_t h,
+                        size_t i) {
+  b = sel_median_idx(v, I, a, b, c);
+  h = sel_median_idx(v, I, g, h, i);
+  if (frag_less(I, v[h], v[f])) {
+    size_t t = f;
+    f = h;
+    h = t;
+  }
+  if (frag_less(I, v[f], v[d])) {
+    size_t t = d;
+    d = f;
+    f = t;
+    if (frag_less(I, v[h], v[f])) f = h;
+    if (frag_less(I, v[f], v[b])) f = b;
+  }
+  swap_sz(v, f, e);
+}
+
+static void sel_median_of_medians(size_t *v, size_t len, size_t k, const float *I);
+
+static size_t sel_median_of_ninthers(size_t *v, size_t len, const float *I) {
+  size_t frac = len <= 1024 ? len / 12 : len <= 128 * 1024 ? len / 64 : len / 1024;
+  size_t pivot = frac / 2;
+  size_t lo = len / 2 - pivot;
+  size_t hi = frac + lo;
+  size_t gap = (len - 9 * frac) / 4;
+  size_t a = lo - 4 * frac - gap;
+  size_t b = hi + gap;
+  for (size_t i = lo; i < hi; i++) {
+    sel_ninther(v, I, a, i - frac, b, a + 1, i, b + 1, a + 2, i + frac, b + 2);
+    a += 3;
+    b += 3;
+  }
+  sel_median_of_medians(v + lo, frac, pivot, I);
+  sel_cmp c = {I, 0};
+  return sel_partition(v, len, lo + pivot, &c);
+}
+
+static void sel_median_of_medians(size_t *v, size_t len, size_t k, const float *I) {
+  for (;;) {
+    if (len <= 16) {
+      if (len >= 2) sel_insertion_sort(v, len, I);
+      return;
+    }
+    if (k == len - 1) {
+      swap_sz(v, sel_max_index(v, len, I), k);
+      return;
+    }
+    if (k == 0) {
+      swap_sz(v, sel_min_index(v, len, I), 0);
+      return;
+    }
+    size_t p = sel_median_of_ninthers(v, len, I);
+    if (p == k) return;
+    if (p > k) {
+      len = p;
+    } else {
+      v += p + 1;
+      len -= p + 1;
+      k -= p + 1;
+    }
+  }
+}
+
+/* select::partition_at_index for 0 < index < len - 1:
+ * partition_at_index_loop with its limit of 16 rounds. */
+static void sel_partition_at_index_loop(size_t *v, size_t len, size_t index,
+                                        const float *I) {
+  int limit = 16;
+  int have_ancestor = 0;
+  size_t ancestor = 0;
+  for (;;) {
+    if (len <= 16) {
+      if (len >= 2) sel_insertion_sort(v, len, I);
+      return;
+    }
+    if (limit == 0) {
+      sel_median_of_medians(v, len, index, I);
+      return;
+    }
+    limit--;
+    size_t pivot_pos = sel_choose_pivot(v, len, I);
+    if (have_ancestor && !frag_less(I, ancestor, v[pivot_pos])) {
+      sel_cmp c = {I, 1};
+      size_t mid = sel_partition(v, len, pivot_pos, &c) + 1;
+      if (mid > index) return;
+      v += mid;
+      len -= mid;
+      index -= mid;
+      have_ancestor = 0;
+      continue;
+    }
+    sel_cmp c = {I, 0};
+    size_t mid = sel_partition(v, len, pivot_pos, &c);
+    if (mid == index) return;
+    if (mid < index) {
+      ancestor = v[mid];
+      have_ancestor = 1;
+      v += mid + 1;
+      len -= mid + 1;
+      index -= mid + 1;
+    } else {
+      len = mid;
+    }
+  }
+}
+
 /* Returns the number of selected fragments; out_order[j] is the original
  * (within-precursor) index of the j-th selected fragment in output order.
  * top_k == SIZE_MAX means no limit.                                      */
@@ -677,18 +908,7 @@ size_t orc_filter_sort_fragments(size_t n, const float *mz, const float *I,
     } else if (k == 0) {
       /* min_index swap; the slice is truncated to length 0 below */
     } else {
-      /* len <= 16: insertion_sort_shift_left (stable, descending I).
-       * len > 16: introselect; restated with the same stable order, which
-       * fixes the same top-k set whenever intensities at the cut differ. */
-      for (size_t i = 1; i < len; i++) {
-        size_t cur = v[i];
-        size_t j = i;
-        while (j > 0 && frag_less(I, cur, v[j - 1])) {
-          v[j] = v[j - 1];
-          j--;
-        }
-        v[j] = cur;
-      }
+      sel_partition_at_index_loop(v, len, k, I);
     }
     len = k;
   }

@@ -185,6 +185,64 @@ def test_fused_plan_fragments(ads, run0, oracle_run0):
         assert np.array_equal(sel[r, : ns[r]].astype(np.int64), want)
 
 
+@pytest.mark.parametrize("frag_min,frag_max,top_k", [(17, 32, 7), (33, 60, 20), (20, 80, 12)],
+                         ids=["lanes32", "lanes64", "thread-per-row"])
+def test_top_k_with_ties_follows_introselect(ads, frag_min, frag_max, top_k):
+    """More than 16 kept fragments, top_k < kept - 1, intensities on a few
+    levels (ties at the cut) and some equal fragment m/z (the m/z sort
+    keeps the selection's order): the device plan (per-row selected
+    fragments) and the candidate table equal the oracle's restated Rust
+    select_nth_unstable_by.  The three libraries take the three plan paths:
+    32-lane and 64-lane groups per row, and one thread per row (rows of at
+    most 48 fragments in LDS, longer ones in global scratch)."""
+    import ctypes
+    import alphadia_search_rs._native as N
+    spec = synthetic.config(2, n_cycles=300, n_windows=4, peaks_per_spectrum=600,
+                            n_targets=300, frag_min=frag_min, frag_max=frag_max,
+                            half_window_min=30, half_window_max=60)
+    gen = synthetic.generate(spec)
+    lib = [a.copy() for a in gen["lib"]]
+    fs, fe = lib[6].astype(np.int64), lib[7].astype(np.int64)
+    rng = np.random.default_rng(frag_max)
+    lib[10] = rng.integers(1, 5, len(lib[10])).astype(np.float32)
+    for r in range(0, len(fs), 3):
+        a = fs[r]
+        lib[9][a + 2] = lib[9][a + 1]   # equal m/z within the row
+    gen = dict(gen, lib=tuple(lib))
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    libo = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    p = params_for(gen["spec"], top_k_fragments=top_k)
+    if gen.get("rt_tolerance_per_precursor") is not None:
+        p.update({"rt_tolerance_per_precursor": gen["rt_tolerance_per_precursor"]})
+    c = p._c()
+    n = libo.num_precursors
+    stride = 64
+    cs = np.zeros(n, np.uint64)
+    ce = np.zeros(n, np.uint64)
+    ns = np.zeros(n, np.uint32)
+    sel = np.zeros((n, stride), np.uint64)
+    N.check(N.lib.adsr_select_plan(d._h, libo._h, ctypes.byref(c), 0, n, N.ptr(cs),
+                                   N.ptr(ce), N.ptr(ns), N.ptr(sel), stride))
+    order = np.argsort(lib[0], kind="stable")  # plan rows follow precursor_idx
+    differs = 0
+    for i in range(n):
+        r = int(order[i])
+        a, b = int(fs[r]), int(fe[r])
+        want = a + oracle.filter_sort_fragments(lib[9][a:b], lib[10][a:b], lib[14][a:b],
+                                                lib[16][a:b], True, True, top_k)
+        assert ns[i] == len(want), i
+        assert np.array_equal(sel[i, : ns[i]].astype(np.int64), want), i
+        stable = a + np.array(sorted(sorted(
+            [j for j in range(b - a) if lib[10][a + j] > 0
+             and not (lib[16][a + j] == 121 and lib[14][a + j] == 1)],
+            key=lambda j: -lib[10][a + j])[:top_k], key=lambda j: lib[9][a + j]), np.int64)
+        differs += not np.array_equal(stable, want)
+    assert differs > 0  # the stable order would differ from the reference's here
+    got = _search(ads, gen, top_k_fragments=top_k)[3]
+    want = _oracle(gen, top_k_fragments=top_k)
+    assert_tables_equal(got, want, f"introselect {frag_min}-{frag_max} top {top_k}")
+
+
 def test_workload0_candidates_match_oracle(ads, run0):
     got = _search(ads, run0)[3]
     want = _oracle(run0)

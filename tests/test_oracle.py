@@ -520,6 +520,40 @@ def test_fragment_selection_brute_force():
         assert order.tolist() == want
 
 
+def test_fragment_introselect_invariants():
+    """More than 16 kept fragments and 0 < k < kept - 1: the restated Rust
+    1.88 select_nth_unstable_by (introselect) keeps k fragments whose
+    intensities are the k largest (every kept one >= every dropped one),
+    each fragment at most once, in ascending m/z.  Which of several tied
+    fragments at the cut is kept follows the restated algorithm; no
+    reference vector fixes it."""
+    rng = np.random.default_rng(17)
+    for trial in range(400):
+        n = int(rng.integers(17, 160))
+        it = rng.integers(1, 6 if trial % 2 else 40, n).astype(np.float32)
+        mz = rng.uniform(200, 1800, n).astype(np.float32)
+        k = int(rng.integers(1, n - 1))
+        order = oracle.filter_sort_fragments(mz, it, [2] * n, [98] * n, True, True, k)
+        assert len(order) == k and len(set(order.tolist())) == k
+        dropped = np.setdiff1d(np.arange(n), order)
+        assert it[order].min() >= it[dropped].max()
+        assert np.all(np.diff(mz[order]) >= 0)
+
+
+def test_fragment_introselect_small_and_extreme_cases():
+    """<= 16 kept fragments: insertion sort, i.e. the stable order by
+    descending intensity; k == kept - 1: the first fragment of the lowest
+    intensity is dropped and the last kept fragment takes its place, for
+    any length."""
+    for n in (17, 40, 100):
+        it = np.array([5.0] * n, np.float32)
+        it[3] = 1.0
+        it[n - 5] = 1.0
+        mz = np.arange(n, dtype=np.float32) + 200
+        order = oracle.filter_sort_fragments(mz, it, [2] * n, [98] * n, True, True, n - 1)
+        assert order.tolist() == [j for j in range(n) if j != 3]
+
+
 # ----------------------------------------------------------------- library
 def test_speclib_sorted_by_precursor_idx():
     case = GOLD["speclib_sort"]
