@@ -935,8 +935,18 @@ __device__ __forceinline__ WaveLds carve(char *base, uint32_t L, uint32_t nsel,
 // ([2], the candidate total, is written by the scan)
 constexpr int kSmallWords = 32;
 constexpr int kMaxChunks = kSmallWords - 8;
+// pipelined chunks: chunk k's queue is small[8 + k], its candidate count
+// small[16 + k]
+constexpr int kPipeMax = 8;
 __global__ void reset_queue_kernel(uint32_t *small) {
   if (threadIdx.x < kSmallWords && threadIdx.x != 2) small[threadIdx.x] = 0;
+}
+
+// The small words into page-locked host memory (`host` is its device
+// address): stores from a kernel, where a copy would be a blit kernel that
+// waits for compute units.
+__global__ void publish_kernel(const uint32_t *small, uint32_t *host) {
+  if (threadIdx.x < kSmallWords) host[threadIdx.x] = small[threadIdx.x];
 }
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t lane) {
@@ -1865,6 +1875,30 @@ __global__ __launch_bounds__(256) void scatter_kernel(
   }
 }
 
+// Chunk k of a pipelined search (rows [r0, r0 + n), r0 a multiple of
+// kScanBlock): the chunk's candidate rows go after those of chunks 0..k-1,
+// whose counts tot[0..k) the scans of those chunks wrote.
+__global__ __launch_bounds__(256) void scatter_chunk_kernel(
+    const uint32_t *counts, const uint32_t *local, const uint32_t *bsum_k,
+    uint32_t r0, uint32_t n, uint32_t cc, const float *cs, const uint32_t *cctr,
+    const uint32_t *tot, uint32_t k, CandRow *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t base = 0;
+  for (uint32_t j = 0; j < k; ++j) base += tot[j];
+  const uint32_t row = r0 + i;
+  const uint32_t cnt = counts[row];
+  const uint32_t off = base + local[row] + bsum_k[i / kScanBlock];
+  for (uint32_t r = 0; r < cnt; ++r) {
+    CandRow c;
+    c.row = row;
+    c.rank = r;
+    c.score = cs[(size_t)row * cc + r];
+    c.center = cctr[(size_t)row * cc + r];
+    out[off + r] = c;
+  }
+}
+
 // A candidate row kept in HBM: the library's precursor_idx resolved, so the
 // table outlives the library handle.
 struct DevCand {
@@ -1937,6 +1971,32 @@ __global__ __launch_bounds__(256) void table_kernel(const CandRow *rows,
   blk[6 * n + i] = center - pl;  // usize arithmetic, mod.rs:176
   blk[7 * n + i] = min(center + pl + 1, nrt);
   reinterpret_cast<float *>(blk + 8 * n)[i] = r.score;
+}
+
+// Chunk k's rows of the host table, in a block whose columns are `stride`
+// rows apart (the rows of chunks 0..k-1 come first), so that each chunk's
+// column segments can be copied as soon as the chunk is done.
+__global__ __launch_bounds__(256) void table_chunk_kernel(const CandRow *rows,
+                                                          const uint32_t *tot, uint32_t k,
+                                                          const uint64_t *p_idx, uint64_t pl,
+                                                          uint64_t nrt, uint64_t stride,
+                                                          uint64_t *blk) {
+  uint64_t base = 0;
+  for (uint32_t j = 0; j < k; ++j) base += tot[j];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= tot[k]) return;
+  const uint64_t q = base + i;
+  const CandRow r = rows[q];
+  const uint64_t center = r.center;
+  blk[q] = p_idx[r.row];
+  blk[stride + q] = r.rank;
+  blk[2 * stride + q] = 0;
+  blk[3 * stride + q] = 0;
+  blk[4 * stride + q] = 0;
+  blk[5 * stride + q] = center;
+  blk[6 * stride + q] = center - pl;  // usize arithmetic, mod.rs:176
+  blk[7 * stride + q] = min(center + pl + 1, nrt);
+  reinterpret_cast<float *>(blk + 8 * stride)[q] = r.score;
 }
 
 // The nine to_arrays columns of the compacted rows straight into caller
@@ -2708,6 +2768,10 @@ struct Workspace {
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_plan[kMaxChunks] = {};
+  // pipelined host tables: chunk k done (ev_pipe[k]); page-locked copies of
+  // the small words after each chunk (hsmall + kSmallWords * k)
+  hipEvent_t ev_pipe[kPipeMax] = {};
+  uint32_t *hsmall = nullptr, *hsmall_dev = nullptr;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_src = nullptr;
 };
@@ -2741,6 +2805,9 @@ static void ws_free(Workspace &w) {
     if (e) (void)hipEventDestroy(e);
   for (auto &e : w.ev_plan)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : w.ev_pipe)
+    if (e) (void)hipEventDestroy(e);
+  if (w.hsmall) (void)hipHostFree(w.hsmall);
   if (w.ev_fork) (void)hipEventDestroy(w.ev_fork);
   if (w.stream2) (void)hipStreamDestroy(w.stream2);
   if (w.stream) (void)hipStreamDestroy(w.stream);
@@ -2765,6 +2832,30 @@ static uint32_t search_chunks(uint32_t n_rows) {
   else if (n_rows <= kChunkAbove) return 1;
   const uint64_t k = ((uint64_t)n_rows + rows - 1) / rows;
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(k, kMaxChunks));
+}
+
+// A search whose table goes to host memory runs in chunks of library rows
+// (multiples of kScanBlock), each compacted and tabled on its own, so that
+// chunk k's columns are copied to the host while chunk k + 1 is selected.
+// ADSR_PIPE_CHUNKS sets the count (0 or 1: off); by default 3 for
+// libraries of kPipeAbove rows or more, below kChunkAbove rows (larger
+// libraries are planned in chunks and compacted once).  The copies to host
+// memory run as blit kernels on the compute units, slowing the select they
+// overlap, and each select launch pays its start and tail: at 400k rows,
+// 3 chunks measured best (about 10% less than one launch and one copy).
+constexpr uint32_t kPipeAbove = 1u << 17;
+
+static uint32_t pipe_chunks(uint32_t n_rows) {
+  static const long env = [] {
+    const char *e = std::getenv("ADSR_PIPE_CHUNKS");
+    return e ? std::strtol(e, nullptr, 10) : -1L;
+  }();
+  if (n_rows <= kScanBlock || search_chunks(n_rows) > 1) return 1;
+  uint32_t k = env >= 0 ? (uint32_t)std::min<long>(env, kPipeMax)
+                        : (n_rows >= kPipeAbove ? 3u : 1u);
+  // every chunk holds at least one scan block
+  k = std::min<uint32_t>(k, (n_rows + kScanBlock - 1) / kScanBlock);
+  return std::max<uint32_t>(k, 1u);
 }
 
 // Idle workspaces of a library (at most kIdleWorkspaces) are taken by the
@@ -2800,10 +2891,21 @@ static adsr_status ws_alloc(Workspace &w, size_t n_rows, size_t ks, size_t cc,
   const size_t nb = (R + kScanBlock - 1) / kScanBlock;
   HIP_TRY(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
   for (auto &e : w.ev) HIP_TRY(hipEventCreate(&e));
-  if (search_chunks((uint32_t)n_rows) > 1) {
+  if (search_chunks((uint32_t)n_rows) > 1 || pipe_chunks((uint32_t)n_rows) > 1)
     HIP_TRY(hipStreamCreateWithFlags(&w.stream2, hipStreamNonBlocking));
+  if (search_chunks((uint32_t)n_rows) > 1) {
     HIP_TRY(hipEventCreateWithFlags(&w.ev_fork, hipEventDisableTiming));
     for (auto &e : w.ev_plan) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  if (pipe_chunks((uint32_t)n_rows) > 1) {
+    for (auto &e : w.ev_pipe) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (alloc_should_fail() ||
+        hipHostMalloc((void **)&w.hsmall, kSmallWords * 4 * kPipeMax,
+                      hipHostMallocMapped) != hipSuccess) {
+      w.hsmall = nullptr;
+      return set_error(ADSR_ERR_OUT_OF_MEMORY, "page-locked status words");
+    }
+    HIP_TRY(hipHostGetDevicePointer((void **)&w.hsmall_dev, w.hsmall, 0));
   }
   HIP_TRY(dev_malloc(&w.rec, R * sizeof(uint4)));
   HIP_TRY(dev_malloc(&w.obs, R * kObsMax * 2));
@@ -3048,11 +3150,22 @@ static int occupancy_blocks(const void *fn, int threads, size_t lds) {
   return std::max(nb, 1);
 }
 
+// A pipelined search's host table (see pipe_chunks): `chunks` chunks, the
+// table's precursor_idx source and cycle bounds, its row stride; `used` is
+// set when enqueue ran the search that way.
+struct PipeSpec {
+  uint32_t chunks = 0;
+  const uint64_t *p_idx = nullptr;
+  uint64_t pl = 0, nrt = 0, stride = 0;
+  bool used = false;
+};
+
 static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
                            LaunchShape shape, float *dbg,
                            uint32_t dbg_stride, uint32_t dbg_row0,
                            uint32_t dbg_rows, bool timed_select,
-                           bool plan_only = false, bool with_bins = false) {
+                           bool plan_only = false, bool with_bins = false,
+                           PipeSpec *pipe = nullptr) {
   hipStream_t s = w.stream;
   uint32_t *queue = w.small, *status = w.small + 1, *total = w.small + 2;
   reset_queue_kernel<<<1, 64, 0, s>>>(w.small);
@@ -3197,6 +3310,45 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
     };
     auto launch_select = [&](const SelArgs &x) { launch_select_on(x, s); };
     if (timed_select) HIP_TRY(hipEventRecord(w.ev[2], s));
+    if (pipe && pipe->chunks > 1 && nchunk == 1 && !split && !dbg && !with_bins &&
+        w.hsmall_dev && w.table) {
+      // chunk k: select, compaction after chunks 0..k-1, table rows, the
+      // small words to page-locked memory, then ev_pipe[k]
+      pipe->used = true;
+      const uint32_t nbl = (n_rows + kScanBlock - 1) / kScanBlock;
+      const uint32_t pk = pipe->chunks;
+      const uint32_t cc = (uint32_t)sp.prm.cand_count;
+      for (uint32_t k = 0; k < pk; ++k) {
+        const uint32_t b0 = (uint32_t)((uint64_t)nbl * k / pk);
+        const uint32_t b1 = (uint32_t)((uint64_t)nbl * (k + 1) / pk);
+        const uint32_t r0 = b0 * kScanBlock;
+        const uint32_t n = std::min<uint32_t>(b1 * kScanBlock, n_rows) - r0;
+        SelArgs x = a;
+        chunk_args(r0, n, x.lib, x.prm, x.plan);
+        x.n_rows = n;
+        x.queue = w.small + 8 + k;
+        x.cand_score += (size_t)r0 * cc;
+        x.cand_center += (size_t)r0 * cc;
+        x.cand_count += r0;
+        x.row0 = r0;
+        launch_select(x);
+        scan_local_kernel<<<b1 - b0, 256, 0, s>>>(w.cand_count + r0, n, w.local + r0,
+                                                  w.bsum + b0);
+        scan_blocks_kernel<<<1, 1024, 0, s>>>(w.bsum + b0, b1 - b0, w.small + 16 + k);
+        scatter_chunk_kernel<<<(n + 255) / 256, 256, 0, s>>>(
+            w.cand_count, w.local, w.bsum + b0, r0, n, cc, w.cand_score, w.cand_center,
+            w.small + 16, k, w.out);
+        const uint64_t most = (uint64_t)n * std::max<uint32_t>(cc, 1u);
+        table_chunk_kernel<<<(uint32_t)((most + 255) / 256), 256, 0, s>>>(
+            w.out, w.small + 16, k, pipe->p_idx, pipe->pl, pipe->nrt, pipe->stride,
+            w.table);
+        publish_kernel<<<1, 64, 0, s>>>(w.small, w.hsmall_dev + (size_t)kSmallWords * k);
+        HIP_TRY(hipEventRecord(w.ev_pipe[k], s));
+      }
+      if (timed_select) HIP_TRY(hipEventRecord(w.ev[3], s));
+      HIP_TRY(hipGetLastError());
+      return ADSR_OK;
+    }
     if (nchunk > 1) {
       // chunk k's rows through the first select launch once its plan is
       // done; long rows of every chunk go to the one long-row launch
@@ -3303,12 +3455,7 @@ static void fill_plan(const adsr_dia *d, const adsr_lib *l,
                   (uint32_t)ks};
 }
 
-static adsr_status check_status(Workspace &w, uint32_t *total = nullptr) {
-  uint32_t host[3] = {0, 0, 0};
-  HIP_TRY(hipMemcpyAsync(host, w.small, 12, hipMemcpyDeviceToHost, w.stream));
-  HIP_TRY(hipStreamSynchronize(w.stream));
-  if (total) *total = host[2];
-  const uint32_t st = host[1];
+static adsr_status status_error(uint32_t st) {
   if (st & kStTooManyFragments)
     return set_error(ADSR_ERR_UNSUPPORTED,
                      "a precursor selects more than %d fragments", kMaxSel);
@@ -3325,6 +3472,14 @@ static adsr_status check_status(Workspace &w, uint32_t *total = nullptr) {
                      "medians fallback of select_nth_unstable (more than 16 "
                      "partition rounds)");
   return ADSR_OK;
+}
+
+static adsr_status check_status(Workspace &w, uint32_t *total = nullptr) {
+  uint32_t host[3] = {0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(host, w.small, 12, hipMemcpyDeviceToHost, w.stream));
+  HIP_TRY(hipStreamSynchronize(w.stream));
+  if (total) *total = host[2];
+  return status_error(host[1]);
 }
 
 // Queues the host table's columns (table_kernel) behind the search, so
@@ -3369,6 +3524,91 @@ static adsr_status fetch(Workspace &w, const adsr_lib *l, uint32_t total,
       return set_error(ADSR_ERR_DEVICE, "candidate table copy failed");
     }
   }
+  *out = c;
+  return ADSR_OK;
+}
+
+// A search whose table ends in host memory, pipelined (pipe_chunks): the
+// host block is laid out with the device table's stride (one row slot per
+// possible candidate), and chunk k's nine column segments are copied on the
+// second stream once ev_pipe[k] says that the chunk is tabled, while later
+// chunks are still being selected.  When enqueue cannot pipeline the search
+// (long rows in a second launch), the table is fetched as usual.
+static adsr_status search_to_host_pipelined(Workspace &w, const SearchPlan &sp,
+                                            uint32_t n, LaunchShape shape,
+                                            const adsr_dia *d, const adsr_lib *l,
+                                            const adsr_selection_params *p,
+                                            uint32_t chunks, adsr_candidates **out) {
+  const size_t cap = std::max<size_t>(w.n_rows, 1) * std::max<size_t>(w.cc, 1);
+  if (!w.table)
+    HIP_TRY(dev_malloc(&w.table, cap * adsr_candidates::kRowBytes));
+  PipeSpec ps;
+  ps.chunks = chunks;
+  ps.p_idx = l->dev.p_idx;
+  ps.pl = p->peak_length;
+  ps.nrt = d->rt_index.size();
+  ps.stride = cap;
+  adsr_status st = enqueue(w, sp, n, shape, nullptr, 0, 0, 0, false, false, false, &ps);
+  if (st) return st;
+  if (!ps.used) {
+    st = queue_table(w, d, l, p);
+    uint32_t total = 0;
+    if (!st) st = check_status(w, &total);
+    if (!st) st = fetch(w, l, total, out);
+    return st;
+  }
+  const size_t bytes = cap * adsr_candidates::kRowBytes;
+  size_t bcap = 0;
+  bool pinned = true;
+  void *blk = pinned_acquire(bytes, &bcap);
+  if (!blk) {
+    pinned = false;
+    bcap = bytes;
+    blk = host_malloc(bytes);
+    if (!blk) {
+      (void)hipStreamSynchronize(w.stream);
+      return set_error(ADSR_ERR_OUT_OF_MEMORY, "candidate table");
+    }
+  }
+  auto *c = new adsr_candidates();
+  c->pinned = pinned;
+  c->block_cap = bcap;
+  c->block = blk;
+  auto fail = [&](adsr_status e) {
+    (void)hipStreamSynchronize(w.stream);
+    (void)hipStreamSynchronize(w.stream2);
+    delete c;
+    return e;
+  };
+  uint64_t *hb = static_cast<uint64_t *>(blk);
+  const uint64_t *db = w.table;
+  uint64_t off = 0;
+  uint32_t status = 0;
+  for (uint32_t k = 0; k < chunks; ++k) {
+    if (hipEventSynchronize(w.ev_pipe[k]) != hipSuccess)
+      return fail(set_error(ADSR_ERR_DEVICE, "search failed: %s",
+                            hipGetErrorString(hipGetLastError())));
+    const uint32_t *hs = w.hsmall + (size_t)kSmallWords * k;
+    status |= hs[1];
+    const uint64_t nk = hs[16 + k];
+    if (status || nk == 0) continue;
+    if (off + nk > cap) return fail(set_error(ADSR_ERR_INTERNAL, "candidate count"));
+    for (int col = 0; col < 8; ++col) {
+      if (hipMemcpyAsync(hb + (size_t)col * cap + off, db + (size_t)col * cap + off,
+                         nk * 8, hipMemcpyDeviceToHost, w.stream2) != hipSuccess)
+        return fail(set_error(ADSR_ERR_DEVICE, "candidate table copy failed"));
+    }
+    if (hipMemcpyAsync(reinterpret_cast<float *>(hb + 8 * cap) + off,
+                       reinterpret_cast<const float *>(db + 8 * cap) + off, nk * 4,
+                       hipMemcpyDeviceToHost, w.stream2) != hipSuccess)
+      return fail(set_error(ADSR_ERR_DEVICE, "candidate table copy failed"));
+    off += nk;
+  }
+  if (hipStreamSynchronize(w.stream2) != hipSuccess)
+    return fail(set_error(ADSR_ERR_DEVICE, "candidate table copy failed"));
+  st = status_error(status);
+  if (st) return fail(st);
+  c->bind(blk, l->n_prec ? off : 0, cap);
   *out = c;
   return ADSR_OK;
 }
@@ -3553,6 +3793,12 @@ static adsr_status select_common(const adsr_dia *dc, const adsr_lib *lc,
   }
   if (tpb) *tpb = shape.waves * 64;
   if (ppw) *ppw = shape.chunk;
+  if (steps <= 0 && warmup <= 0 && out && !dout && pipe_chunks(n) > 1) {
+    st = search_to_host_pipelined(w, sp, n, shape, d, l, p, pipe_chunks(n), out);
+    if (st) return fail(st);
+    ws_recycle(l, w);
+    return ADSR_OK;
+  }
   // first search: checks the device-side limits before anything is timed
   st = enqueue(w, sp, n, shape, nullptr, 0, 0, 0, false);
   if (!st) st = check_status(w);

@@ -215,15 +215,17 @@ struct adsr_candidates {
   bool pinned = false;
 
   static constexpr size_t kRowBytes = 8 * 8 + 4;
-  // points the columns into `blk`, laid out for n rows
-  void bind(void *blk, size_t rows) {
+  // points the columns into `blk`: n rows, columns `stride` rows apart
+  // (stride 0: n)
+  void bind(void *blk, size_t rows, size_t stride = 0) {
+    if (stride == 0) stride = rows;
     n = rows;
     block = blk;
     uint64_t *u = static_cast<uint64_t *>(blk);
     uint64_t **cols[8] = {&precursor_idx, &rank, &scan_center, &scan_start,
                           &scan_stop, &cycle_center, &cycle_start, &cycle_stop};
-    for (int i = 0; i < 8; ++i) *cols[i] = u + (size_t)i * rows;
-    score = reinterpret_cast<float *>(u + 8 * rows);
+    for (int i = 0; i < 8; ++i) *cols[i] = u + (size_t)i * stride;
+    score = reinterpret_cast<float *>(u + 8 * stride);
   }
   ~adsr_candidates() {
     if (!block) return;
