@@ -262,7 +262,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 // median-of-3 pivot (recursive for 64 and more elements) by the branchless
 // cyclic Lomuto partition of the unstable quicksort.  Returns false when the
 // 16 partition rounds run out (the reference then switches to median of
-// medians, which is not restated here; the caller reports it).
+// medians, which is not restated here) or for 4096 or more elements; the
+// caller reports it.
 template <class V>
 __device__ void rs_insertion_sort(const V &v, uint32_t o, uint32_t n) {
   for (uint32_t i = 1; i < n; ++i) {
@@ -284,18 +285,26 @@ __device__ uint32_t rs_median3(const V &v, uint32_t a, uint32_t b, uint32_t c) {
   return a;
 }
 
-template <int D, class V>
-__device__ uint32_t rs_median3_rec(const V &v, uint32_t a, uint32_t b, uint32_t c,
-                                   uint32_t n) {
-  if constexpr (D > 0) {
-    if (n * 8 >= 64) {
-      const uint32_t n8 = n / 8;
-      a = rs_median3_rec<D - 1>(v, a, a + n8 * 4, a + n8 * 7, n8);
-      b = rs_median3_rec<D - 1>(v, b, b + n8 * 4, b + n8 * 7, n8);
-      c = rs_median3_rec<D - 1>(v, c, c + n8 * 4, c + n8 * 7, n8);
+// pivot::choose_pivot for 16 < len < 4096: median3, or median3_rec with its
+// (at most two) levels of recursion written out; len = n * 8 + r
+template <class V>
+__device__ uint32_t rs_choose_pivot(const V &v, uint32_t o, uint32_t len) {
+  const uint32_t n = len / 8;
+  const uint32_t a = o, b = o + n * 4, c = o + n * 7;
+  if (len < 64) return rs_median3(v, a, b, c);
+  const uint32_t n8 = n / 8, n64 = n8 / 8;
+  uint32_t m[3];
+  const uint32_t x3[3] = {a, b, c};
+  for (int i = 0; i < 3; ++i) {
+    uint32_t p = x3[i], q = x3[i] + n8 * 4, r = x3[i] + n8 * 7;
+    if (n8 * 8 >= 64) {
+      p = rs_median3(v, p, p + n64 * 4, p + n64 * 7);
+      q = rs_median3(v, q, q + n64 * 4, q + n64 * 7);
+      r = rs_median3(v, r, r + n64 * 4, r + n64 * 7);
     }
+    m[i] = rs_median3(v, p, q, r);
   }
-  return rs_median3(v, a, b, c);
+  return rs_median3(v, m[0], m[1], m[2]);
 }
 
 // partition of v[o, o + len) around the element at o + pivot; `le` selects
@@ -330,6 +339,7 @@ __device__ uint32_t rs_partition(const V &v, uint32_t o, uint32_t len, uint32_t 
 
 template <class V>
 __device__ bool rs_select(const V &v, uint32_t len, uint32_t index) {
+  if (len >= 4096) return false;  // median3_rec deeper than two levels
   uint32_t o = 0;
   int limit = 16;
   bool have_anc = false;
@@ -341,10 +351,7 @@ __device__ bool rs_select(const V &v, uint32_t len, uint32_t index) {
     }
     if (limit == 0) return false;
     --limit;
-    const uint32_t d8 = len / 8;
-    const uint32_t pivot =
-        len < 64 ? rs_median3(v, o, o + d8 * 4, o + d8 * 7)
-                 : rs_median3_rec<6>(v, o, o + d8 * 4, o + d8 * 7, d8);
+    const uint32_t pivot = rs_choose_pivot(v, o, len);
     if (have_anc && !v.less(anc, v.get(pivot))) {
       const uint32_t mid = rs_partition(v, o, len, pivot - o, true) + 1;
       if (mid > index) return true;
@@ -3470,7 +3477,7 @@ static adsr_status status_error(uint32_t st) {
     return set_error(ADSR_ERR_UNSUPPORTED,
                      "a precursor's top-k fragment selection needs the median-of-"
                      "medians fallback of select_nth_unstable (more than 16 "
-                     "partition rounds)");
+                     "partition rounds) or has 4096 or more kept fragments");
   return ADSR_OK;
 }
 
