@@ -2771,10 +2771,8 @@ struct Workspace {
   size_t nfs = 0;             // key_scratch entries
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  // chunked searches: plans on stream2, chunk k's select waits for ev_plan[k]
+  // pipelined host tables: the column copies' stream
   hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_plan[kMaxChunks] = {};
   // pipelined host tables: chunk k done (ev_pipe[k]); page-locked copies of
   // the small words after each chunk (hsmall + kSmallWords * k)
   hipEvent_t ev_pipe[kPipeMax] = {};
@@ -2810,12 +2808,9 @@ static void ws_free(Workspace &w) {
   (void)hipFree(w.table);
   for (auto &e : w.ev)
     if (e) (void)hipEventDestroy(e);
-  for (auto &e : w.ev_plan)
-    if (e) (void)hipEventDestroy(e);
   for (auto &e : w.ev_pipe)
     if (e) (void)hipEventDestroy(e);
   if (w.hsmall) (void)hipHostFree(w.hsmall);
-  if (w.ev_fork) (void)hipEventDestroy(w.ev_fork);
   if (w.stream2) (void)hipStreamDestroy(w.stream2);
   if (w.stream) (void)hipStreamDestroy(w.stream);
   w = Workspace();
@@ -2823,10 +2818,10 @@ static void ws_free(Workspace &w) {
 
 // Libraries of more than kChunkAbove rows are searched in chunks of at most
 // kChunkRows rows (ADSR_CHUNK_ROWS overrides the chunk size; 0 turns
-// chunking off): chunk k + 1's plan runs on a second stream while chunk k
-// is selected, and one compaction over all rows keeps library order.
+// chunking off): each chunk is planned and selected in turn, and one
+// compaction over all rows keeps library order.
 constexpr uint32_t kChunkAbove = 1u << 20;
-constexpr uint32_t kChunkRows = 1u << 19;
+constexpr uint32_t kChunkRows = 1u << 20;  // 2 chunks at 2M rows: 1.7% over one launch, 4 chunks 4%
 
 static uint32_t search_chunks(uint32_t n_rows) {
   static const long env = [] {
@@ -2898,13 +2893,8 @@ static adsr_status ws_alloc(Workspace &w, size_t n_rows, size_t ks, size_t cc,
   const size_t nb = (R + kScanBlock - 1) / kScanBlock;
   HIP_TRY(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
   for (auto &e : w.ev) HIP_TRY(hipEventCreate(&e));
-  if (search_chunks((uint32_t)n_rows) > 1 || pipe_chunks((uint32_t)n_rows) > 1)
-    HIP_TRY(hipStreamCreateWithFlags(&w.stream2, hipStreamNonBlocking));
-  if (search_chunks((uint32_t)n_rows) > 1) {
-    HIP_TRY(hipEventCreateWithFlags(&w.ev_fork, hipEventDisableTiming));
-    for (auto &e : w.ev_plan) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  }
   if (pipe_chunks((uint32_t)n_rows) > 1) {
+    HIP_TRY(hipStreamCreateWithFlags(&w.stream2, hipStreamNonBlocking));
     for (auto &e : w.ev_pipe) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (alloc_should_fail() ||
         hipHostMalloc((void **)&w.hsmall, kSmallWords * 4 * kPipeMax,
@@ -3216,28 +3206,14 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
     if (pl.b0) pl.b0 += (size_t)r0 * kObsMax * pl.ks;
   };
   const uint32_t nchunk =
-      (w.stream2 && !dbg && !with_bins && !plan_only) ? search_chunks(n_rows) : 1u;
+      (!dbg && !with_bins && !plan_only) ? search_chunks(n_rows) : 1u;
   const uint32_t crows = (n_rows + nchunk - 1) / nchunk;
   if (n_rows && plan_only) {
     launch_plan();
   } else if (n_rows) {
-    if (nchunk > 1) {
-      // the plans of all chunks on the second stream, each chunk's select
-      // waiting for its plan below
-      HIP_TRY(hipEventRecord(w.ev_fork, s));
-      HIP_TRY(hipStreamWaitEvent(w.stream2, w.ev_fork, 0));
-      for (uint32_t k = 0; k < nchunk; ++k) {
-        const uint32_t r0 = k * crows, n = std::min(crows, n_rows - r0);
-        KLib lb;
-        KParams pm;
-        KPlan pl;
-        chunk_args(r0, n, lb, pm, pl);
-        plan_rows(lb, pm, pl, n, w.stream2);
-        HIP_TRY(hipEventRecord(w.ev_plan[k], w.stream2));
-      }
-    } else {
-      launch_plan();
-    }
+    // a chunked search plans each chunk right before its select (below):
+    // a plan running beside a select slows both about as much as it gains
+    if (nchunk == 1) launch_plan();
     // the select kernel finds bin and CSR ranges itself; the per-window
     // range table is only built for the byte count and the backward pass
     if (with_bins) {
@@ -3357,8 +3333,8 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
       return ADSR_OK;
     }
     if (nchunk > 1) {
-      // chunk k's rows through the first select launch once its plan is
-      // done; long rows of every chunk go to the one long-row launch
+      // chunk k's plan, then its rows through the first select launch;
+      // long rows of every chunk go to the one long-row launch
       const SelArgs b = a;
       for (uint32_t k = 0; k < nchunk; ++k) {
         const uint32_t r0 = k * crows, n = std::min(crows, n_rows - r0);
@@ -3371,7 +3347,7 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
         x.cand_count += r0;
         x.row0 = r0;
         if (split) x.lds_per_wave = fit;
-        HIP_TRY(hipStreamWaitEvent(s, w.ev_plan[k], 0));
+        plan_rows(x.lib, x.prm, x.plan, n, s);
         launch_select(x);
       }
       if (split) {

@@ -1,6 +1,6 @@
 """Chunked searches: a library searched as device-resident chunks of rows
-(plans of all chunks on a second stream, chunk k's select once its plan is
-done, one long-row launch and one compaction over all rows).  Libraries of
+(each chunk planned and selected in turn, one long-row launch and one
+compaction over all rows).  Libraries of
 more than 1M rows are chunked by default (workload 4, covered at full size
 by test_gpu_fullsize.py); here ADSR_CHUNK_ROWS=700 forces 1,000 rows into
 two chunks in a child process, on workload 4's RT windows (rows too long for
