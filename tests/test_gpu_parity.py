@@ -623,3 +623,57 @@ def test_many_fragments_per_tile_with_empty_ones_bitwise(ads):
     want = oracle.select(od, ol, prm)
     assert len(want[0]) > 0
     assert_tables_equal(got, want, "40 fragments, empty ones interleaved")
+
+
+@pytest.mark.parametrize("kernel_size", [20, 16, 28, 30])
+def test_few_cell_convolution_scores_bitwise(ads, kernel_size):
+    """Sparse XICs (a few hit cells per fragment row, pairs of cells at every
+    distance, including the midpoints where two cells form one pair term) with
+    negative (some scores NaN), zero and subnormal intensities, and a few huge ones (1e30, for which
+    the device keeps the reference's full chain), at half widths 10, 8, 14 and
+    15 (where every block takes the full chain): whole score rows bit for bit
+    against the oracle, then the candidate table."""
+    import ctypes
+    import alphadia_search_rs._native as N
+    spec = synthetic.config(0, n_cycles=300, n_windows=4, peaks_per_spectrum=500,
+                            n_targets=400, half_window_min=60, half_window_max=60,
+                            kernel_size=kernel_size)
+    gen = synthetic.generate(spec)
+    dia = list(gen["dia"])
+    it = dia[8].copy()
+    it[::5] = -it[::5]
+    it[1::7] = 0.0
+    it[2::11] = np.float32(3e-39)
+    it[3::4001] = np.float32(1e30)
+    dia[8] = it
+    gen = dict(gen, dia=tuple(dia))
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    p = params_for(spec)
+    n, stride = 400, 128
+    scores = np.zeros((n, stride), np.float32)
+    cs = np.zeros(n, np.uint64)
+    ce = np.zeros(n, np.uint64)
+    c = p._c()
+    N.check(N.lib.adsr_select_scores(d._h, lib._h, ctypes.byref(c), 0, n, stride,
+                                     N.ptr(scores), N.ptr(cs), N.ptr(ce)))
+    od, ol = oracle.Dia(gen["dia"]), oracle.Lib(gen["lib"])
+    prm = oracle_params_for(spec)
+    checked = 0
+    for r in range(n):
+        one = oracle.select_one(od, ol, prm, r)
+        if one is None:
+            assert ce[r] <= cs[r]
+            continue
+        L = one["cycle_stop"] - one["cycle_start"]
+        assert cs[r] == one["cycle_start"] and L <= stride
+        got, want = scores[r, :L], one["score"]
+        # ln of a negative argument is NaN, whose sign and payload the
+        # candidate table never shows (NaN is never a strict maximum)
+        same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+        bad = np.flatnonzero(~same)
+        assert bad.size == 0, f"row {r}: score differs at cycles {bad[:8]}"
+        checked += 1
+    assert checked > 300
+    got = ads.PeakGroupSelection(p).search(d, lib).to_arrays()
+    assert_tables_equal(got, oracle.select(od, ol, prm), f"kernel_size {kernel_size}")
