@@ -1041,24 +1041,6 @@ __device__ __forceinline__ void list_slots(uint32_t hm, bool active, uint32_t &p
   }
 }
 
-// One step of a wave-wide arg-max over (value, position) pairs through a
-// DPP lane permutation; lanes without a source keep their own pair.
-// Order: larger value first, then the lower position; ~0u marks no pair.
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ void argmax_dpp(float &bv, uint32_t &bi) {
-  const int vb = __builtin_bit_cast(int, bv);
-  const float ov = __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(vb, vb, CTRL, ROWMASK, 0xf, false));
-  const uint32_t oi = (uint32_t)__builtin_amdgcn_update_dpp(
-      (int)bi, (int)bi, CTRL, ROWMASK, 0xf, false);
-  const bool take = oi != 0xffffffffu &&
-                    (bi == 0xffffffffu || ov > bv || (ov == bv && oi < bi));
-  if (take) {
-    bv = ov;
-    bi = oi;
-  }
-}
-
 // Inclusive prefix maximum over the wave's lanes through DPP: row_shr 1, 2,
 // 4, 8 within each row of 16 lanes, then row_bcast 15 and 31 carry rows
 // 0..2 forward.  Lanes without a source keep their own value (max is
@@ -1073,18 +1055,26 @@ __device__ __forceinline__ uint32_t wave_incl_max_dpp(uint32_t x) {
   return x;
 }
 
-// Wave-wide arg-max: row_shr 1, 2, 4, 8 reduce each row of 16 lanes into
-// its lane 15; row_bcast 15 and 31 carry rows 0..2 into lane 63.  The
-// result is returned in every lane.
-__device__ __forceinline__ void wave_argmax(float &bv, uint32_t &bi) {
-  argmax_dpp<0x111, 0xf>(bv, bi);
-  argmax_dpp<0x112, 0xf>(bv, bi);
-  argmax_dpp<0x114, 0xf>(bv, bi);
-  argmax_dpp<0x118, 0xf>(bv, bi);
-  argmax_dpp<0x142, 0xa>(bv, bi);
-  argmax_dpp<0x143, 0xc>(bv, bi);
-  bv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, bv), 63));
-  bi = (uint32_t)__builtin_amdgcn_readlane((int)bi, 63);
+// Order-preserving u32 image of a float that is not NaN: a larger float
+// gives a larger key, -0 and +0 give the same key, and 0 lies below the key
+// of every float.
+__device__ __forceinline__ uint32_t f32_order_key(float v) {
+  const uint32_t u = v == 0.0f ? 0u : __builtin_bit_cast(uint32_t, v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Wave-wide arg-max over one (key, position) pair per lane: the largest
+// key, ties to the lowest position; key 0 marks a lane without a pair.
+// Returns the winning position in every lane, ~0u when no lane has a pair.
+__device__ __forceinline__ uint32_t wave_argmax_key(uint32_t key, uint32_t pos) {
+  const uint32_t m = __builtin_amdgcn_readlane((int)wave_incl_max_dpp(key), 63);
+  if (m == 0u) return 0xffffffffu;
+  const uint64_t tie = ballot64(key == m);
+  if ((tie & (tie - 1)) == 0ull)
+    return (uint32_t)__builtin_amdgcn_readlane((int)pos, (int)__builtin_ctzll(tie));
+  // several lanes hold the largest key: the lowest position among them
+  const uint32_t c = key == m ? ~pos : 0u;
+  return ~(uint32_t)__builtin_amdgcn_readlane((int)wave_incl_max_dpp(c), 63);
 }
 
 // Occupancy: at most 128 VGPRs (4 waves per SIMD) and, with the default
@@ -1246,7 +1236,6 @@ void select_kernel(SelArgs a) {
         continue;
       }
       const WaveLds l = carve(wave_lds, L, nsel, a.lds_per_wave);
-      const uint32_t nw = (L + 63) / 64;  // 64-cycle words per row
       for (uint32_t i = 4 * lane; i < L; i += 4 * kWave)
         *reinterpret_cast<float4 *>(l.R + i) = make_float4(0.f, 0.f, 0.f, 0.f);
       float k10[11];
@@ -1618,7 +1607,6 @@ void select_kernel(SelArgs a) {
         __builtin_amdgcn_wave_barrier();
         // ---- compaction of (fragment, block) items, fragment-major -------
         uint32_t nitems = 0;
-        const uint32_t nbw = (nw * 16 + 63) / 64;  // 64-block words per row
         auto flush = [&](uint32_t n) {
           // conv + ln + weighted score, 1 block per lane per pass; score
           // updates are applied fragment by fragment (reference row order)
@@ -1716,10 +1704,23 @@ void select_kernel(SelArgs a) {
           }
           __builtin_amdgcn_wave_barrier();
         };
+        // The group's block words (G x wb, fragment-major; a row of L cycles
+        // has ceil(L / 256) = wb words) are read 64 at a time, one per lane,
+        // and each word is then taken from its lane: no LDS round trip per
+        // word.
+        const uint32_t nwords = G * l.wb;
+        uint32_t wlo = 0, whi = 0;
         for (uint32_t f = 0; f < G; ++f) {
-          for (uint32_t q = 0; q < nbw; ++q) {
-            const uint64_t bw = l.BD[f * l.wb + q];
-            const uint32_t blo = uni((uint32_t)bw), bhi = uni((uint32_t)(bw >> 32));
+          for (uint32_t q = 0; q < l.wb; ++q) {
+            const uint32_t wq = f * l.wb + q;
+            if ((wq & (uint32_t)(kWave - 1)) == 0u) {
+              const uint32_t wi = wq + (uint32_t)lane;
+              const uint64_t bw = wi < nwords ? l.BD[wi] : 0ull;
+              wlo = (uint32_t)bw;
+              whi = (uint32_t)(bw >> 32);
+            }
+            const uint32_t blo = rdlane(wlo, wq & (uint32_t)(kWave - 1));
+            const uint32_t bhi = rdlane(whi, wq & (uint32_t)(kWave - 1));
             if ((blo | bhi) == 0u) continue;
             const uint32_t before =
                 __builtin_amdgcn_mbcnt_hi(bhi, __builtin_amdgcn_mbcnt_lo(blo, 0u));
@@ -1745,8 +1746,12 @@ void select_kernel(SelArgs a) {
       }
       // ---- local maxima and top-k: mod.rs:23-64, 167-192 -----------------
       // lane l tests positions 256m + 4l + k (k < 4) from three float4
-      // reads; bit 4m + k of ismax marks a strict 5-point maximum
+      // reads; bit 4m + k of ismax marks a strict 5-point maximum.  Each
+      // lane keeps its best maximum (lbv, lbi): the largest score, ties to
+      // the lower position (positions are visited in increasing order).
       uint64_t ismax = 0;
+      float lbv = 0.0f;
+      uint32_t lbi = 0xffffffffu;
       for (uint32_t m = 0; 256 * m < L; ++m) {
         const uint32_t i0 = 256 * m + 4 * lane;
         if (i0 < L) {
@@ -1760,33 +1765,45 @@ void select_kernel(SelArgs a) {
             const uint32_t i = i0 + k;
             const float v = w[4 + k];
             if (i >= 2 && i + 2 < L && w[2 + k] < w[3 + k] && w[3 + k] < v &&
-                v > w[5 + k] && w[5 + k] > w[6 + k])
+                v > w[5 + k] && w[5 + k] > w[6 + k]) {
               ismax |= 1ull << (4 * m + k);
+              if (lbi == 0xffffffffu || v > lbv) {
+                lbv = v;
+                lbi = i;
+              }
+            }
           }
         }
       }
       if (more) stage3();
-      // ranks: descending score, ties to the lower cycle (stable sort)
+      // ranks: descending score, ties to the lower cycle (stable sort).  A
+      // maximum is never NaN (it compares greater than its neighbours), so
+      // the order key ranks the lanes' best maxima exactly; only the lane
+      // that supplied a rank looks for its next best.
       flush_out();  // rows without extraction still own pending outputs
       uint32_t ncand = 0;
       for (uint32_t r = 0; r < cc; ++r) {
-        float bv = 0.0f;
-        uint32_t bi = 0xffffffffu;
-        uint64_t it = ismax;
-        while (it) {
-          const uint32_t b = __builtin_ctzll(it);
-          it &= it - 1;
-          const uint32_t i = 256 * (b >> 2) + 4 * lane + (b & 3);
-          const float v = l.R[i];
-          if (bi == 0xffffffffu || v > bv) {
-            bv = v;
-            bi = i;
+        const uint32_t bi =
+            wave_argmax_key(lbi == 0xffffffffu ? 0u : f32_order_key(lbv), lbi);
+        if (bi == 0xffffffffu) break;
+        const uint32_t wl = (bi & 255u) >> 2;
+        const float bv = __builtin_bit_cast(
+            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, lbv), (int)wl));
+        if ((uint32_t)lane == wl) {
+          ismax &= ~(1ull << (((bi >> 8) << 2) | (bi & 3u)));
+          lbi = 0xffffffffu;
+          uint64_t it = ismax;
+          while (it) {
+            const uint32_t b = __builtin_ctzll(it);
+            it &= it - 1;
+            const uint32_t i = 256 * (b >> 2) + 4 * lane + (b & 3);
+            const float v = l.R[i];
+            if (lbi == 0xffffffffu || v > lbv) {
+              lbv = v;
+              lbi = i;
+            }
           }
         }
-        wave_argmax(bv, bi);
-        if (bi == 0xffffffffu) break;
-        if (((bi & 255u) >> 2) == (uint32_t)lane)
-          ismax &= ~(1ull << (((bi >> 8) << 2) | (bi & 3u)));
         if (r < (uint32_t)kWave) {
           if ((uint32_t)lane == r) {
             out_s = bv;
