@@ -904,6 +904,7 @@ struct WaveLds {
   uint32_t lp;    // XIC row stride (floats)
   uint32_t wb;    // BD row stride (u64 words)
   uint32_t gcap;  // fragment rows in the XIC tile
+  uint32_t tile_bytes;  // bytes from X to the end of the wave's share
 };
 
 // A wave's LDS for one row of L <= lcap cycles and nsel fragments, carved
@@ -918,13 +919,10 @@ __device__ __forceinline__ WaveLds carve(char *base, uint32_t L, uint32_t nsel,
   l.lp = sh.lp;
   l.wb = sh.wb;
   l.gcap = max(1u, min(nsel, (bytes - lds_fixed(l.lrow)) / lds_per_frag(l.lrow)));
+  // the parts every row has first, then the XIC tile (dense or block-sparse)
   char *p = base;
   l.R = (float *)p + 4;
   p += (l.lrow + 12) * 4;
-  l.X = (float *)p;
-  p += l.gcap * l.lp * 4;
-  l.BD = (uint64_t *)p;
-  p += l.gcap * l.wb * 8;
   l.hits = (uint2 *)p;
   p += kHitCap * 8;
   l.items = (uint16_t *)p;
@@ -933,8 +931,44 @@ __device__ __forceinline__ WaveLds carve(char *base, uint32_t L, uint32_t nsel,
   p += 32 * 4;
   l.w = (float *)p;
   p += kMaxSel * 4;
+  l.X = (float *)p;
+  l.tile_bytes = bytes - (uint32_t)(p - base);
+  p += l.gcap * l.lp * 4;
+  l.BD = (uint64_t *)p;
+  p += l.gcap * l.wb * 8;
   l.own = (uint8_t *)p;
   return l;
+}
+
+// The block-sparse XIC tile of a row: every selected fragment at once, with
+// only the 4-cycle cell blocks that hold a hit stored.  Per fragment: NZB, a
+// bit per cell block that holds a hit; BD, a bit per 4-output block that a
+// hit reaches (the convolution's work items); ST, the storage slot of each
+// cell block (byte b + 4 for block b, -4 <= b < nb + 4; 0 = the zero block).
+// CB holds the slots' cells (slot 0 all zero) and own a claim byte per cell.
+struct SparseTile {
+  uint64_t *NZB;
+  uint64_t *BD;
+  uint8_t *ST;
+  float4 *CB;
+  uint8_t *own;
+  uint32_t sts;  // ST bytes per fragment
+  uint32_t cap;  // slots (including the zero block), at most 255
+};
+
+__device__ __forceinline__ SparseTile carve_sparse(const WaveLds &l, uint32_t nsel) {
+  SparseTile s;
+  char *p = reinterpret_cast<char *>(l.X);
+  const uint32_t words = nsel * l.wb;
+  s.sts = l.lrow / 4 + 8;
+  s.NZB = reinterpret_cast<uint64_t *>(p);
+  s.BD = s.NZB + words;
+  s.ST = reinterpret_cast<uint8_t *>(s.BD + words);
+  const uint32_t head = (16u * words + nsel * s.sts + 15u) & ~15u;
+  s.CB = reinterpret_cast<float4 *>(p + head);
+  s.cap = head + 40u <= l.tile_bytes ? min(255u, (l.tile_bytes - head) / 20u) : 0u;
+  s.own = reinterpret_cast<uint8_t *>(s.CB + s.cap);
+  return s;
 }
 
 // small[0]: queue, [1]: status, [3]: queue of the long-row launch,
@@ -978,16 +1012,17 @@ __device__ __forceinline__ bool skipped_duplicate(const KDia &dia, uint32_t o,
 }
 
 // Four consecutive outputs i0..i0+3 of src/convolution/scalar.rs:14-36
-// (s = 0 + x[i]*k[h], then s += (x[i-d] + x[i+d]) * k[h-d] for d = h..1),
-// from the zero-padded row: xs points at X[i0 - 12] (16-byte aligned).
-// The leading 0 + is dropped: it only turns a -0 product into +0, and an
-// output that ends up +-0 scores ln(+-0 + 1) = 0 either way.
-__device__ __forceinline__ void conv4_h10(const float *xs, const float (&k)[11],
+// (s = 0 + x[i]*k[h], then s += (x[i-d] + x[i+d]) * k[h-d] for d = h..1)
+// for h = 10: xb holds the cells i0 - 12 .. i0 + 15 of the zero-padded row
+// as seven 4-cell blocks.  The leading 0 + is dropped: it only turns a -0
+// product into +0, and an output that ends up +-0 scores ln(+-0 + 1) = 0
+// either way.
+__device__ __forceinline__ void conv4_h10(const float4 (&xb)[7], const float (&k)[11],
                                           float (&y)[4]) {
   float x[28];
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
-    const float4 v = *reinterpret_cast<const float4 *>(xs + 4 * q);
+    const float4 v = xb[q];
     x[4 * q + 0] = v.x;
     x[4 * q + 1] = v.y;
     x[4 * q + 2] = v.z;
@@ -1266,9 +1301,12 @@ void select_kernel(SelArgs a) {
       // Hit key: t (12 bits) | fragment (6) << 12 | window (2) << 18.
       uint32_t nh = 0;
       uint32_t gA = 0, gG = 0;  // fragment group being accumulated
+      // spm: the row's XIC is the block-sparse tile st (all fragments)
+      bool spm = false;
+      SparseTile st;
       // Adds the hits in list slots [h0, h1) of fragments [gA, gA + gG)
-      // into X; the list holds intensities (loaded) or window-relative
-      // entries of window `base`.
+      // into X (or the sparse tile); the list holds intensities (loaded) or
+      // window-relative entries of window `base`.
       auto accumulate = [&](bool loaded, uint64_t base, uint32_t h0, uint32_t h1) {
         for (uint32_t c = h0; c < h1; c += kWave) {
           const uint32_t idx = c + lane;
@@ -1279,14 +1317,25 @@ void select_kernel(SelArgs a) {
           const uint32_t fl = v ? f - gA : 0u;
           float inten = 0.0f;
           if (v) inten = loaded ? __builtin_bit_cast(float, h.x) : a.dia.intens[base + h.x];
-          uint8_t *ow = l.own + fl * l.lrow + t;
+          uint8_t *ow;
+          float *xc;  // the hit's XIC cell
+          uint64_t *br;
+          if (spm) {
+            const uint32_t slot = v ? st.ST[fl * st.sts + (t >> 2) + 4u] : 0u;
+            ow = st.own + slot * 4u + (t & 3u);
+            xc = reinterpret_cast<float *>(st.CB + slot) + (t & 3u);
+            br = st.BD + fl * l.wb;
+          } else {
+            ow = l.own + fl * l.lrow + t;
+            xc = l.X + fl * l.lp + kXPad + t;
+            br = l.BD + fl * l.wb;
+          }
           if (v) *ow = (uint8_t)lane;
           __builtin_amdgcn_wave_barrier();
           // lanes whose cell another lane overwrote share it with that lane
           const bool lost = v && *ow != (uint8_t)lane;
-          float *xr = l.X + fl * l.lp + kXPad;
           if (ballot64(lost) == 0) {
-            if (v) xr[t] = xr[t] + inten;
+            if (v) *xc = *xc + inten;
           } else {
             // The losers write again, so that the lane that kept a shared
             // cell reads another id: every lane of a shared cell is then
@@ -1295,13 +1344,13 @@ void select_kernel(SelArgs a) {
             if (lost) *ow = (uint8_t)lane;
             __builtin_amdgcn_wave_barrier();
             const bool shared = lost || (v && *ow != (uint8_t)lane);
-            if (v && !shared) xr[t] = xr[t] + inten;
+            if (v && !shared) *xc = *xc + inten;
             __builtin_amdgcn_wave_barrier();
             uint64_t it2 = ballot64(shared);
             while (it2) {
               const int j2 = __builtin_ctzll(it2);
               it2 &= it2 - 1;
-              if (lane == j2) xr[t] = xr[t] + inten;
+              if (lane == j2) *xc = *xc + inten;
               __builtin_amdgcn_wave_barrier();
             }
           }
@@ -1312,7 +1361,6 @@ void select_kernel(SelArgs a) {
             const int ihi = min((int)(t + half), (int)L - (int)half - 1);
             if (ilo <= ihi) {
               const uint32_t blo = (uint32_t)ilo >> 2, bhi = (uint32_t)ihi >> 2;
-              uint64_t *br = l.BD + fl * l.wb;
               const uint32_t wlo = blo >> 6, whi = bhi >> 6;
               const uint64_t ones = ~0ull;
               if (wlo == whi) {
@@ -1578,16 +1626,55 @@ void select_kernel(SelArgs a) {
       }
 
       if (more) stage2();
+      // A row whose hits are all listed takes the block-sparse tile when its
+      // cell blocks fit: every fragment in one group.  Each hit marks its
+      // cell block; the first hit of a block (an LDS atomic's old value)
+      // gives it the next storage slot, in list order.  Otherwise the dense
+      // tile, gcap fragments at a time.
+      if (HALF == 10 && nobs > 0 && nsel > 0 && fstop == nsel) {
+        st = carve_sparse(l, nsel);
+        if (st.cap >= 2u) {
+          const uint32_t w64 = 2u * nsel * l.wb;  // NZB and BD
+          for (uint32_t i = lane; i < w64; i += kWave) st.NZB[i] = 0ull;
+          uint2 *stz = reinterpret_cast<uint2 *>(st.ST);
+          for (uint32_t i = lane; i < nsel * st.sts / 8u; i += kWave) stz[i] = make_uint2(0u, 0u);
+          if (lane == 0) st.CB[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+          __builtin_amdgcn_wave_barrier();
+          uint32_t ns = 0;  // slots given out (slot 0 excluded)
+          for (uint32_t c = 0; c < nh; c += kWave) {
+            const uint32_t idx = c + (uint32_t)lane;
+            const bool v = idx < nh;
+            const uint32_t key = v ? l.hits[idx].y : 0u;
+            const uint32_t f = (key >> 12) & 63u, be = (key & 0xfffu) >> 2;
+            uint64_t old = ~0ull;
+            if (v)
+              old = atomicOr((unsigned long long *)&st.NZB[f * l.wb + (be >> 6)],
+                             1ull << (be & 63u));
+            const bool first = v && ((old >> (be & 63u)) & 1ull) == 0ull;
+            const uint64_t fb = ballot64(first);
+            const uint32_t slot = ns + 1u +
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(fb >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)fb, 0u));
+            if (first && slot < st.cap) {
+              st.ST[f * st.sts + be + 4u] = (uint8_t)slot;
+              st.CB[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            ns += (uint32_t)__builtin_popcountll(fb);
+          }
+          __builtin_amdgcn_wave_barrier();
+          spm = ns < st.cap;
+        }
+      }
       // With one window the list is in fragment order, so each group's
       // hits follow the previous group's: [hcur, hend)
       uint32_t hcur = 0;
       for (uint32_t g0 = 0, G = 0; g0 < nsel && nobs > 0; g0 += G) {
         const bool listed = g0 < fstop;
-        G = min(l.gcap, (listed ? fstop : nsel) - g0);
-        zero_group(G);
+        G = spm ? nsel : min(l.gcap, (listed ? fstop : nsel) - g0);
+        if (!spm) zero_group(G);
         gA = g0;
         gG = G;
-        if (listed && nobs == 1) {
+        if (listed && nobs == 1 && !spm) {
           uint32_t hend = hcur;
           for (;;) {
             const uint32_t idx = hend + (uint32_t)lane;
@@ -1611,7 +1698,7 @@ void select_kernel(SelArgs a) {
           // conv + ln + weighted score, 1 block per lane per pass; score
           // updates are applied fragment by fragment (reference row order)
           for (uint32_t c = 0; c < n; c += kWave) {
-            if (HALF == 10 && n - c <= (uint32_t)kTailItems) {
+            if (HALF == 10 && !spm && n - c <= (uint32_t)kTailItems) {
               // The last items, one output per lane (4 lanes per item): a
               // pass of 64 blocks for them would leave most lanes idle.
               // Each output is the block pass's chain for that output.
@@ -1653,8 +1740,27 @@ void select_kernel(SelArgs a) {
             const uint32_t f = item >> 10, i0 = (item & 0x3ffu) * 4;
             const float *row = l.X + f * l.lp + kXPad;
             float y[4];
-            if (HALF == 10) conv4_h10(row + i0 - 12, k10, y);
-            else conv4_any(row, i0, half, l.kern, y);
+            if (HALF == 10) {
+              float4 xb[7];  // cells i0 - 12 .. i0 + 15
+              if (spm) {
+                // slots of cell blocks b - 3 .. b + 3: ST bytes sa .. sa + 6
+                const uint32_t sa = f * st.sts + (item & 0x3ffu) + 1u;
+                const uint32_t *stw = reinterpret_cast<const uint32_t *>(st.ST) + (sa >> 2);
+                const uint32_t w0 = stw[0], w1 = stw[1], w2 = stw[2];
+                const uint32_t u0 = __builtin_amdgcn_alignbyte(w1, w0, sa & 3u);
+                const uint32_t u1 = __builtin_amdgcn_alignbyte(w2, w1, sa & 3u);
+#pragma unroll
+                for (int j = 0; j < 7; ++j)
+                  xb[j] = st.CB[((j < 4 ? u0 : u1) >> (8 * (j & 3))) & 0xffu];
+              } else {
+                const float4 *xs = reinterpret_cast<const float4 *>(row + i0 - 12);
+#pragma unroll
+                for (int j = 0; j < 7; ++j) xb[j] = xs[j];
+              }
+              conv4_h10(xb, k10, y);
+            } else {
+              conv4_any(row, i0, half, l.kern, y);
+            }
             const float wf = l.w[g0 + f];
             // outputs outside [half, L - half) stay zero (scalar.rs:30-38)
             float t[4], u[4];
@@ -1709,13 +1815,14 @@ void select_kernel(SelArgs a) {
         // and each word is then taken from its lane: no LDS round trip per
         // word.
         const uint32_t nwords = G * l.wb;
+        const uint64_t *const bdw = spm ? st.BD : l.BD;
         uint32_t wlo = 0, whi = 0;
         for (uint32_t f = 0; f < G; ++f) {
           for (uint32_t q = 0; q < l.wb; ++q) {
             const uint32_t wq = f * l.wb + q;
             if ((wq & (uint32_t)(kWave - 1)) == 0u) {
               const uint32_t wi = wq + (uint32_t)lane;
-              const uint64_t bw = wi < nwords ? l.BD[wi] : 0ull;
+              const uint64_t bw = wi < nwords ? bdw[wi] : 0ull;
               wlo = (uint32_t)bw;
               whi = (uint32_t)(bw >> 32);
             }
