@@ -1001,14 +1001,17 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) {
 // True when entry e (relative to the window) of a hit with cycle == c0 is
 // followed, within its bin, by another entry with cycle == c0: the
 // reference's binary_search(c0) starts at the last such entry
-// (src/quadrupole_observation/mod.rs:92-94).  Rare; reads global memory.
+// (src/quadrupole_observation/mod.rs:92-94).  `next` is the cycle of the
+// window's entry e + 1; only when it equals c0 is e's bin found (a walk over
+// the fragment's bins from b0, in global memory).
 __device__ __forceinline__ bool skipped_duplicate(const KDia &dia, uint32_t o,
-                                                  uint64_t base, uint32_t b0,
-                                                  uint32_t e, uint32_t c0) {
+                                                  uint32_t b0, uint32_t e,
+                                                  uint32_t c0, uint32_t next) {
+  if (next != c0) return false;
   const uint32_t *ss = dia.ss + (size_t)o * (dia.n_grid + 1);
   uint32_t b = b0;
   while (ss[b + 1] <= e) ++b;
-  return e + 1 < ss[b + 1] && dia.cycles[base + e + 1] == c0;
+  return e + 1 < ss[b + 1];
 }
 
 // Four consecutive outputs i0..i0+3 of src/convolution/scalar.rs:14-36
@@ -1504,7 +1507,9 @@ void select_kernel(SelArgs a) {
                 for (int k = 0; k < 8; ++k) {
                   if (((hm >> k) & 1u) && cyc[k] == c0) {
                     const uint32_t e = 8 * tg[u] + k - bo;
-                    if (skipped_duplicate(a.dia, o, base, tb0[u], e, c0))
+                    const uint32_t next = k < 7 ? cyc[k < 7 ? k + 1 : 7]
+                                                : (uint32_t)a.dia.cycles[base + e + 1];
+                    if (skipped_duplicate(a.dia, o, tb0[u], e, c0, next))
                       hm &= ~(1u << k);
                   }
                 }
@@ -2437,7 +2442,9 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
           if (((hm >> k) & 1u) && cyc[k] == c0 &&
-              skipped_duplicate(a.dia, o, base, fb0, 8 * tg + k - bo, c0))
+              skipped_duplicate(a.dia, o, fb0, 8 * tg + k - bo, c0,
+                                k < 7 ? cyc[k < 7 ? k + 1 : 7]
+                                      : (uint32_t)a.dia.cycles[base + 8 * tg + k - bo + 1]))
             hm &= ~(1u << k);
         }
       }
