@@ -905,39 +905,50 @@ struct WaveLds {
   uint32_t wb;    // BD row stride (u64 words)
   uint32_t gcap;  // fragment rows in the XIC tile
   uint32_t tile_bytes;  // bytes from X to the end of the wave's share
+  uint32_t hcap;  // hit list capacity (entries)
+  uint32_t room;  // bytes from the hit list to the end of the wave's share
 };
 
 // A wave's LDS for one row of L <= lcap cycles and nsel fragments, carved
-// from the row's own length: the XIC tile holds as many fragment rows as
-// `bytes` leaves, so rows with short RT windows take more fragments per
-// group.  `bytes` >= lds_bytes(lcap, 1) guarantees at least one.
-__device__ __forceinline__ WaveLds carve(char *base, uint32_t L, uint32_t nsel,
-                                         uint32_t bytes) {
+// from the row's own length: the score row, the item list, kernel taps and
+// weights, then the hit list.  The hit list holds kHitCap hits, plus what
+// the share holds beyond the fixed parts and two dense fragment rows.  The
+// XIC tile (carve_tile) follows the part of the hit list in use, so that
+// rows with few hits or short RT windows take more fragments per group.
+// `bytes` >= lds_bytes(lcap, 1) guarantees at least one.
+__device__ __forceinline__ WaveLds carve(char *base, uint32_t L, uint32_t bytes) {
   WaveLds l;
   l.lrow = max(64u, (L + 63) / 64 * 64);
   const LdsShape sh = lds_shape(l.lrow);
   l.lp = sh.lp;
   l.wb = sh.wb;
-  l.gcap = max(1u, min(nsel, (bytes - lds_fixed(l.lrow)) / lds_per_frag(l.lrow)));
-  // the parts every row has first, then the XIC tile (dense or block-sparse)
+  const uint32_t need = lds_fixed(l.lrow) + 2u * lds_per_frag(l.lrow);
+  l.hcap = (uint32_t)kHitCap + (bytes > need ? (bytes - need) / 128u * 16u : 0u);
   char *p = base;
   l.R = (float *)p + 4;
   p += (l.lrow + 12) * 4;
-  l.hits = (uint2 *)p;
-  p += kHitCap * 8;
   l.items = (uint16_t *)p;
   p += kItemCap * 2;
   l.kern = (float *)p;
   p += 32 * 4;
   l.w = (float *)p;
   p += kMaxSel * 4;
+  l.hits = (uint2 *)p;
+  l.room = bytes - (uint32_t)(p - base);
+  return l;
+}
+
+// The dense XIC tile after the first `used` entries of the hit list.
+__device__ __forceinline__ void carve_tile(WaveLds &l, uint32_t used, uint32_t nsel) {
+  const uint32_t off = (used * 8u + 15u) / 16u * 16u;
+  char *p = reinterpret_cast<char *>(l.hits) + off;
   l.X = (float *)p;
-  l.tile_bytes = bytes - (uint32_t)(p - base);
+  l.tile_bytes = l.room - off;
+  l.gcap = max(1u, min(nsel, l.tile_bytes / lds_per_frag(l.lrow)));
   p += l.gcap * l.lp * 4;
   l.BD = (uint64_t *)p;
   p += l.gcap * l.wb * 8;
   l.own = (uint8_t *)p;
-  return l;
 }
 
 // The block-sparse XIC tile of a row: every selected fragment at once, with
@@ -1273,7 +1284,7 @@ void select_kernel(SelArgs a) {
         }
         continue;
       }
-      const WaveLds l = carve(wave_lds, L, nsel, a.lds_per_wave);
+      WaveLds l = carve(wave_lds, L, a.lds_per_wave);
       for (uint32_t i = 4 * lane; i < L; i += 4 * kWave)
         *reinterpret_cast<float4 *>(l.R + i) = make_float4(0.f, 0.f, 0.f, 0.f);
       float k10[11];
@@ -1523,7 +1534,7 @@ void select_kernel(SelArgs a) {
                 uint32_t pre, tot;
                 list_slots(hm, true, pre, tot);
                 if (tot == 0) continue;
-                if (nh + tot <= (uint32_t)kHitCap) {
+                if (nh + tot <= l.hcap) {
                   uint32_t pos = nh + pre, mm = hm;
                   const uint32_t key = (j << 18) | (tfr[u] << 12);
                   while (mm) {
@@ -1549,7 +1560,7 @@ void select_kernel(SelArgs a) {
                 uint32_t pre, tot;
                 list_slots(hm, (lane >> 5) == hh, pre, tot);
                 if (tot == 0) continue;
-                if (deferred && nh + tot > (uint32_t)kHitCap) {
+                if (deferred && nh + tot > l.hcap) {
                   if (nobs != 1) return fa;
                   // lane 32 hh scans this half-tile's first fragment
                   const uint32_t fcut = rdlane(tfr[u], 32 * hh);
@@ -1575,7 +1586,7 @@ void select_kernel(SelArgs a) {
                 }
                 nh += tot;
                 __builtin_amdgcn_wave_barrier();
-                if (!deferred && nh > (uint32_t)(kHitCap - kTileHits)) {
+                if (!deferred && nh > l.hcap - (uint32_t)kTileHits) {
                   accumulate(false, base, 0, nh);
                   nh = 0;
                 }
@@ -1631,6 +1642,9 @@ void select_kernel(SelArgs a) {
       }
 
       if (more) stage2();
+      // the list is kept whole while the tile is in use; fragments that did
+      // not fit it are extracted through the whole list capacity
+      carve_tile(l, fstop < nsel ? l.hcap : nh, nsel);
       // A row whose hits are all listed takes the block-sparse tile when its
       // cell blocks fit: every fragment in one group.  Each hit marks its
       // cell block; the first hit of a block (an LDS atomic's old value)
