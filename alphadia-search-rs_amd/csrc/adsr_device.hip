@@ -842,6 +842,12 @@ struct SelArgs {
   const uint32_t *rows;
   const uint32_t *n_rows_at;
   uint32_t row0;  // library row of row 0 (chunked searches): defer list entries
+  // differentiable forward: per row, the XIC around the rank-0 apex
+  // (xwin_stride floats: fragment-major, kWin cycles from apex - half) and
+  // whether it was written (rows of the dense tile are not)
+  float *xwin;
+  uint32_t *xwin_ok;
+  uint32_t xwin_stride;
 };
 
 // Per-wave LDS.  XIC rows have kXPad zero floats on both sides so that a
@@ -853,6 +859,9 @@ struct SelArgs {
 constexpr int kXPad = 16;
 constexpr int kItemCap = 512;   // u16 (fragment, block) items
 constexpr int kHitCap = 256;    // (u32 entry or intensity, u32 key) hits
+// cycles of XIC kept around a row's rank-0 apex for the backward pass
+// (2 * half + 1 <= 31)
+constexpr int kWin = 32;
 constexpr int kTileHits = 4 * 64;  // hits one extraction tile can add
 // extraction tiles in flight (8 entries per lane each); one measured
 // fastest: a second tile's registers cost more than its latency hiding
@@ -1132,7 +1141,9 @@ __device__ __forceinline__ uint32_t wave_argmax_key(uint32_t key, uint32_t pos) 
 // static LDS of select_kernel per block (the logf table)
 constexpr uint32_t kSelectStaticLds = 16 * sizeof(LogfEntry);
 
-template <int WAVES, int HALF>
+// SAVE (h = 10 only): the differentiable forward, which leaves each row's
+// apex XIC window for the backward pass (SelArgs::xwin)
+template <int WAVES, int HALF, bool SAVE = false>
 __global__ __launch_bounds__(WAVES *kWave) __attribute__((amdgpu_waves_per_eu(4)))
 void select_kernel(SelArgs a) {
   extern __shared__ __align__(16) char smem[];
@@ -1907,11 +1918,12 @@ void select_kernel(SelArgs a) {
       // the order key ranks the lanes' best maxima exactly; only the lane
       // that supplied a rank looks for its next best.
       flush_out();  // rows without extraction still own pending outputs
-      uint32_t ncand = 0;
+      uint32_t ncand = 0, apex = 0;
       for (uint32_t r = 0; r < cc; ++r) {
         const uint32_t bi =
             wave_argmax_key(lbi == 0xffffffffu ? 0u : f32_order_key(lbv), lbi);
         if (bi == 0xffffffffu) break;
+        if (r == 0) apex = bi;
         const uint32_t wl = (bi & 255u) >> 2;
         const float bv = __builtin_bit_cast(
             float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, lbv), (int)wl));
@@ -1940,6 +1952,32 @@ void select_kernel(SelArgs a) {
           a.cand_center[(size_t)row * cc + r] = c0 + bi;
         }
         ++ncand;
+      }
+      if (HALF == 10 && SAVE) {
+        // differentiable forward: a row in the block-sparse tile leaves the
+        // XIC cells around its rank-0 apex for the backward pass, which
+        // re-extracts the other rows' windows itself
+        const bool saved = spm && ncand > 0;
+        if (saved) {
+          // fragment-major, 2 * half + 1 cells from apex - half; two
+          // fragments per pass, one per half-wave
+          float *xo = a.xwin + (size_t)row * a.xwin_stride;
+          const int t0 = (int)apex - (int)half;
+          const uint32_t wlen = 2 * half + 1;
+          for (uint32_t f0 = 0; f0 < nsel; f0 += 2) {
+            const uint32_t f = f0 + ((uint32_t)lane >> 5), t = (uint32_t)lane & 31u;
+            if (f < nsel && t < wlen) {
+              const int cell = t0 + (int)t;
+              float v = 0.0f;
+              if (cell >= 0 && cell < (int)L) {
+                const uint32_t slot = st.ST[f * st.sts + ((uint32_t)cell >> 2) + 4u];
+                v = reinterpret_cast<const float *>(st.CB + slot)[cell & 3];
+              }
+              xo[f * wlen + t] = v;
+            }
+          }
+        }
+        a.xwin_ok[row] = saved ? 1u : 0u;  // every lane, the same value
       }
       out_row = row;
       out_n = ncand;
@@ -2237,9 +2275,11 @@ struct BwdArgs {
   float *grad_w;        // per fragment
   double *contrib;      // per row: dLoss/ds_star * ds_star/dsigma
   uint32_t *queue;
+  const float *xwin;    // apex XIC windows left by the forward (or nullptr)
+  const uint32_t *xwin_ok;
+  uint32_t xwin_stride;
 };
 
-constexpr int kWin = 32;  // cycles kept around the apex (2*half+1 <= 31)
 
 __global__ void fill_f32_kernel(float *p, uint64_t n, float v) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2331,6 +2371,7 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
   const uint32_t g_raw = __builtin_bit_cast(uint32_t, a.grad_s[row]);
   const uint32_t ncand_raw = a.cand_count[row];
   const uint32_t cen_raw = a.cand_center[(size_t)row * a.cc];
+  const uint32_t ok_raw = a.xwin_ok ? a.xwin_ok[row] : 0u;
   const bool fl = lane < (int)ks;
   const float smz_l = fl ? a.plan.sel_mz[(size_t)row * ks + lane] : 0.0f;
   const uint32_t sel_l = fl ? (uint32_t)a.plan.sel[(size_t)row * ks + lane] : 0u;
@@ -2349,12 +2390,14 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
   // cycles [hc0, hc1) of the apex window; x[t] = X[hc0 + t]
   const uint32_t hc0 = c0 + istar - half;
   const uint32_t wl = max(hc0, c0), wh = min(hc0 + 2 * half + 1, c1);
+  // the forward left this row's XIC window (block-sparse tile rows)
+  const bool saved = uni(ok_raw) != 0u;
   // bin range of fragment `lane`; window bases; kernel taps and their
   // sigma derivatives (lane j holds tap j)
   uint32_t bst = 0, ben = 0;
-  if (lane < (int)nsel) bin_range(a.dia, smz_l, a.prm.mass_tol, bst, ben);
+  if (!saved && lane < (int)nsel) bin_range(a.dia, smz_l, a.prm.mass_tol, bst, ben);
   const uint32_t nob = min(nobs, (uint32_t)kObsMax);
-  const uint64_t eb_l = lane < (int)nob ? a.dia.entry_base[obs_l] : 0ull;
+  const uint64_t eb_l = !saved && lane < (int)nob ? a.dia.entry_base[obs_l] : 0ull;
   const float kk_l = lane <= (int)half ? a.prm.kernels[(size_t)rec.w * a.prm.klen + lane] : 0.0f;
   const double dk_l = lane <= (int)half ? a.dk[(size_t)rec.w * (half + 1) + lane] : 0.0;
   // CSR entry range of fragment `lane` in window j
@@ -2368,13 +2411,24 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
       ry = ssw[ben];
     }
   };
-  uint32_t rx0, ry0;
-  csr_range(0, rx0, ry0);
-  for (uint32_t i = lane; i < nsel * kWin; i += kWave) xw[i] = 0.0f;
+  if (saved) {
+    // fragment-major, 2 * half + 1 cells per fragment
+    const float *xi = a.xwin + (size_t)row * a.xwin_stride;
+    const uint32_t wlen = 2 * half + 1;
+    for (uint32_t f0 = 0; f0 < nsel; f0 += 2) {
+      const uint32_t f = f0 + ((uint32_t)lane >> 5), t = (uint32_t)lane & 31u;
+      if (f < nsel && t < wlen) xw[f * kWin + t] = xi[f * wlen + t];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  uint32_t rx0 = 0, ry0 = 0;
+  if (!saved) csr_range(0, rx0, ry0);
+  if (!saved)
+    for (uint32_t i = lane; i < nsel * kWin; i += kWave) xw[i] = 0.0f;
   __builtin_amdgcn_wave_barrier();
   // the XIC window, summed in the forward pass's order: windows, then the
   // fragments' CSR entries in order (8-entry groups, 64 per tile)
-  for (uint32_t j = 0; j < nob; ++j) {
+  for (uint32_t j = 0; j < (saved ? 0u : nob); ++j) {
     const uint32_t o = rdlane(obs_l, j);
     const uint64_t base = ((uint64_t)rdlane((uint32_t)(eb_l >> 32), j) << 32) |
                           rdlane((uint32_t)eb_l, j);
@@ -3167,6 +3221,10 @@ struct SearchPlan {
   uint64_t max_raw = 0;  // longest raw fragment list of the library
   std::vector<float> kernels;
   size_t n_taps = 0;
+  // differentiable forward: apex XIC windows (SelArgs::xwin)
+  float *xwin = nullptr;
+  uint32_t *xwin_ok = nullptr;
+  uint32_t xwin_stride = 0;
 };
 
 // Largest number of RT-index points inside any closed window of width
@@ -3277,7 +3335,9 @@ static void set_lds_limits() {
     const void *fns[] = {
         (const void *)select_kernel<1, 10>, (const void *)select_kernel<2, 10>,
         (const void *)select_kernel<4, 10>, (const void *)select_kernel<1, 0>,
-        (const void *)select_kernel<2, 0>,  (const void *)select_kernel<4, 0>};
+        (const void *)select_kernel<2, 0>,  (const void *)select_kernel<4, 0>,
+        (const void *)select_kernel<1, 10, true>, (const void *)select_kernel<2, 10, true>,
+        (const void *)select_kernel<4, 10, true>};
     for (const void *fn : fns)
       (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)kMaxDynLds);
@@ -3386,6 +3446,9 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
     a.dbg_row0 = dbg_row0;
     a.dbg_rows = dbg_rows;
     a.row0 = 0;
+    a.xwin = sp.xwin;
+    a.xwin_ok = sp.xwin_ok;
+    a.xwin_stride = sp.xwin_stride;
     while (shape.waves > 1 &&
            (size_t)sp.lds_per_wave * shape.waves > kMaxDynLds)
       shape.waves >>= 1;
@@ -3412,13 +3475,17 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
     }
     const bool split = a.split_l < sp.lcap;
     const bool h10 = sp.prm.half == 10;
+    const bool save = h10 && sp.xwin != nullptr;
     const void *fn =
-        h10 ? (shape.waves == 1   ? (const void *)select_kernel<1, 10>
-               : shape.waves == 2 ? (const void *)select_kernel<2, 10>
-                                  : (const void *)select_kernel<4, 10>)
-            : (shape.waves == 1   ? (const void *)select_kernel<1, 0>
-               : shape.waves == 2 ? (const void *)select_kernel<2, 0>
-                                  : (const void *)select_kernel<4, 0>);
+        save ? (shape.waves == 1   ? (const void *)select_kernel<1, 10, true>
+                : shape.waves == 2 ? (const void *)select_kernel<2, 10, true>
+                                   : (const void *)select_kernel<4, 10, true>)
+        : h10 ? (shape.waves == 1   ? (const void *)select_kernel<1, 10>
+                 : shape.waves == 2 ? (const void *)select_kernel<2, 10>
+                                    : (const void *)select_kernel<4, 10>)
+              : (shape.waves == 1   ? (const void *)select_kernel<1, 0>
+                 : shape.waves == 2 ? (const void *)select_kernel<2, 0>
+                                    : (const void *)select_kernel<4, 0>);
     auto launch_select_on = [&](const SelArgs &x, hipStream_t s) {
       const size_t lds = (size_t)x.lds_per_wave * shape.waves;
       const int per_cu = occupancy_blocks(fn, shape.waves * kWave, lds);
@@ -3426,7 +3493,11 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
       const uint32_t blocks = (uint32_t)std::max<uint64_t>(
           1, std::min<uint64_t>((uint64_t)per_cu * sp.cus,
                                 (want + shape.waves - 1) / shape.waves));
-      if (h10) {
+      if (save) {
+        if (shape.waves == 1) select_kernel<1, 10, true><<<blocks, 64, lds, s>>>(x);
+        else if (shape.waves == 2) select_kernel<2, 10, true><<<blocks, 128, lds, s>>>(x);
+        else select_kernel<4, 10, true><<<blocks, 256, lds, s>>>(x);
+      } else if (h10) {
         if (shape.waves == 1) select_kernel<1, 10><<<blocks, 64, lds, s>>>(x);
         else if (shape.waves == 2) select_kernel<2, 10><<<blocks, 128, lds, s>>>(x);
         else select_kernel<4, 10><<<blocks, 256, lds, s>>>(x);
@@ -3459,6 +3530,10 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
         x.cand_center += (size_t)r0 * cc;
         x.cand_count += r0;
         x.row0 = r0;
+        if (x.xwin) {
+          x.xwin += (size_t)r0 * x.xwin_stride;
+          x.xwin_ok += r0;
+        }
         launch_select(x);
         scan_local_kernel<<<b1 - b0, 256, 0, s>>>(w.cand_count + r0, n, w.local + r0,
                                                   w.bsum + b0);
@@ -3491,6 +3566,10 @@ static adsr_status enqueue(Workspace &w, const SearchPlan &sp, uint32_t n_rows,
         x.cand_center += (size_t)r0 * sp.prm.cand_count;
         x.cand_count += r0;
         x.row0 = r0;
+        if (x.xwin) {
+          x.xwin += (size_t)r0 * x.xwin_stride;
+          x.xwin_ok += r0;
+        }
         if (split) x.lds_per_wave = fit;
         plan_rows(x.lib, x.prm, x.plan, n, s);
         launch_select(x);
@@ -4422,6 +4501,9 @@ struct DiffState {
   double *gsig = nullptr;
   double *gpart = nullptr;  // per-window, per-chunk sigma partial sums
   uint32_t *bq = nullptr;
+  // the forward's apex XIC windows (sp.xwin, sp.xwin_ok), when allocated
+  float *xwin = nullptr;
+  uint32_t *xwin_ok = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float fwd_ms = 0.0f, bwd_ms = 0.0f;
   bool have_forward = false;
@@ -4513,6 +4595,8 @@ adsr_status adsr_diff_create(const adsr_dia *dc, const adsr_lib *lc,
     (void)hipFree(s->gsig);
     (void)hipFree(s->gpart);
     (void)hipFree(s->bq);
+    (void)hipFree(s->xwin);
+    (void)hipFree(s->xwin_ok);
     delete s;
     return e;
   };
@@ -4525,6 +4609,27 @@ adsr_status adsr_diff_create(const adsr_dia *dc, const adsr_lib *lc,
           hipSuccess ||
       dev_malloc(&s->bq, 64) != hipSuccess)
     return fail(set_error(ADSR_ERR_OUT_OF_MEMORY, "diff buffers"));
+  // Apex XIC windows: the backward reads the forward's instead of
+  // re-extracting them (ks x (2 half + 1) floats per row; the forward
+  // writes them when the half width is 10).  Without them (another width,
+  // ADSR_DIFF_REEXTRACT=1 at creation, or the allocation failed) every row
+  // is re-extracted, with the same results.
+  {
+    const char *rx = std::getenv("ADSR_DIFF_REEXTRACT");
+    const bool reextract = rx && rx[0] && rx[0] != '0';
+    const size_t stride = (size_t)s->sp.plan.ks * (2 * half + 1);
+    if (!reextract && half == 10 && stride <= 0xffffffffu && l->n_prec &&
+        dev_malloc(&s->xwin, (size_t)l->n_prec * stride * 4) == hipSuccess &&
+        dev_malloc(&s->xwin_ok, (size_t)l->n_prec * 4) == hipSuccess) {
+      s->sp.xwin = s->xwin;
+      s->sp.xwin_ok = s->xwin_ok;
+      s->sp.xwin_stride = (uint32_t)stride;
+    } else {
+      (void)hipGetLastError();
+      (void)hipFree(s->xwin);
+      s->xwin = nullptr;
+    }
+  }
   for (auto &e : s->ev)
     if (hipEventCreate(&e) != hipSuccess)
       return fail(set_error(ADSR_ERR_DEVICE, "event creation failed"));
@@ -4652,6 +4757,9 @@ adsr_status adsr_diff_backward(adsr_diff *h, const float *grad_s,
     a.grad_w = grad_w;
     a.contrib = s->contrib;
     a.queue = nullptr;
+    a.xwin = s->sp.xwin;
+    a.xwin_ok = s->sp.xwin_ok;
+    a.xwin_stride = s->sp.xwin_stride;
     // four rows (waves) per block
     const size_t lds = 4 * (size_t)bwd_lds_bytes(s->sp.plan.ks);
     diff_backward_kernel<<<(n + 3) / 4, 256, lds, st_>>>(a);
@@ -4713,6 +4821,8 @@ void adsr_diff_free(adsr_diff *h) {
   (void)hipFree(s->gsig);
   (void)hipFree(s->gpart);
   (void)hipFree(s->bq);
+  (void)hipFree(s->xwin);
+  (void)hipFree(s->xwin_ok);
   for (auto &e : s->ev)
     if (e) (void)hipEventDestroy(e);
   delete s;

@@ -306,3 +306,42 @@ def test_forward_without_grad_raises_at_once(env):
     st.forward(sig, w, asynchronous=True)
     with pytest.raises(N.AdsrError, match="isolation windows"):
         st.settle()
+
+
+def test_backward_from_saved_windows_equals_reextraction(env):
+    """The forward leaves the XIC window around each block-sparse row's
+    apex for the backward, which re-extracts only the other rows' windows.
+    A state created with ADSR_DIFF_REEXTRACT=1 re-extracts every window; the
+    two give the same s* and gradients bit for bit (the windows hold the
+    same f32 sums, added in the same entry order)."""
+    import os
+    torch, ads, gen = env["torch"], env["ads"], env["gen"]
+    from alphadia_search_rs.autograd import SelectionGradState
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    P = lib.num_precursors
+    rng = np.random.default_rng(11)
+    g = torch.tensor(rng.choice([-1.0, 0.0, 0.5, 1.0], size=P).astype(np.float32),
+                     device="cuda")
+    sig = torch.tensor(env["sigma"])
+    w = torch.tensor(env["weights"], device="cuda")
+    out = []
+    for reextract in (False, True):
+        if reextract:
+            os.environ["ADSR_DIFF_REEXTRACT"] = "1"
+        try:
+            st = SelectionGradState(d, lib, _params(gen))
+        finally:
+            os.environ.pop("ADSR_DIFF_REEXTRACT", None)
+        s, has = st.forward(sig, w)
+        gw, gs = st.backward(g)
+        out.append((s.cpu().numpy(), has.cpu().numpy(), gw.cpu().numpy(),
+                    np.asarray(gs, np.float64)))
+        del st
+    (s0, h0, gw0, gs0), (s1, h1, gw1, gs1) = out
+    assert h0.sum() > P // 2
+    np.testing.assert_array_equal(s0.view(np.uint32), s1.view(np.uint32))
+    np.testing.assert_array_equal(h0, h1)
+    assert np.count_nonzero(gw0) > 0
+    np.testing.assert_array_equal(gw0.view(np.uint32), gw1.view(np.uint32))
+    np.testing.assert_array_equal(gs0.view(np.uint64), gs1.view(np.uint64))
