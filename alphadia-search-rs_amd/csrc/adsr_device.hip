@@ -240,6 +240,17 @@ __device__ __forceinline__ uint64_t ballot64(bool p) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// The lane index as a value the compiler cannot move.  A per-lane address
+// built from it stays at its use, as a uniform 64-bit base in SGPRs plus a
+// 32-bit lane offset, instead of being hoisted out of a row loop as a 64-bit
+// per-lane pointer that the register allocator then spills to scratch and
+// reloads once per row.
+__device__ __forceinline__ uint32_t opaque_lane() {
+  uint32_t v = threadIdx.x & (kWave - 1);
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // inclusive wave scan of u32
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   const int lane = lane_id();
@@ -1184,8 +1195,9 @@ void select_kernel(SelArgs a) {
   auto flush_out = [&]() {
     if (out_row != 0xffffffffu) {
       if ((uint32_t)lane < min(out_n, (uint32_t)kWave)) {
-        a.cand_score[(size_t)out_row * cc + lane] = out_s;
-        a.cand_center[(size_t)out_row * cc + lane] = out_c;
+        const uint32_t ol = opaque_lane();
+        (a.cand_score + (size_t)out_row * cc)[ol] = out_s;
+        (a.cand_center + (size_t)out_row * cc)[ol] = out_c;
       }
       a.cand_count[out_row] = out_n;
       out_row = 0xffffffffu;
@@ -1203,17 +1215,19 @@ void select_kernel(SelArgs a) {
     const uint32_t r = a.rows ? a.rows[q] : q;
     p_row = r;
     p_rec = a.plan.rec[r];
-    p_sw = lane < (int)ks ? a.plan.sel_w[(size_t)r * ks + lane] : 0.0f;
-    p_smz = lane < (int)ks ? a.plan.sel_mz[(size_t)r * ks + lane] : 0.0f;
-    p_obs = lane < kObsMax ? a.plan.obs[(size_t)r * kObsMax + lane] : 0u;
+    const uint32_t ol = opaque_lane();
+    p_sw = lane < (int)ks ? (a.plan.sel_w + (size_t)r * ks)[ol] : 0.0f;
+    p_smz = lane < (int)ks ? (a.plan.sel_mz + (size_t)r * ks)[ol] : 0.0f;
+    p_obs = lane < kObsMax ? (a.plan.obs + (size_t)r * kObsMax)[ol] : 0u;
   };
   // src/quadrupole_observation/mod.rs:79-83 and src/mz_index/mod.rs:95-127:
   // the fragment's ppm window and the bucket entries of its two bounds
   auto stage2 = [&]() {
     const uint32_t kidx = uni(p_rec.w), pobs = uni(p_rec.z >> 16);
     const uint32_t pnsel = uni(p_rec.z & 0xffffu);
-    p_tap = lane < (int)a.prm.klen ? a.prm.kernels[(size_t)kidx * a.prm.klen + lane]
-                                   : 0.0f;
+    p_tap = lane < (int)a.prm.klen
+                ? (a.prm.kernels + (size_t)kidx * a.prm.klen)[opaque_lane()]
+                : 0.0f;
     p_eb = lane < (int)min(pobs, (uint32_t)kObsMax) ? a.dia.entry_base[p_obs] : 0ull;
     const float delta = p_smz * mtol * 1e-6f;
     p_lo = p_smz - delta;
@@ -1296,7 +1310,7 @@ void select_kernel(SelArgs a) {
         continue;
       }
       WaveLds l = carve(wave_lds, L, a.lds_per_wave);
-      for (uint32_t i = 4 * lane; i < L; i += 4 * kWave)
+      for (uint32_t i = 4 * opaque_lane(); i < L; i += 4 * kWave)
         *reinterpret_cast<float4 *>(l.R + i) = make_float4(0.f, 0.f, 0.f, 0.f);
       float k10[11];
       if (HALF == 10) {
@@ -1879,7 +1893,7 @@ void select_kernel(SelArgs a) {
       if (a.dbg_scores && row >= a.dbg_row0 && row < a.dbg_row0 + a.dbg_rows &&
           L <= a.dbg_stride) {
         float *out = a.dbg_scores + (size_t)(row - a.dbg_row0) * a.dbg_stride;
-        for (uint32_t i = lane; i < L; i += kWave) out[i] = l.R[i];
+        for (uint32_t i = opaque_lane(); i < L; i += kWave) out[i] = l.R[i];
       }
       // ---- local maxima and top-k: mod.rs:23-64, 167-192 -----------------
       // lane l tests positions 256m + 4l + k (k < 4) from three float4
