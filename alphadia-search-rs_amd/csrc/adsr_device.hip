@@ -1856,35 +1856,51 @@ void select_kernel(SelArgs a) {
         };
         // The group's block words (G x wb, fragment-major; a row of L cycles
         // has ceil(L / 256) = wb words) are read 64 at a time, one per lane,
-        // and each word is then taken from its lane: no LDS round trip per
-        // word.
+        // and then taken from their lanes two at a time: lane l writes the
+        // items of bit l of both words, with no branch.  A lane whose bit is
+        // clear writes to the first slot after the pair's items, which a
+        // later pair overwrites or no pass reads (nitems stays below
+        // kItemCap - 2 x 64 between pairs, so that slot is inside the list).
         const uint32_t nwords = G * l.wb;
         const uint64_t *const bdw = spm ? st.BD : l.BD;
         uint32_t wlo = 0, whi = 0;
-        for (uint32_t f = 0; f < G; ++f) {
-          for (uint32_t q = 0; q < l.wb; ++q) {
-            const uint32_t wq = f * l.wb + q;
-            if ((wq & (uint32_t)(kWave - 1)) == 0u) {
-              const uint32_t wi = wq + (uint32_t)lane;
-              const uint64_t bw = wi < nwords ? bdw[wi] : 0ull;
-              wlo = (uint32_t)bw;
-              whi = (uint32_t)(bw >> 32);
-            }
-            const uint32_t blo = rdlane(wlo, wq & (uint32_t)(kWave - 1));
-            const uint32_t bhi = rdlane(whi, wq & (uint32_t)(kWave - 1));
-            if ((blo | bhi) == 0u) continue;
-            const uint32_t before =
-                __builtin_amdgcn_mbcnt_hi(bhi, __builtin_amdgcn_mbcnt_lo(blo, 0u));
-            const uint64_t bwu = ((uint64_t)bhi << 32) | blo;
-            if ((bwu >> lane) & 1ull)
-              l.items[nitems + before] = (uint16_t)((f << 10) | (q * 64 + lane));
-            nitems += (uint32_t)__builtin_popcountll(bwu);
-            if (nitems > (uint32_t)(kItemCap - kWave)) {
+        uint32_t fa = 0, qa = 0;  // fragment and word of word wq
+        for (uint32_t wq = 0; wq < nwords; wq += 2) {
+          const uint32_t j = wq & (uint32_t)(kWave - 1);
+          if (j == 0u) {
+            const uint32_t wi = wq + (uint32_t)lane;
+            const uint64_t bw = wi < nwords ? bdw[wi] : 0ull;
+            wlo = (uint32_t)bw;
+            whi = (uint32_t)(bw >> 32);
+          }
+          // word wq + 1 (j + 1 <= 63: j is even); zero past the last word
+          const bool two = wq + 1u < nwords;
+          const uint32_t fb = qa + 1u == l.wb ? fa + 1u : fa;
+          const uint32_t qb = qa + 1u == l.wb ? 0u : qa + 1u;
+          const uint32_t alo = rdlane(wlo, j), ahi = rdlane(whi, j);
+          const uint32_t blo = two ? rdlane(wlo, j + 1u) : 0u;
+          const uint32_t bhi = two ? rdlane(whi, j + 1u) : 0u;
+          if ((alo | ahi | blo | bhi) != 0u) {
+            const uint64_t A = ((uint64_t)ahi << 32) | alo;
+            const uint64_t B = ((uint64_t)bhi << 32) | blo;
+            const uint32_t na = (uint32_t)__builtin_popcountll(A);
+            const uint32_t nb = (uint32_t)__builtin_popcountll(B);
+            const uint32_t spare = nitems + na + nb;
+            const uint32_t pa = nitems + __builtin_amdgcn_mbcnt_hi(ahi, __builtin_amdgcn_mbcnt_lo(alo, 0u));
+            const uint32_t pb =
+                nitems + na + __builtin_amdgcn_mbcnt_hi(bhi, __builtin_amdgcn_mbcnt_lo(blo, 0u));
+            l.items[((A >> lane) & 1ull) ? pa : spare] = (uint16_t)((fa << 10) | (qa * 64 + lane));
+            l.items[((B >> lane) & 1ull) ? pb : spare] = (uint16_t)((fb << 10) | (qb * 64 + lane));
+            nitems = spare;
+            if (nitems > (uint32_t)(kItemCap - 2 * kWave - 1)) {
               __builtin_amdgcn_wave_barrier();
               flush(nitems);
               nitems = 0;
             }
           }
+          // advance two words
+          fa = qb + 1u == l.wb ? fb + 1u : fb;
+          qa = qb + 1u == l.wb ? 0u : qb + 1u;
         }
         __builtin_amdgcn_wave_barrier();
         if (nitems) flush(nitems);
