@@ -9,6 +9,13 @@
 // rounding to f32 at the end.  The table holds 1/c and log(c) for each
 // subinterval.  Evaluated with fused multiply-adds, the result matches
 // glibc logf on every positive normal float (checked exhaustively).
+//
+// The 16-entry table and the three polynomial coefficients below are those
+// of that routine, which bit-identical results require.  Their origin:
+//   Copyright (c) 2017-2018, Arm Limited.
+//   SPDX-License-Identifier: MIT OR Apache-2.0 WITH LLVM-exception
+// (ARM optimized-routines, math/logf.c and math/logf_data.c; the same
+// constants are in glibc's sysdeps/ieee754/flt-32/e_logf_data.c.)
 #pragma once
 
 #include <hip/hip_runtime.h>
