@@ -47,6 +47,7 @@ __constant__ LogfEntry kDevLogfTab[16] = ADSR_LOGF_TABLE;
 
 constexpr int kWave = 64;
 constexpr int kMaxSel = 64;      // selected fragments per precursor
+constexpr int kScoreMaxSel = 128;  // the same for scoring and quantification
 constexpr int kObsMax = 4;       // valid windows per precursor
 constexpr int kHalfMax = 15;     // kernel half width (kernel_size <= 30)
 constexpr int kLCapMax = 4096;   // cycles per precursor window
@@ -413,7 +414,7 @@ __device__ uint32_t select_fragments(const A &acc, uint32_t n, uint32_t top_k,
   for (uint32_t j = 0; j < n; ++j) kept += acc.I(j) > 0.0f ? 1u : 0u;
   const uint32_t k = min(top_k, kept);
   if (k == 0) return 0;
-  if (k > (uint32_t)kMaxSel || k > ks) {
+  if (k > ks) {  // ks <= kMaxSel for a search, kScoreMaxSel for scoring
     status |= kStTooManyFragments;
     return 0;
   }
@@ -711,7 +712,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_group_kernel(KDia dia, KLib
   const uint32_t pos_kept = (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1));
   const uint32_t k = min(prm.top_k, nkept);
   uint32_t nsel = 0;
-  if (k > 0 && (k > (uint32_t)kMaxSel || k > plan.ks)) {
+  if (k > 0 && k > plan.ks) {  // ks <= kMaxSel for a search
     st |= kStTooManyFragments;
   } else if (k > 0) {
     nsel = k;
@@ -3280,7 +3281,7 @@ static uint32_t window_bound(const adsr_dia *d, float eff) {
 static adsr_status prepare(const adsr_dia *d, const adsr_lib *l,
                            const adsr_selection_params *p, SearchPlan &sp,
                            size_t &ks_out, float *taps = nullptr,
-                           size_t taps_cap = 0) {
+                           size_t taps_cap = 0, size_t kmax = kMaxSel) {
   if (p->kernel_size == 0)
     return set_error(ADSR_ERR_INVALID_ARGUMENT,
                      "kernel_size must be >= 1 (the reference indexes "
@@ -3311,7 +3312,7 @@ static adsr_status prepare(const adsr_dia *d, const adsr_lib *l,
                                  klen);
   // selected-fragment stride
   uint64_t ks = std::min<uint64_t>(p->top_k_fragments, l->max_raw_fragments);
-  ks = std::min<uint64_t>(ks, kMaxSel);
+  ks = std::min<uint64_t>(ks, kmax);
   if (ks == 0) ks = 1;
   ks_out = ks;
   // window bound
@@ -3643,10 +3644,11 @@ static void fill_plan(const adsr_dia *d, const adsr_lib *l,
 
 static adsr_status build_search(const adsr_dia *d, const adsr_lib *l,
                                 const adsr_selection_params *p, Workspace &w,
-                                SearchPlan &sp, bool pooled = false) {
+                                SearchPlan &sp, bool pooled = false,
+                                size_t kmax = kMaxSel) {
   set_lds_limits();
   size_t ks = 1;
-  adsr_status st = prepare(d, l, p, sp, ks);
+  adsr_status st = prepare(d, l, p, sp, ks, nullptr, 0, kmax);
   if (st) return st;
   const size_t cc = (size_t)std::min<uint64_t>(p->candidate_count, 1u << 20);
   st = ws_alloc(w, l->n_prec, ks, cc, sp.kernels.size(),
@@ -5871,7 +5873,11 @@ __host__ __device__ inline uint32_t score_lds_bytes(uint32_t ks, uint32_t L) {
 // QUANT: PeakGroupQuantification.quantify (src/peak_group_quantification/
 // mod.rs:86-168) -- the same extraction, filtered median profile,
 // correlations, summed intensities and m/z errors, written per fragment.
-template <bool QUANT>
+//
+// FPL: fragments per lane.  Lane l owns fragments l + 64 u (u < FPL), so a
+// candidate holds up to 64 FPL selected fragments; every sum over fragments
+// still runs in fragment order over the LDS columns.
+template <bool QUANT, int FPL>
 __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   extern __shared__ __align__(16) char smem[];
   __shared__ LogfEntry ltab[16];
@@ -5946,30 +5952,40 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     s.inb = reinterpret_cast<uint32_t *>(p); p += ks;
     s.vo = reinterpret_cast<uint32_t *>(p);
   }
-  const bool mine = (uint32_t)lane < F;
-  const uint32_t f = (uint32_t)lane;
-  float fmz = 0.0f, fint = 0.0f, fmzl = 0.0f;
-  uint32_t ftype = 0, fnum = 0, df = 0;
+  // fragment u of this lane (u < FPL) and whether the candidate has it
+  auto fr = [&](int u) { return (uint32_t)lane + (uint32_t)(kWave * u); };
+  auto has = [&](int u) { return fr(u) < F; };
+  float fmz[FPL], fint[FPL];
+  uint32_t ftype[FPL], fnum[FPL], df[FPL];
   // mod.rs:249-268 and quantification: RT at cycle_center, loaded early
   const uint64_t ccen = a.c_center[ci];
   const float rt_cen = ccen < a.dia.n_rt ? a.dia.rt[ccen] : 0.0f;
   const float prt = a.lib.p_rt[row];
-  if (mine) {
-    const size_t so = (size_t)row * ks + f;
-    const uint64_t j = a.lib.f_start[row] + a.plan.sel[so];
-    fmz = a.plan.sel_mz[so];
-    fint = a.plan.sel_w[so];
-    ftype = a.lib.f_type[j];
-    fnum = a.lib.f_num[j];
-    fmzl = a.f_mzl[j];
-    // src/idf/mod.rs:35-49, loaded here so that its latency overlaps the
-    // extraction
-    df = a.idf_counts ? a.idf_counts[closest_bin(a.dia, fmzl)] : 0u;
-    s.fint[f] = fint;
-    s.tn[f] = ftype | (fnum << 8);
-    for (uint32_t c = 0; c < L; ++c) {
-      s.xic[f * L + c] = 0.0f;
-      s.mz[f * L + c] = 0.0f;
+#pragma unroll
+  for (int u = 0; u < FPL; ++u) {
+    fmz[u] = 0.0f;
+    fint[u] = 0.0f;
+    ftype[u] = 0;
+    fnum[u] = 0;
+    df[u] = 0;
+    if (has(u)) {
+      const uint32_t f = fr(u);
+      const size_t so = (size_t)row * ks + f;
+      const uint64_t j = a.lib.f_start[row] + a.plan.sel[so];
+      fmz[u] = a.plan.sel_mz[so];
+      fint[u] = a.plan.sel_w[so];
+      ftype[u] = a.lib.f_type[j];
+      fnum[u] = a.lib.f_num[j];
+      const float fmzl = a.f_mzl[j];
+      // src/idf/mod.rs:35-49, loaded here so that its latency overlaps the
+      // extraction
+      df[u] = a.idf_counts ? a.idf_counts[closest_bin(a.dia, fmzl)] : 0u;
+      s.fint[f] = fint[u];
+      s.tn[f] = ftype[u] | (fnum[u] << 8);
+      for (uint32_t c = 0; c < L; ++c) {
+        s.xic[f * L + c] = 0.0f;
+        s.mz[f * L + c] = 0.0f;
+      }
     }
   }
   // dense_xic_observation/mod.rs:98-137 with quadrupole_observation/mod.rs:
@@ -5996,20 +6012,27 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     }
     return;
   }
-  uint32_t nb_f = 0;
-  if (mine) {
-    const float delta = fmz * a.mass_tol * 1e-6f;
-    uint32_t bs, be;
-    mz_range(a.dia, fmz - delta, fmz + delta, bs, be);
-    nb_f = be - bs;
-    s.ibs[f] = bs;
-    s.inb[f] = nb_f;
+  // bin items of each fragment, numbered in fragment order
+  uint32_t items_f[FPL], ipre_f[FPL];
+  uint32_t NI = 0;
+#pragma unroll
+  for (int u = 0; u < FPL; ++u) {
+    uint32_t nb_f = 0;
+    if (has(u)) {
+      const uint32_t f = fr(u);
+      const float delta = fmz[u] * a.mass_tol * 1e-6f;
+      uint32_t bs, be;
+      mz_range(a.dia, fmz[u] - delta, fmz[u] + delta, bs, be);
+      nb_f = be - bs;
+      s.ibs[f] = bs;
+      s.inb[f] = nb_f;
+    }
+    items_f[u] = nb_f * nvo;
+    const uint32_t incl = wave_incl_scan(items_f[u]);
+    ipre_f[u] = NI + incl - items_f[u];
+    NI += rdlane(incl, kWave - 1);
+    if (has(u)) s.ipre[fr(u)] = ipre_f[u];
   }
-  const uint32_t items_f = nb_f * nvo;
-  const uint32_t incl = wave_incl_scan(items_f);
-  const uint32_t ipre_f = incl - items_f;
-  const uint32_t NI = rdlane(incl, kWave - 1);
-  if (mine) s.ipre[f] = ipre_f;
   __builtin_amdgcn_wave_barrier();
   const uint16_t key16 = (uint16_t)c0;
   for (uint32_t p0 = 0; p0 < NI; p0 += kWave) {
@@ -6067,8 +6090,11 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
       }
     }
     __builtin_amdgcn_wave_barrier();
-    if (mine && items_f) {
-      const uint32_t k0 = max(ipre_f, p0), k1 = min(ipre_f + items_f, p0 + kWave);
+#pragma unroll
+    for (int u = 0; u < FPL; ++u) {
+    if (has(u) && items_f[u]) {
+      const uint32_t f = fr(u);
+      const uint32_t k0 = max(ipre_f[u], p0), k1 = min(ipre_f[u] + items_f[u], p0 + kWave);
       float *X = s.xic + f * L, *M = s.mz + f * L;
       for (uint32_t kk = k0; kk < k1; ++kk) {
         const uint4 rc = s.rec[kk - p0];
@@ -6097,11 +6123,15 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
         }
       }
     }
+    }
     __builtin_amdgcn_wave_barrier();
   }
   // utils.rs:71-108 normalize_profiles(.., 1) and :7-29 filter_non_zero
   const uint32_t cidx = L / 2;
-  if (mine) {
+#pragma unroll
+  for (int u = 0; u < FPL; ++u) {
+    if (!has(u)) continue;
+    const uint32_t f = fr(u);
     const uint32_t s0 = cidx >= 1 ? cidx - 1 : 0, s1 = min(cidx + 2, L);
     float sum = 0.0f;
     for (uint32_t j = s0; j < s1; ++j) sum = sum + s.xic[f * L + j];
@@ -6115,9 +6145,17 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     s.nz[f] = any;
   }
   __builtin_amdgcn_wave_barrier();
-  // bit g: fragment g has a non-zero profile (lane g holds fragment g)
-  const uint64_t nzm = ballot64(mine && s.nz[f] != 0u);
-  const uint32_t Ff = (uint32_t)__builtin_popcountll(nzm);
+  // bit g: fragment g has a non-zero profile (lane g holds fragment g);
+  // with more than 64 fragments the flags are read from s.nz
+  const uint64_t nzm = ballot64(has(0) && s.nz[fr(0)] != 0u);
+  uint32_t Ff = (uint32_t)__builtin_popcountll(nzm);
+#pragma unroll
+  for (int u = 1; u < FPL; ++u)
+    Ff += (uint32_t)__builtin_popcountll(ballot64(has(u) && s.nz[fr(u)] != 0u));
+  auto nz_of = [&](uint32_t g) {
+    if constexpr (FPL == 1) return ((nzm >> g) & 1ull) != 0ull;
+    else return s.nz[g] != 0u;
+  };
   // utils.rs:35-67 median_axis_0 of all rows and of the non-zero rows:
   // every (column, fragment) pair, one per lane, places its value at its
   // stable rank in the column
@@ -6131,10 +6169,10 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
         const float u = s.norm[g * L + c];
         const bool before = u < v || (u == v && g < fq);
         ra += before ? 1u : 0u;
-        rf += (before && ((nzm >> g) & 1ull)) ? 1u : 0u;
+        rf += (before && nz_of(g)) ? 1u : 0u;
       }
       s.sortA[c * ks + ra] = v;
-      if ((nzm >> fq) & 1ull) s.sortF[c * ks + rf] = v;
+      if (nz_of(fq)) s.sortF[c * ks + rf] = v;
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -6171,13 +6209,18 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     }
   }
   // mod.rs:152-160: correlations with the filtered median, summed rows
-  float corr = 0.0f, oint = 0.0f;
-  if (mine) {
-    corr = corr_safe(s.medf, s.norm + f * L, L);
-    oint = ndarray_sum(s.xic + f * L, L);
-    s.corr[f] = corr;
-    s.oint[f] = oint;
-    s.matched[f] = oint > 0.0f ? 1u : 0u;
+  float corr[FPL], oint[FPL];
+#pragma unroll
+  for (int u = 0; u < FPL; ++u) {
+    corr[u] = 0.0f;
+    oint[u] = 0.0f;
+    if (!has(u)) continue;
+    const uint32_t f = fr(u);
+    corr[u] = corr_safe(s.medf, s.norm + f * L, L);
+    oint[u] = ndarray_sum(s.xic + f * L, L);
+    s.corr[f] = corr[u];
+    s.oint[f] = oint[u];
+    s.matched[f] = oint[u] > 0.0f ? 1u : 0u;
   }
   __builtin_amdgcn_wave_barrier();
   const float int_corr = corr_safe(s.oint, s.fint, F);
@@ -6186,13 +6229,16 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
   for (uint32_t g = 0; g < F; ++g) csum = csum + s.corr[g];
   const float mean_corr = F ? csum / (float)F : 0.0f;
   // utils/mod.rs:120-137 median and :146-155 sample std of the correlations
-  if (mine) {
+#pragma unroll
+  for (int u = 0; u < FPL; ++u) {
+    if (!has(u)) continue;
+    const uint32_t f = fr(u);
     uint32_t r = 0;
     for (uint32_t g = 0; g < F; ++g) {
-      const float u = s.corr[g];
-      r += (u < corr || (u == corr && g < f)) ? 1u : 0u;
+      const float cg = s.corr[g];
+      r += (cg < corr[u] || (cg == corr[u] && g < f)) ? 1u : 0u;
     }
-    s.buf[r] = corr;
+    s.buf[r] = corr[u];
   }
   __builtin_amdgcn_wave_barrier();
   const float med_corr = F == 0 ? 0.0f
@@ -6208,53 +6254,86 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     }
     std_corr = sqrtf(v / (float)(F - 1));
   }
-  auto count = [&](float thr, bool sel) {
-    return (float)__builtin_popcountll(ballot64(mine && sel && corr > thr));
+  // fragments over a correlation threshold among those sel(u) picks
+  auto count = [&](float thr, auto sel) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int u = 0; u < FPL; ++u)
+      c += (uint32_t)__builtin_popcountll(ballot64(has(u) && sel(u) && corr[u] > thr));
+    return (float)c;
   };
   // utils/mod.rs:175-207 ranks by library intensity (descending, then
   // fragment order)
-  uint32_t rint = 0;
-  if (mine)
+  uint32_t rint[FPL];
+#pragma unroll
+  for (int u = 0; u < FPL; ++u) {
+    rint[u] = 0;
+    if (!has(u)) continue;
+    const uint32_t f = fr(u);
     for (uint32_t g = 0; g < F; ++g) {
-      const float u = s.fint[g];
-      rint += (u > fint || (u == fint && g < f)) ? 1u : 0u;
+      const float ig = s.fint[g];
+      rint[u] += (ig > fint[u] || (ig == fint[u] && g < f)) ? 1u : 0u;
     }
+  }
   // mod.rs:203-247
-  const bool matched = mine && oint > 0.0f;
+  bool matched[FPL];
+#pragma unroll
+  for (int u = 0; u < FPL; ++u) matched[u] = has(u) && oint[u] > 0.0f;
   // utils.rs:314-371 longest b / y series: matched numbers of a type at
   // their sorted rank, then one scan
   float longest[2];
   for (int k = 0; k < 2; ++k) {
     const uint32_t want = k == 0 ? (uint32_t)kFragB : (uint32_t)kFragY;
-    const bool in = matched && ftype == want;
-    const uint32_t cnt = (uint32_t)__builtin_popcountll(ballot64(in));
-    if (in) {
-      uint32_t r = 0;
-      for (uint32_t g = 0; g < F; ++g) {
-        const uint32_t tg = s.tn[g];
-        if (!s.matched[g] || (tg & 0xffu) != want) continue;
-        const uint32_t ng = tg >> 8;
-        r += (ng < fnum || (ng == fnum && g < f)) ? 1u : 0u;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int u = 0; u < FPL; ++u) {
+      const bool in = matched[u] && ftype[u] == want;
+      cnt += (uint32_t)__builtin_popcountll(ballot64(in));
+      if (in) {
+        const uint32_t f = fr(u);
+        uint32_t r = 0;
+        for (uint32_t g = 0; g < F; ++g) {
+          const uint32_t tg = s.tn[g];
+          if (!s.matched[g] || (tg & 0xffu) != want) continue;
+          const uint32_t ng = tg >> 8;
+          r += (ng < fnum[u] || (ng == fnum[u] && g < f)) ? 1u : 0u;
+        }
+        s.buf[r] = (float)fnum[u];
       }
-      s.buf[r] = (float)fnum;
     }
     __builtin_amdgcn_wave_barrier();
-    // bit i: sorted number i continues number i - 1; the longest run of
-    // such bits plus one is the longest series
-    const uint32_t il = (uint32_t)lane;
-    uint64_t run = ballot64(il >= 1u && il < cnt &&
-                            (uint32_t)s.buf[il] == (uint32_t)s.buf[il - 1] + 1u);
     uint32_t best = cnt ? 1u : 0u;
-    while (run) {
-      run &= run >> 1;
-      ++best;
+    if constexpr (FPL == 1) {
+      // bit i: sorted number i continues number i - 1; the longest run of
+      // such bits plus one is the longest series
+      const uint32_t il = (uint32_t)lane;
+      uint64_t run = ballot64(il >= 1u && il < cnt &&
+                              (uint32_t)s.buf[il] == (uint32_t)s.buf[il - 1] + 1u);
+      while (run) {
+        run &= run >> 1;
+        ++best;
+      }
+    } else {
+      // more numbers than lanes: one scan over the sorted numbers
+      uint32_t cur = 1;
+      for (uint32_t i = 1; i < cnt; ++i) {
+        if ((uint32_t)s.buf[i] == (uint32_t)s.buf[i - 1] + 1u) {
+          ++cur;
+          best = max(best, cur);
+        } else {
+          cur = 1;
+        }
+      }
     }
     longest[k] = (float)(best & 0xffu);
     __builtin_amdgcn_wave_barrier();
   }
   // utils/mod.rs:11-69 observed m/z and ppm error; :83-109 weighted mean
   // absolute error; utils.rs:374-399 inverse-mass-error weights
-  if (mine) {
+#pragma unroll
+  for (int u = 0; u < FPL; ++u) {
+    if (!has(u)) continue;
+    const uint32_t f = fr(u);
     float ws = 0.0f, wt = 0.0f;
     for (uint32_t j = 0; j < L; ++j) {
       const float w = s.xic[f * L + j];
@@ -6264,11 +6343,11 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
       }
     }
     const float m = wt > 0.0f ? ws / wt : 0.0f;
-    const float e = m > 0.0f ? ((m - fmz) / fmz) * 1e6f : 0.0f;
+    const float e = m > 0.0f ? ((m - fmz[u]) / fmz[u]) * 1e6f : 0.0f;
     s.err[f] = e;
     if constexpr (QUANT) {
       const size_t q = (size_t)ci * ks + f;
-      a.q_int[q] = oint;
+      a.q_int[q] = oint[u];
       a.q_mzo[q] = m;
       // quantification/mod.rs:121-128 correlates (profile, median)
       a.q_corr[q] = corr_safe(s.norm + f * L, s.medf, L);
@@ -6278,7 +6357,7 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     s.wie[f] = 1.0f / (fabsf(e) + 0.1f);
     // src/idf/mod.rs:35-49
     s.idf[f] = a.idf_total == 0.0f ? 1.0f
-               : logf_exact(a.idf_total / (float)(df > 1u ? df : 1u), ltab);
+               : logf_exact(a.idf_total / (float)(df[u] > 1u ? df[u] : 1u), ltab);
   }
   __builtin_amdgcn_wave_barrier();
   if constexpr (QUANT) {
@@ -6324,21 +6403,30 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs a) {
     dot_int = dot_int + s.idf[g] * s.oint[g];
   }
   if (F == 0) dot_xic = dot_int = 0.0f;
-  uint32_t ridf = 0;
-  if (mine) {
+  uint32_t ridf[FPL];
+#pragma unroll
+  for (int u = 0; u < FPL; ++u) {
+    ridf[u] = 0;
+    if (!has(u)) continue;
+    const uint32_t f = fr(u);
     const float v = s.idf[f];
     for (uint32_t g = 0; g < F; ++g) {
-      const float u = s.idf[g];
-      ridf += (u > v || (u == v && g < f)) ? 1u : 0u;
+      const float ig = s.idf[g];
+      ridf[u] += (ig > v || (ig == v && g < f)) ? 1u : 0u;
     }
   }
-  const float cnt_o0_i6 = count(0.0f, ridf < 6), cnt_o50_i6 = count(0.5f, ridf < 6);
-  const float c95 = count(0.95f, true), c90 = count(0.90f, true), c80 = count(0.80f, true);
-  const float c50 = count(0.5f, true), c0v = count(0.0f, true);
-  const float r0 = count(0.0f, rint < 6), r1 = count(0.0f, rint >= 6 && rint < 12);
-  const float r2 = count(0.0f, rint >= 12 && rint < 18), r3 = count(0.0f, rint >= 18 && rint < 24);
-  const float q0 = count(0.5f, rint < 6), q1 = count(0.5f, rint >= 6 && rint < 12);
-  const float q2 = count(0.5f, rint >= 12 && rint < 18), q3 = count(0.5f, rint >= 18 && rint < 24);
+  auto all = [&](int) { return true; };
+  auto idf6 = [&](int u) { return ridf[u] < 6; };
+  auto rk = [&](uint32_t lo, uint32_t hi) {
+    return [&, lo, hi](int u) { return rint[u] >= lo && rint[u] < hi; };
+  };
+  const float cnt_o0_i6 = count(0.0f, idf6), cnt_o50_i6 = count(0.5f, idf6);
+  const float c95 = count(0.95f, all), c90 = count(0.90f, all), c80 = count(0.80f, all);
+  const float c50 = count(0.5f, all), c0v = count(0.0f, all);
+  const float r0 = count(0.0f, rk(0, 6)), r1 = count(0.0f, rk(6, 12));
+  const float r2 = count(0.0f, rk(12, 18)), r3 = count(0.0f, rk(18, 24));
+  const float q0 = count(0.5f, rk(0, 6)), q1 = count(0.5f, rk(6, 12));
+  const float q2 = count(0.5f, rk(12, 18)), q3 = count(0.5f, rk(18, 24));
   if (lane == 0) a.found[ci] = 1;
   F_OUT(ADSR_F_SCORE, a.c_score[ci]);
   F_OUT(ADSR_F_MEAN_CORRELATION, mean_corr);
@@ -6496,7 +6584,7 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   sp_p.candidate_count = 1;
   Workspace w;
   SearchPlan sp;
-  st = build_search(d, l, &sp_p, w, sp);
+  st = build_search(d, l, &sp_p, w, sp, false, kScoreMaxSel);
   if (st) {
     ws_free(w);
     return st;
@@ -6517,7 +6605,7 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
       return fail(set_error(ADSR_ERR_DEVICE, "fragment plan failed"));
     if (host[1] & kStTooManyFragments)
       return fail(set_error(ADSR_ERR_UNSUPPORTED,
-                            "a precursor selects more than %d fragments", kMaxSel));
+                            "a precursor selects more than %d fragments", kScoreMaxSel));
     if (host[1] & kStSelectFallback)
       return fail(set_error(ADSR_ERR_UNSUPPORTED,
                             "a precursor's top-k fragment selection needs the "
@@ -6588,20 +6676,38 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   a.q_row = q_row;
   a.q_rt = q_rt;
   a.lds_per_wave = (score_lds_bytes(sp.plan.ks, (uint32_t)lmax) + 15) / 16 * 16;
+  // fragments per lane: the plan's stride is at most kScoreMaxSel = 2 x 64
+  const int fpl = sp.plan.ks > (uint32_t)kWave ? 2 : 1;
+  static_assert(kScoreMaxSel <= 2 * kWave, "two fragments per lane at most");
   int wpb = 4;
   while (wpb > 1 && (size_t)a.lds_per_wave * wpb > 64 * 1024) wpb >>= 1;
-  if ((size_t)a.lds_per_wave > 64 * 1024)
+  if ((size_t)a.lds_per_wave > kMaxDynLds)
     return fail(set_error(ADSR_ERR_UNSUPPORTED, "scoring LDS of %u bytes", a.lds_per_wave));
+  if ((size_t)a.lds_per_wave > 64 * 1024) {
+    // one wave per block beyond 64 KB, with the dynamic LDS limit raised
+    static std::once_flag once;
+    std::call_once(once, [] {
+      const void *fns[] = {(const void *)score_kernel<false, 1>, (const void *)score_kernel<true, 1>,
+                           (const void *)score_kernel<false, 2>, (const void *)score_kernel<true, 2>};
+      for (const void *fn : fns)
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kMaxDynLds);
+    });
+  }
   (void)hipEventRecord(w.ev[2], w.stream);
   if (n) {
     cand_row_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, w.stream>>>(
         l->dev.p_idx, (uint32_t)l->n_prec, cpidx, n, crow);
     const dim3 grid((uint32_t)((n + wpb - 1) / wpb)), block(wpb * kWave);
     const size_t lds = (size_t)a.lds_per_wave * wpb;
-    if (quant)
-      score_kernel<true><<<grid, block, lds, w.stream>>>(a);
+    if (quant && fpl == 2)
+      score_kernel<true, 2><<<grid, block, lds, w.stream>>>(a);
+    else if (quant)
+      score_kernel<true, 1><<<grid, block, lds, w.stream>>>(a);
+    else if (fpl == 2)
+      score_kernel<false, 2><<<grid, block, lds, w.stream>>>(a);
     else
-      score_kernel<false><<<grid, block, lds, w.stream>>>(a);
+      score_kernel<false, 1><<<grid, block, lds, w.stream>>>(a);
     if (hipGetLastError() != hipSuccess)
       return fail(set_error(ADSR_ERR_DEVICE, "score kernel launch failed"));
   }

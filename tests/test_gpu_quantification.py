@@ -78,3 +78,16 @@ def test_quantify_uneven_fragments_top_k_and_dropped_rows(ads):
         q, want = _quantify(ads, gen, tol=15.0, top_k=top_k, extra_rows=extra,
                             top_k_fragments=24)
         _assert_same(q.to_dict_arrays(), want, f"uneven, top_k {top_k}")
+
+
+def test_quantify_more_than_64_fragments(ads):
+    """Precursors with 40-128 fragments, all of them quantified (top_k
+    10000): the kernel's two-fragments-per-lane instance, identical to the
+    oracle."""
+    gen = synthetic.generate(synthetic.config(0, n_cycles=400, n_windows=6, n_targets=200,
+                                              frag_min=40, frag_max=128))
+    nf = (gen["lib"][7] - gen["lib"][6]).astype(np.int64)
+    assert nf.max() > 64
+    q, want = _quantify(ads, gen, tol=15.0)
+    assert q.num_fragments == len(want[1]["mz"]) > 0
+    _assert_same(q.to_dict_arrays(), want, "40-128 fragments")

@@ -148,3 +148,31 @@ def test_scoring_too_many_windows_is_unsupported(ads):
                                                 [30], [25], [36])
     with pytest.raises(N.AdsrError, match="isolation windows"):
         ads.PeakGroupScoring(ads.ScoringParameters()).score(d, lib, cands)
+
+
+def test_scoring_more_than_64_fragments(ads):
+    """Precursors with 40-128 fragments: with top_k above 64 each lane owns
+    two fragments (the kernel's two-fragments-per-lane instance); every
+    feature still equals the oracle's.  top_k 65 puts the stride just past
+    one fragment per lane; 100 is the reference's default."""
+    gen = synthetic.generate(synthetic.config(0, n_cycles=400, n_windows=6, n_targets=200,
+                                              frag_min=40, frag_max=128))
+    for top_k in (128, 100, 65):
+        _, _, _, got, want = _run(ads, gen, tol=15.0, top_k=top_k)
+        assert int(np.max(want["num_fragments"])) > 64
+        _assert_features_equal(got, want, f"40-128 fragments, top_k {top_k}")
+
+
+def test_scoring_more_than_128_fragments_is_unsupported(ads):
+    """A candidate keeping more than 128 fragments exceeds the kernel's two
+    fragments per lane: the device reports the limit."""
+    import alphadia_search_rs._native as N
+    gen = synthetic.generate(synthetic.config(0, n_cycles=200, n_windows=4, n_targets=40,
+                                              frag_min=130, frag_max=140))
+    d = ads.DIAData.from_arrays(*gen["dia"])
+    lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
+    cands = ads.PeakGroupSelection(params_for(gen["spec"])).search(d, lib)
+    sp = ads.ScoringParameters()
+    sp.update({"top_k_fragments": 1000})
+    with pytest.raises(N.AdsrError, match="more than 128 fragments"):
+        ads.PeakGroupScoring(sp).score(d, lib, cands)
