@@ -47,7 +47,7 @@ __constant__ LogfEntry kDevLogfTab[16] = ADSR_LOGF_TABLE;
 
 constexpr int kWave = 64;
 constexpr int kMaxSel = 64;      // selected fragments per precursor
-constexpr int kScoreMaxSel = 128;  // the same for scoring and quantification
+constexpr int kScoreMaxSel = 256;  // the same for scoring and quantification
 constexpr int kObsMax = 4;       // valid windows per precursor
 constexpr int kHalfMax = 15;     // kernel half width (kernel_size <= 30)
 constexpr int kLCapMax = 4096;   // cycles per precursor window
@@ -6676,9 +6676,9 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
   a.q_row = q_row;
   a.q_rt = q_rt;
   a.lds_per_wave = (score_lds_bytes(sp.plan.ks, (uint32_t)lmax) + 15) / 16 * 16;
-  // fragments per lane: the plan's stride is at most kScoreMaxSel = 2 x 64
-  const int fpl = sp.plan.ks > (uint32_t)kWave ? 2 : 1;
-  static_assert(kScoreMaxSel <= 2 * kWave, "two fragments per lane at most");
+  // fragments per lane: the plan's stride is at most kScoreMaxSel = 4 x 64
+  const int fpl = sp.plan.ks > 2u * kWave ? 4 : sp.plan.ks > (uint32_t)kWave ? 2 : 1;
+  static_assert(kScoreMaxSel <= 4 * kWave, "four fragments per lane at most");
   int wpb = 4;
   while (wpb > 1 && (size_t)a.lds_per_wave * wpb > 64 * 1024) wpb >>= 1;
   if ((size_t)a.lds_per_wave > kMaxDynLds)
@@ -6688,7 +6688,8 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
     static std::once_flag once;
     std::call_once(once, [] {
       const void *fns[] = {(const void *)score_kernel<false, 1>, (const void *)score_kernel<true, 1>,
-                           (const void *)score_kernel<false, 2>, (const void *)score_kernel<true, 2>};
+                           (const void *)score_kernel<false, 2>, (const void *)score_kernel<true, 2>,
+                           (const void *)score_kernel<false, 4>, (const void *)score_kernel<true, 4>};
       for (const void *fn : fns)
         (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)kMaxDynLds);
@@ -6700,10 +6701,14 @@ static adsr_status score_common(const adsr_dia *dc_, const adsr_lib *lc,
         l->dev.p_idx, (uint32_t)l->n_prec, cpidx, n, crow);
     const dim3 grid((uint32_t)((n + wpb - 1) / wpb)), block(wpb * kWave);
     const size_t lds = (size_t)a.lds_per_wave * wpb;
-    if (quant && fpl == 2)
+    if (quant && fpl == 4)
+      score_kernel<true, 4><<<grid, block, lds, w.stream>>>(a);
+    else if (quant && fpl == 2)
       score_kernel<true, 2><<<grid, block, lds, w.stream>>>(a);
     else if (quant)
       score_kernel<true, 1><<<grid, block, lds, w.stream>>>(a);
+    else if (fpl == 4)
+      score_kernel<false, 4><<<grid, block, lds, w.stream>>>(a);
     else if (fpl == 2)
       score_kernel<false, 2><<<grid, block, lds, w.stream>>>(a);
     else

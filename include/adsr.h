@@ -374,7 +374,7 @@ enum {
 const char *adsr_feature_name(size_t i);
 /* PeakGroupScoring::score (mod.rs:43-102): every candidate in table order;
  * candidates whose precursor_idx is not in the library are skipped, as the
- * reference's filter_map does.  Limits: at most 128 selected fragments per
+ * reference's filter_map does.  Limits: at most 256 selected fragments per
  * precursor and cycle_stop - cycle_start <= 32 (peak_length <= 15); a
  * candidate whose cycle indices fall outside the RT axis, where the
  * reference panics, returns INVALID_ARGUMENT.                             */

@@ -91,3 +91,15 @@ def test_quantify_more_than_64_fragments(ads):
     q, want = _quantify(ads, gen, tol=15.0)
     assert q.num_fragments == len(want[1]["mz"]) > 0
     _assert_same(q.to_dict_arrays(), want, "40-128 fragments")
+
+
+def test_quantify_more_than_128_fragments(ads):
+    """Precursors with 130-250 fragments, all quantified: four fragments per
+    lane, identical to the oracle."""
+    gen = synthetic.generate(synthetic.config(0, n_cycles=300, n_windows=4, n_targets=100,
+                                              frag_min=130, frag_max=250))
+    nf = (gen["lib"][7] - gen["lib"][6]).astype(np.int64)
+    assert nf.max() > 128
+    q, want = _quantify(ads, gen, tol=15.0)
+    assert q.num_fragments == len(want[1]["mz"]) > 0
+    _assert_same(q.to_dict_arrays(), want, "130-250 fragments")

@@ -163,16 +163,27 @@ def test_scoring_more_than_64_fragments(ads):
         _assert_features_equal(got, want, f"40-128 fragments, top_k {top_k}")
 
 
-def test_scoring_more_than_128_fragments_is_unsupported(ads):
-    """A candidate keeping more than 128 fragments exceeds the kernel's two
+def test_scoring_more_than_128_fragments(ads):
+    """Precursors with 130-250 fragments at top_k 256 and 200: four
+    fragments per lane, every feature equal to the oracle's."""
+    gen = synthetic.generate(synthetic.config(0, n_cycles=300, n_windows=4, n_targets=100,
+                                              frag_min=130, frag_max=250))
+    for top_k in (256, 200):
+        _, _, _, got, want = _run(ads, gen, tol=15.0, top_k=top_k)
+        assert int(np.max(want["num_fragments"])) > 128
+        _assert_features_equal(got, want, f"130-250 fragments, top_k {top_k}")
+
+
+def test_scoring_more_than_256_fragments_is_unsupported(ads):
+    """A candidate keeping more than 256 fragments exceeds the kernel's four
     fragments per lane: the device reports the limit."""
     import alphadia_search_rs._native as N
-    gen = synthetic.generate(synthetic.config(0, n_cycles=200, n_windows=4, n_targets=40,
-                                              frag_min=130, frag_max=140))
+    gen = synthetic.generate(synthetic.config(0, n_cycles=200, n_windows=4, n_targets=20,
+                                              frag_min=260, frag_max=270))
     d = ads.DIAData.from_arrays(*gen["dia"])
     lib = ads.SpecLibFlat.from_arrays(*gen["lib"])
     cands = ads.PeakGroupSelection(params_for(gen["spec"])).search(d, lib)
     sp = ads.ScoringParameters()
     sp.update({"top_k_fragments": 1000})
-    with pytest.raises(N.AdsrError, match="more than 128 fragments"):
+    with pytest.raises(N.AdsrError, match="more than 256 fragments"):
         ads.PeakGroupScoring(sp).score(d, lib, cands)
