@@ -2354,6 +2354,7 @@ __global__ void sstar_kernel(const float *cs, const uint32_t *cnt, uint32_t cc,
 }
 
 constexpr int kBwdHitCap = 256;
+constexpr int kBwdPreFrag = 24;  // saved-window fragments loaded with the row's first loads
 
 // Bin range [st, en) of one fragment m/z (src/quadrupole_observation/mod.rs:
 // 79-83, src/mz_index/mod.rs:95-127) from the bucket entries of its bounds.
@@ -2403,6 +2404,18 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
   const uint32_t ncand_raw = a.cand_count[row];
   const uint32_t cen_raw = a.cand_center[(size_t)row * a.cc];
   const uint32_t ok_raw = a.xwin_ok ? a.xwin_ok[row] : 0u;
+  // The apex window the forward may have left, loaded with the row's first
+  // loads rather than after its flag: fragments 2p + (lane >> 5), cells
+  // lane & 31, for the first kBwdPreFrag fragments of the stride.
+  const uint32_t wlen = 2 * a.prm.half + 1;
+  const uint32_t hl = (uint32_t)lane >> 5, tl = (uint32_t)lane & 31u;
+  float pre[kBwdPreFrag / 2];
+#pragma unroll
+  for (int q = 0; q < kBwdPreFrag / 2; ++q) {
+    const uint32_t f = 2u * q + hl;
+    pre[q] = a.xwin && f < ks && tl < wlen ? a.xwin[(size_t)row * a.xwin_stride + f * wlen + tl]
+                                           : 0.0f;
+  }
   const bool fl = lane < (int)ks;
   const float smz_l = fl ? a.plan.sel_mz[(size_t)row * ks + lane] : 0.0f;
   const uint32_t sel_l = fl ? (uint32_t)a.plan.sel[(size_t)row * ks + lane] : 0u;
@@ -2444,11 +2457,15 @@ __global__ __launch_bounds__(256) void diff_backward_kernel(BwdArgs a) {
   };
   if (saved) {
     // fragment-major, 2 * half + 1 cells per fragment
+#pragma unroll
+    for (int q = 0; q < kBwdPreFrag / 2; ++q) {
+      const uint32_t f = 2u * q + hl;
+      if (f < nsel && tl < wlen) xw[f * kWin + tl] = pre[q];
+    }
     const float *xi = a.xwin + (size_t)row * a.xwin_stride;
-    const uint32_t wlen = 2 * half + 1;
-    for (uint32_t f0 = 0; f0 < nsel; f0 += 2) {
-      const uint32_t f = f0 + ((uint32_t)lane >> 5), t = (uint32_t)lane & 31u;
-      if (f < nsel && t < wlen) xw[f * kWin + t] = xi[f * wlen + t];
+    for (uint32_t f0 = kBwdPreFrag; f0 < nsel; f0 += 2) {
+      const uint32_t f = f0 + hl;
+      if (f < nsel && tl < wlen) xw[f * kWin + tl] = xi[f * wlen + tl];
     }
     __builtin_amdgcn_wave_barrier();
   }
