@@ -158,9 +158,14 @@ def test_scoring_more_than_64_fragments(ads):
     gen = synthetic.generate(synthetic.config(0, n_cycles=400, n_windows=6, n_targets=200,
                                               frag_min=40, frag_max=128))
     for top_k in (128, 100, 65):
-        _, _, _, got, want = _run(ads, gen, tol=15.0, top_k=top_k)
+        d, lib, sel, got, want = _run(ads, gen, tol=15.0, top_k=top_k)
         assert int(np.max(want["num_fragments"])) > 64
         _assert_features_equal(got, want, f"40-128 fragments, top_k {top_k}")
+    # the table kept in HBM scores the same
+    sp = ads.ScoringParameters()
+    sp.update({"mass_tolerance": 15.0, "top_k_fragments": 65})
+    dev = ads.PeakGroupScoring(sp).score(d, lib, sel.search_device(d, lib)).to_dict_arrays()
+    _assert_features_equal(dev, want, "40-128 fragments, top_k 65, device table")
 
 
 def test_scoring_more_than_128_fragments(ads):
