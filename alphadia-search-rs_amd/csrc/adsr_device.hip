@@ -241,6 +241,10 @@ __device__ __forceinline__ uint64_t ballot64(bool p) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Two u16 lanes of one VGPR, for packed integer arithmetic.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+static_assert(kLCapMax < 65536, "window lengths fit 16 bits");
+
 // The lane index as a value the compiler cannot move.  A per-lane address
 // built from it stays at its use, as a uniform 64-bit base in SGPRs plus a
 // 32-bit lane offset, instead of being hoisted out of a row loop as a 64-bit
@@ -1528,11 +1532,30 @@ void select_kernel(SelArgs a) {
                                        raw[u].y & 0xffffu, raw[u].y >> 16,
                                        raw[u].z & 0xffffu, raw[u].z >> 16,
                                        raw[u].w & 0xffffu, raw[u].w >> 16};
-              uint32_t hm = 0, tmin = 0xffffffffu;
+              // Entry k is in the window when (cycle - c0) mod 2^16 < L:
+              // cycles, c0 and c0 + L all fit 16 bits, so the wrapped
+              // difference of a cycle below c0 is at least L.  Two entries
+              // per packed u16 operation: L - d saturated at 0, then
+              // min(., 1), leaves 1 in each half whose entry is in the
+              // window; d == 0 marks an entry at c0.
+              uint32_t hm, tmin;
+              {
+                const u16x2 c0v = {(uint16_t)c0, (uint16_t)c0};
+                const u16x2 lv = {(uint16_t)L, (uint16_t)L};
+                const u16x2 one = {1, 1};
+                const uint32_t rw[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
+                u16x2 dmin = {0xffff, 0xffff};
+                uint32_t t = 0;
 #pragma unroll
-              for (int k = 0; k < 8; ++k) {
-                hm |= (cyc[k] - c0 < L ? 1u : 0u) << k;
-                tmin = min(tmin, cyc[k] - c0);
+                for (int q = 0; q < 4; ++q) {
+                  const u16x2 d = __builtin_bit_cast(u16x2, rw[q]) - c0v;
+                  dmin = __builtin_elementwise_min(dmin, d);
+                  const u16x2 r = __builtin_elementwise_min(__builtin_elementwise_sub_sat(lv, d), one);
+                  t |= __builtin_bit_cast(uint32_t, r) << (2 * q);
+                }
+                hm = (t & 0x55u) | ((t >> 15) & 0xaau);
+                const uint32_t dm = __builtin_bit_cast(uint32_t, dmin);
+                tmin = min(dm & 0xffffu, dm >> 16);
               }
               hm &= tvm[u];
               // the reference's binary_search(c0) starts at the last entry
